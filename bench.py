@@ -1,0 +1,206 @@
+"""Benchmark of the UMWM timestep hot path on one MI355X.
+
+Workload: the 720x360 global 0.5 deg lat-lon grid (GRF land 30 %, log-normal
+depth, GRF winds 0-25 m/s, JONSWAP start from the local wind), om = 37,
+pm = 36.  A "step" is one substep (source + advect + refract + stress) over
+the whole grid.  The chosen path (FP32) and the plain path (FP64) run side
+by side on the same inputs; after the timed steps both finish the 6 h run
+and the FP32 Hs / Tm / ustar fields are compared with the FP64 ones.
+
+Prints one JSON line.  --gpus N with N > 1 exits: the build targets one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from umwm_amd import configs, hip  # noqa: E402
+
+METRIC = "spectral-bin updates/s (source+advect+refract), chosen vs plain at Hs RMSE ≤2%"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_domain(wl, precision):
+    d = hip.Domain(wl.params, wl.grid_input(), wl.forcing_input(0), precision=precision)
+    d.init_jonswap(wl.jonswap)
+    d.stokes_drift(True)
+    d.forcinginput(wl.forcing_input(1))
+    return d
+
+
+def timed_steps(d, warmup, steps):
+    """Cold substep + warmup, then `steps` timed substeps.  Returns wall
+    seconds, per-kernel mean ms, and the substeps taken."""
+    d.substep()  # cold first substep (dts = 0), not counted
+    for _ in range(warmup):
+        d.substep()
+    d.synchronize()
+    d.set_timing(True)
+    d.timing(reset=True)
+    t0 = time.perf_counter()
+    sumt = 0.0
+    for _ in range(steps):
+        sumt, _ = d.substep()
+    d.synchronize()
+    wall = time.perf_counter() - t0
+    ms, cnt = d.timing(reset=True)
+    d.set_timing(False)
+    return wall, [m / max(c, 1) for m, c in zip(ms, cnt)], sumt
+
+
+def finish_run(d, wl, hours):
+    """Completes the current hour and the remaining hours of the run."""
+    done = 0
+    while d.scalar("sumt") < wl.params.dtg:
+        d.substep()
+    d.stokes_drift(False)
+    d.diag()
+    done = 1
+    for h in range(done, hours):
+        d.forcinginput(wl.forcing_input(h + 1))
+        d.run_dtg()
+
+
+def cpu_baseline(wl_full, seconds_budget=20.0):
+    """The FP64 oracle on a longitude slab of the same grid, all host cores
+    (capped at 16), timed over substeps after the cold one."""
+    try:
+        from oracle import oracle_py
+    except Exception as e:  # pragma: no cover
+        return {"value": None, "error": str(e)}
+    mm_s = 120
+    g = dict(wl_full.grid)
+    for key in ("lon", "lat", "z"):
+        g[key] = np.ascontiguousarray(g[key][:mm_s])
+    f = {k: np.ascontiguousarray(v[:mm_s]) for k, v in wl_full.forcing.items()}
+    from umwm_amd import abi
+    kw = abi.params_dict(wl_full.params)
+    depths = kw.pop("depths")
+    for k in ("lm",):
+        kw.pop(k)
+    kw.update(mm=mm_s, isglobal=0)
+    p = abi.make_params(depths=depths, **kw)
+    threads = min(os.cpu_count() or 1, 16)
+    o = oracle_py.Oracle(p, abi.make_grid_input(**g), abi.make_forcing_input(**f), threads=threads)
+    o.init_jonswap(wl_full.jonswap)
+    o.stokes_drift(True)
+    fi = abi.make_forcing_input(**f)
+    o.forcinginput(fi)
+    o.substep()  # cold
+    bins = o.im * o.om * o.pm
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.substep()
+        n += 1
+        if time.perf_counter() - t0 > seconds_budget or n >= 20:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n * bins / dt, "unit": "bin-updates/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, {n} substeps of a {mm_s}x{wl_full.params.nm} longitude "
+                      f"slab of the same grid ({o.im} sea points x {o.om}x{o.pm} bins), "
+                      f"{threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--hours", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--small", action="store_true", help="quarter-size grid for smoke runs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note "
+              "only (see DESIGN.md), so --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    mm, nm = (720, 360) if not args.small else (180, 90)
+    log(f"bench: building c2 {mm}x{nm}")
+    wl = configs.c2(seed=2, mm=mm, nm=nm, hours=args.hours)
+
+    res = {}
+    for prec, tag in ((8, "fp64"), (4, "fp32")):
+        log(f"bench: {tag} create")
+        d = make_domain(wl, prec)
+        bins = d.im * d.om * d.pm
+        log(f"bench: {tag} timed {args.steps} substeps of {bins} bins")
+        wall, kms, sumt = timed_steps(d, args.warmup, args.steps)
+        rate = args.steps * bins / wall
+        log(f"bench: {tag} {rate / 1e9:.2f} G bin-updates/s, kernels ms {kms}")
+        log(f"bench: {tag} finishing {args.hours} h")
+        t0 = time.perf_counter()
+        finish_run(d, wl, args.hours)
+        log(f"bench: {tag} run done in {time.perf_counter() - t0:.1f} s")
+        res[tag] = dict(rate=rate, wall=wall, kms=kms, bins=bins, im=d.im,
+                        hs=d.get("ht"), tm=d.get("mwp"), ust=d.get("ustar"),
+                        us=d.get("us")[0], vs=d.get("vs")[0], tile=None)
+        d.close()
+
+    r32, r64 = res["fp32"], res["fp64"]
+    hs_ref = r64["hs"]
+    hs_rmse = float(np.sqrt(np.mean((r32["hs"] - hs_ref) ** 2)) / np.sqrt(np.mean(hs_ref ** 2)))
+
+    def maxrel(a, b, floor):
+        return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+    errs = {
+        "hs_rmse_rel": hs_rmse,
+        "hs_max_rel": maxrel(r32["hs"], hs_ref, 1e-3),
+        "tm_max_rel": maxrel(r32["tm"], r64["tm"], 1e-3),
+        "ustar_max_rel": maxrel(r32["ust"], r64["ust"], 1e-3),
+        "stokes_surface_rmse_rel": float(
+            np.sqrt(np.mean((np.hypot(r32["us"], r32["vs"]) - np.hypot(r64["us"], r64["vs"])) ** 2))
+            / max(np.sqrt(np.mean(np.hypot(r64["us"], r64["vs"]) ** 2)), 1e-30)),
+    }
+
+    # roofline of the fused substep: B_alg = 3 words per bin-update
+    def roof(r, word):
+        t = sum(r["kms"]) * 1e-3
+        ach = 3 * word * r["bins"] / t / 1e9
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "fused substep (k_rates + k_integrate + k_advect)",
+                "kernel_ms": [round(x, 4) for x in r["kms"]]}
+
+    out = {
+        "metric": METRIC,
+        "value": r32["rate"],
+        "unit": "bin-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r32["wall"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "c2: 720x360 global 0.5 deg, om=37 pm=36, GRF land/depth/winds, "
+                               f"JONSWAP start, {args.hours} h" if not args.small else "c2 small",
+                   "sea_points": r32["im"], "bins_per_step": r32["bins"],
+                   "chosen": "fp32 fused", "plain": "fp64 fused"},
+        "plain": {"value": r64["rate"], "dtype": "f64", "ms_per_step": 1e3 * r64["wall"] / args.steps},
+        "chosen_vs_plain": r32["rate"] / r64["rate"],
+        "errors_vs_fp64": errs,
+        "roofline": roof(r32, 4),
+        "roofline_plain": roof(r64, 8),
+    }
+    if not args.no_cpu:
+        log("bench: cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

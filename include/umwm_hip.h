@@ -1,0 +1,268 @@
+/*
+ * umwm_hip.h -- C-ABI of the MI355X (gfx950) implementation of the UMWM
+ * timestep hot path.
+ *
+ * The reference operators are argument-less Fortran module subroutines that
+ * read and write module globals (src/umwm_module.F90:103-207).  Here every
+ * operator takes an opaque domain handle that owns the device-resident state
+ * of one model domain; a Fortran host binds each entry point with
+ * iso_c_binding (bind(C, name='umwm_hip_...')).  All entry points return 0 on
+ * success and a negative umwm_status code on failure; umwm_hip_error_string()
+ * and umwm_hip_last_error() describe the failure.  No entry point aborts the
+ * process: the caller keeps the reference's fatal convention
+ * (src/umwm_util.F90:129-153, raiseexception + stop).
+ *
+ * Array conventions at the boundary (what the get/set entry points exchange):
+ *   - host arrays are double precision and use the reference's column-major
+ *     shapes: spectra (om,pm,im), per-frequency tables (om,im), 2-D fields
+ *     (mm,nm), per-point fields (im) or (imm);
+ *   - index maps use the reference's 1-based values (ghost point = 0).
+ * On the device the spectrum is stored E[o][p][i] with the sea-point index
+ * contiguous; the transposition happens only inside get/set calls.
+ */
+#ifndef UMWM_HIP_H
+#define UMWM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes returned by every entry point. */
+typedef enum umwm_status {
+  UMWM_OK = 0,
+  UMWM_ERR_ARGUMENT = -1,   /* bad argument or namelist value             */
+  UMWM_ERR_HIP = -2,        /* HIP runtime error (message has details)    */
+  UMWM_ERR_NOMEM = -3,      /* device or host allocation failed           */
+  UMWM_ERR_STATE = -4,      /* call made in the wrong order               */
+  UMWM_ERR_UNKNOWN_FIELD = -5,
+  UMWM_ERR_SIZE = -6        /* buffer size does not match the field       */
+} umwm_status;
+
+/* Floating-point type of the spectrum and of the source/advection math. */
+enum { UMWM_FP32 = 4, UMWM_FP64 = 8 };
+
+/*
+ * Namelist contents (namelists/main.nml groups DOMAIN, PHYSICS, GRID,
+ * FORCING, FORCING_CONSTANT, OUTPUT, STOKES), read by the reference in
+ * src/umwm_init.F90:67-92 and src/umwm_stokes.F90:231-240.  Logical switches
+ * are int32 0/1.  Layout is plain C so a Fortran bind(C) derived type with
+ * the same member order maps onto it.
+ */
+typedef struct umwm_params {
+  /* &DOMAIN */
+  int32_t mm, nm, om, pm;
+  int32_t isglobal, restart;
+  /* &GRID switches */
+  int32_t gridfromfile, topofromfile, fillestuaries, filllakes;
+  /* &FORCING switches: field read from forcing snapshots when 1 */
+  int32_t winds, currents, air_density, water_density, seaice;
+  /* &OUTPUT */
+  int32_t stokes;
+  int32_t lm;          /* ignored on input: the library counts the levels
+                          as count(depths >= 0), src/umwm_stokes.F90:243 */
+  int32_t reserved0;
+  double fmin, fmax, fprog, dtg;
+  /* &PHYSICS */
+  double g, nu_air, nu_water, sfct, kappa, z, gustiness, dmin, explim;
+  double sin_fac, sin_diss1, sin_diss2, sds_fac, sds_power, mss_fac;
+  double snl_fac, sdt_fac, sbf_fac, sbp_fac;
+  /* &GRID values */
+  double delx, dely, dpt;
+  /* &FORCING_CONSTANT */
+  double wspd0, wdir0, uc0, vc0, rhoa0, rhow0, fice0, fice_lth, fice_uth;
+  /* &STOKES: depths [m], positive down, first lm entries used */
+  double depths[100];
+} umwm_params;
+
+/*
+ * Grid inputs that the reference reads from input/umwm.gridtopo
+ * (src/umwm_init.F90:397-552).  Each pointer may be NULL when the matching
+ * switch in umwm_params is 0.  2-D arrays are (mm,nm), m fastest.
+ */
+typedef struct umwm_grid_input {
+  const double* lon;        /* degrees east, used when gridfromfile        */
+  const double* lat;        /* degrees north, used when gridfromfile       */
+  const double* z;          /* bathymetry, negative below sea level; cells
+                               with z >= 0 are land (topofromfile)         */
+  const int32_t* exclude;   /* 2*nexclude 1-based (m,n) lake seeds         */
+  int32_t nexclude;         /* (namelists/exclude.nml, filllakes)          */
+  int32_t reserved0;
+} umwm_grid_input;
+
+/*
+ * One forcing snapshot, the content of input/umwmin_<time>.nc
+ * (src/umwm_io.F90:13-93).  A field is used only when its switch in
+ * umwm_params is set; NULL otherwise.  2-D arrays are (mm,nm).
+ */
+typedef struct umwm_forcing_input {
+  const double* uw;   const double* vw;     /* winds    */
+  const double* uc;   const double* vc;     /* currents */
+  const double* rhoa; const double* rhow;   /* densities */
+  const double* fice;                       /* sea ice fraction */
+} umwm_forcing_input;
+
+/* JONSWAP initial spectrum (no counterpart in the reference, which
+ * cold-starts from e = tiny at src/umwm_init.F90:1182).  See
+ * umwm_hip_init_jonswap. */
+typedef struct umwm_jonswap {
+  double alpha, gamma, sigma_a, sigma_b;
+  double fp;          /* peak frequency [Hz] when fp_from_wind == 0       */
+  int32_t fp_from_wind;   /* 1: fp = clamp(0.13 g / max(U,3), 0.05, 0.5)  */
+  int32_t dir_from_wind;  /* 1: mean direction = local wdir, else thw     */
+  double thw;         /* mean direction [rad, mathematical convention]   */
+} umwm_jonswap;
+
+typedef struct umwm_domain umwm_domain;   /* opaque, one model domain      */
+typedef struct umwm_batch umwm_batch;     /* opaque, many domains at once  */
+
+/* ---------------------------------------------------------------------- */
+/* Library                                                                 */
+/* ---------------------------------------------------------------------- */
+const char* umwm_hip_error_string(int status);
+/* Message of the most recent failure on the calling thread. */
+const char* umwm_hip_last_error(void);
+/* Number of visible HIP devices (0 when none). */
+int umwm_hip_device_count(int* count);
+
+/* ---------------------------------------------------------------------- */
+/* Setup: replaces umwm_initialize (src/umwm_top.F90:7-29)                 */
+/* ---------------------------------------------------------------------- */
+/*
+ * Builds grid, masks, the sea-first unrolled index maps and every
+ * coefficient table (src/umwm_init.F90 grid :381-568, masks :571-694,
+ * remap :875-1237, init :1240-1419, dispersion :1422-1658) on the host in
+ * double precision, then allocates the device state and uploads it.
+ * `forcing0` holds the snapshot at the start time (input_nc(starttime),
+ * src/umwm_io.F90:13-93) and may be NULL when no forcing switch is set.
+ * `precision` is UMWM_FP32 or UMWM_FP64.  The spectrum starts at
+ * e = tiny(e) as in the reference.
+ */
+int umwm_hip_create(const umwm_params* params, const umwm_grid_input* grid,
+                    const umwm_forcing_input* forcing0, int precision,
+                    umwm_domain** out);
+int umwm_hip_destroy(umwm_domain* dom);
+/* Dimensions: out[0]=im, out[1]=imm, out[2]=om, out[3]=pm, out[4]=mm,
+ * out[5]=nm, out[6]=lm, out[7]=precision. */
+int umwm_hip_dims(const umwm_domain* dom, int32_t out[8]);
+
+/* Replaces the cold start with a JONSWAP x cos^2 spectrum. */
+int umwm_hip_init_jonswap(umwm_domain* dom, const umwm_jonswap* js);
+
+/* stokes_drift('init') and stokes_drift() (src/umwm_stokes.F90:13-132). */
+int umwm_hip_stokes_drift(umwm_domain* dom, int init);
+
+/* ---------------------------------------------------------------------- */
+/* Reference operators, one entry point each.  Together they reproduce one */
+/* iteration of the inner loop of umwm_run (src/umwm_top.F90:89-150) with  */
+/* the reference's module arrays (ssin, sds, snl, ...) materialised.       */
+/* ---------------------------------------------------------------------- */
+/* forcinginterpolate (src/umwm_forcing.F90:40-99). */
+int umwm_hip_forcinginterpolate(umwm_domain* dom);
+/* forcinginput (src/umwm_forcing.F90:11-37): level n <- n+1, load n+1. */
+int umwm_hip_forcinginput(umwm_domain* dom, const umwm_forcing_input* next);
+/* sin_d12, sds_d12, snl_d12, s_ice (src/umwm_source_functions.F90:12). */
+int umwm_hip_sin_d12(umwm_domain* dom);
+int umwm_hip_sds_d12(umwm_domain* dom);
+int umwm_hip_snl_d12(umwm_domain* dom);
+int umwm_hip_s_ice(umwm_domain* dom);
+/* source (src/umwm_physics.F90:9-86): dts, integration, e <- (e+ef)/2. */
+int umwm_hip_source(umwm_domain* dom);
+/* propagation, refraction (src/umwm_advection.F90:18,102). */
+int umwm_hip_propagation(umwm_domain* dom);
+int umwm_hip_refraction(umwm_domain* dom);
+/* e = ef (src/umwm_top.F90:113). */
+int umwm_hip_update(umwm_domain* dom);
+/* stress(option), option "atm" or "ocn" (src/umwm_stress.F90:17). */
+int umwm_hip_stress(umwm_domain* dom, const char* option);
+/* diag (src/umwm_physics.F90:89-205). */
+int umwm_hip_diag(umwm_domain* dom);
+/* Marks the end of the first substep (first = .false.,
+ * src/umwm_top.F90:139). */
+int umwm_hip_end_first(umwm_domain* dom);
+
+/* ---------------------------------------------------------------------- */
+/* Fused path: same results as the operator sequence above               */
+/* ---------------------------------------------------------------------- */
+/*
+ * One substep: forcinginterpolate, sin_d12, sds_d12, snl_d12, s_ice,
+ * source, propagation, refraction, e = ef, stress('atm'), and on the first
+ * substep diag and first = .false. (src/umwm_top.F90:91-141), in three
+ * fused kernels.  Returns the new sumt and the dts taken.
+ */
+int umwm_hip_substep(umwm_domain* dom, double* sumt, double* dts);
+/*
+ * The inner loop to sumt >= dtg (src/umwm_top.F90:88-150) followed by the
+ * per-dtg stokes_drift (when stokes) and diag (:152-155).  `nsub` receives
+ * the number of substeps taken.  The first call of a run includes the
+ * cold (dts = 0) substep.
+ */
+int umwm_hip_run_dtg(umwm_domain* dom, int32_t* nsub);
+
+/* ---------------------------------------------------------------------- */
+/* State exchange                                                          */
+/* ---------------------------------------------------------------------- */
+/*
+ * Copies a named field to/from a host double array in the reference's
+ * shape (see top of file).  Spectra: e ef ssin sds snl dummy rotl rotr;
+ * (om,im) tables: k kdk k3dk k4 oneoverk4 cp0 cg0 invcp0 fkovg logl2overz
+ * cothkd sbf sdv sdt snl_arg bf1_renorm bf2_renorm dwn sice; per point:
+ * wspd wdir uc vc rhoa rhow rhorat fice ustar cd shelt fcutoff oc
+ * physics_time_step taux tauy taux_form tauy_form taux_skin tauy_skin
+ * taux_diag tauy_diag tailatmx tailatmy ht mwp mwd mss mwl momx momy
+ * cgmxx cgmxy cgmyy dwd dwl dwp dcp0 dcg0 dcp dcg dx dy d ar dxn dxs dye
+ * dyw; maps (returned as doubles holding the reference's integers): ii mi
+ * ni iw ie is in iiw iie iis iin mask; Stokes: us vs (im,lm), ds (im).
+ * `n` is the number of doubles in `buf` and must equal the field size.
+ */
+int umwm_hip_get_field(umwm_domain* dom, const char* name, double* buf,
+                       int64_t n);
+int umwm_hip_set_field(umwm_domain* dom, const char* name, const double* buf,
+                       int64_t n);
+/* Scalars: sumt dts dta dtr dtamin cfllim cgmax mindelx dlnf dth bf1 bf2
+ * first. */
+int umwm_hip_get_scalar(umwm_domain* dom, const char* name, double* v);
+int umwm_hip_set_scalar(umwm_domain* dom, const char* name, double v);
+/* Sizes in doubles of a named field (0 if unknown). */
+int64_t umwm_hip_field_size(const umwm_domain* dom, const char* name);
+
+/* ---------------------------------------------------------------------- */
+/* Timing of the fused kernels (HIP events on the domain's stream)        */
+/* ---------------------------------------------------------------------- */
+/* Enables per-kernel event timing of the fused path (off by default). */
+int umwm_hip_set_timing(umwm_domain* dom, int enable);
+/* Accumulated milliseconds and launch counts of the three fused kernels
+ * since the last reset: ms[0..2] = rates, integrate, advect+stress;
+ * counts[0..2] likewise.  Resets the accumulators when reset != 0. */
+int umwm_hip_get_timing(umwm_domain* dom, double ms[3], int64_t counts[3],
+                        int reset);
+/* Waits for all work queued on the domain's stream. */
+int umwm_hip_synchronize(umwm_domain* dom);
+/* Overrides the fused kernels' tile shape: D directions per thread (3, 6,
+ * 9, 12 or 18) and W waves per workgroup (1..8, D*W >= pm).  Used by the
+ * autotuner (umwm_amd/autotune.py). */
+int umwm_hip_set_tile(umwm_domain* dom, int32_t D, int32_t W);
+
+/* ---------------------------------------------------------------------- */
+/* Batched ensembles: many independent domains of uneven sizes advance in */
+/* one launch per kernel per substep, each with its own dts.  No         */
+/* counterpart in the reference (one domain per process).                 */
+/* ---------------------------------------------------------------------- */
+/* All domains must share the precision; they are borrowed, not owned. */
+int umwm_hip_batch_create(umwm_domain* const* doms, int32_t n,
+                          umwm_batch** out);
+int umwm_hip_batch_destroy(umwm_batch* b);
+/* Inner loop of every member to its own sumt >= dtg, replaying one
+ * hipGraph per substep when use_graph != 0.  nsub[k] receives member k's
+ * substep count (array of n, may be NULL). */
+int umwm_hip_batch_run_dtg(umwm_batch* b, int use_graph, int32_t* nsub);
+/* Tile shape chosen by the autotuner for the batch (directions per thread
+ * and waves per block): out[0]=D, out[1]=W. */
+int umwm_hip_batch_tile(const umwm_batch* b, int32_t out[2]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UMWM_HIP_H */

@@ -1,0 +1,1504 @@
+/*
+ * umwm_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 CPU restatement of the UMWM timestep hot path.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, always as
+ * the checker; the product library never links or calls it.
+ *
+ * Every function cites the reference routine it restates.  Arithmetic
+ * follows the reference's expression order (Fortran evaluates a*b*c as
+ * (a*b)*c) and its real-exponent powers (x**4. -> pow); build it with
+ * -ffp-contract=off and without -ffast-math so that no operation is fused
+ * or reassociated.  The reference itself cannot be compiled here: its
+ * modules use NetCDF-Fortran and datetime-fortran, neither of which exists
+ * in this image.
+ *
+ * Pinning status of each part is recorded in tests/test_oracle_pins.py.
+ */
+#include "umwm_oracle.h"
+
+#include <float.h>
+#include <stddef.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define TINY DBL_MIN /* Fortran tiny(1.0d0) */
+
+struct oracle_model {
+  umwm_params P;
+  int mm, nm, om, pm, im, imm, lm;
+
+  /* 2-D (mm,nm) */
+  double *lon, *lat, *dx_2d, *dy_2d, *curv, *d_2d, *ar_2d;
+  int *mask, *ii;
+
+  /* unrolled over all points, index 1..imm (element 0 unused) */
+  int *mi, *ni, *iw, *ie, *is_, *in_, *iiw, *iie, *iis, *iin;
+  double *dx, *dy, *d;
+  double *wspd, *wdir, *uc, *vc, *rhoa, *rhow, *rhorat, *fice;
+
+  /* forcing levels, 2-D */
+  double *uwb, *uwf, *vwb, *vwf, *ucb, *ucf, *vcb, *vcf;
+  double *rhoa_2d, *rhow_2d, *fice_2d, *ficeb, *ficef;
+  double *rhoab, *rhoaf, *rhowb, *rhowf; /* unrolled, 1..imm */
+
+  /* per sea point, 1..im */
+  double *ar, *oneovar, *oneovdx, *oneovdy, *dxn, *dxs, *dye, *dyw;
+  double *cd, *ustar, *shelt, *fcutoff, *pts;
+  int *oc;
+  double *taux, *tauy, *taux_form, *tauy_form, *taux_skin, *tauy_skin;
+  double *taux_diag, *tauy_diag, *tailatmx, *tailatmy;
+  double *ht, *mwp, *mwd, *mss, *mwl, *momx, *momy, *cgmxx, *cgmxy, *cgmyy;
+  double *dwd, *dwl, *dwp, *dcp0, *dcg0, *dcp, *dcg;
+
+  /* (om, 0..im) tables */
+  double *k, *kdk, *k3dk, *k4, *oneoverk4, *fkovg, *logl2overz, *cothkd;
+  double *invcp0, *sbf, *sdv, *sdt, *snl_arg, *bf1r, *bf2r, *dwn, *l2;
+  double *sice, *cp0, *cg0;
+
+  /* (om, pm, 0..im) spectra */
+  double *e, *ef, *ssin, *sds, *snl, *dummy, *rotl, *rotr;
+
+  /* spectral constants */
+  double *f, *th, *cth, *sth, *dom, *cth2, *cth2pp;
+  int *pl, *pr;
+  double *cth_curv, *sth_curv; /* (pm, 0..im) */
+
+  /* Stokes */
+  double *depth, *us, *vs, *ds, *util; /* util (om, im, lm) */
+
+  /* scalars */
+  double dlnf, dth, dthg, oneovdth, log10overz, twopisds_fac, twonu;
+  double fieldscale1, fieldscale2, inv_sds_power, cfllim, bf1, bf2;
+  double mindelx, cgmax, dtamin;
+  double dts, dta, dtr, dtr_temp, sumt;
+  int first, firstdtg, zerocurrents;
+};
+
+/* ------------------------------------------------------------------ */
+/* indexing                                                           */
+/* ------------------------------------------------------------------ */
+#define SP(a, o, p, i) ((a)[((size_t)(i) * m->pm + (p) - 1) * m->om + (o) - 1])
+#define TB(a, o, i) ((a)[(size_t)(i) * m->om + (o) - 1])
+#define M2(a, mm_, nn_) ((a)[((size_t)(nn_) - 1) * m->mm + (mm_) - 1])
+#define PI_(a, p, i) ((a)[(size_t)(i) * m->pm + (p) - 1])
+
+static int g_threads = 1;
+void oracle_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
+static void* xcalloc(size_t n, size_t s) {
+  void* p = calloc(n ? n : 1, s);
+  if (!p) {
+    fprintf(stderr, "oracle: out of memory\n");
+    abort();
+  }
+  return p;
+}
+#define ALLOCD(n) ((double*)xcalloc((n), sizeof(double)))
+#define ALLOCI(n) ((int*)xcalloc((n), sizeof(int)))
+
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_util.F90:211-232 distance_haversine                        */
+/* ------------------------------------------------------------------ */
+double oracle_distance_haversine(double lon1, double lon2, double lat1,
+                                 double lat2) {
+  double dlon = fabs(lon2 - lon1);
+  double dlat = fabs(lat2 - lat1);
+  double s1 = sin(0.5 * dlat), s2 = sin(0.5 * dlon);
+  return 2 * asin(sqrt(s1 * s1 + cos(lat1) * cos(lat2) * (s2 * s2)));
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_sheltering.F90:49-79 sheltering_coare35                    */
+/* ------------------------------------------------------------------ */
+double oracle_sheltering_coare35(double wspd) {
+  const double x1 = 15, x2 = 33, x3 = 60;
+  const double y1 = 0.10, y2 = 0.09, y3 = 0.06;
+  const double intercept = 0.04, curvature = 0.65, decay = 1.6;
+  double slope1 = (y1 - intercept) / x1;
+  double slope2 = (y3 - y2) / (x3 - x2);
+  double c = curvature * (slope2 - slope1) / (x2 - x1);
+  double b = slope1 - 2 * c * x1;
+  double a = y1 - b * x1 - c * (x1 * x1);
+  double s1 = a + b * x2 + c * (x2 * x2);
+  if (wspd <= x1) return intercept + slope1 * wspd;
+  if (wspd > x1 && wspd <= x2) return a + b * wspd + c * (wspd * wspd);
+  return s1 * exp(-(wspd - x2) / (decay * wspd));
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_stress.F90:268-311 drag and tail helpers                   */
+/* ------------------------------------------------------------------ */
+static double drag_coefficient(double taux, double tauy, double rhoa,
+                               double wspd) {
+  return sqrt(taux * taux + tauy * tauy) / (rhoa * (wspd * wspd));
+}
+
+double oracle_friction_velocity_skin(double wspd, double z, double nu,
+                                     double kappa) {
+  /* src/umwm_stress.F90:285-298: z0 starts at 1e-3, exactly 6 passes */
+  double z0 = 1e-3, ust = 0;
+  for (int n = 1; n <= 6; ++n) {
+    ust = kappa * wspd / log(z / z0);
+    z0 = 0.132 * nu / ust;
+  }
+  return ust;
+}
+
+double oracle_drag_coefficient_skin(double cd_form, double wspd,
+                                    double wspdrel, double z, double nu,
+                                    double kappa) {
+  /* src/umwm_stress.F90:275-283 */
+  double u = oracle_friction_velocity_skin(wspdrel, z, nu, kappa);
+  double cd = (u * u) / (wspd * wspd);
+  cd = cd * (1 + 2 * cd / (cd + cd_form + TINY)) / 3;
+  if (cd > 1e-2) cd = 1e-2;
+  return cd;
+}
+
+double oracle_stress_tail(double wspd, double kmax) {
+  /* src/umwm_stress.F90:300-311 */
+  const double kcap = 1e3, a = 0.000112, b = -0.01451, c = -1.0186;
+  double tail = a * (wspd * wspd) + b * wspd + c;
+  double kpt = pow(kmax, tail);
+  return (pow(kcap, tail + 1) - kmax * kpt) / (kpt * (tail + 1));
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_util.F90:46-64 remap_mn2i                                  */
+/* ------------------------------------------------------------------ */
+static void remap_mn2i(oracle_model* m, const double* fmn, double* fi) {
+  for (int n = 1; n <= m->nm; ++n)
+    for (int mm_ = 1; mm_ <= m->mm; ++mm_) fi[M2(m->ii, mm_, n)] = M2(fmn, mm_, n);
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_init.F90:381-568 grid                                      */
+/* ------------------------------------------------------------------ */
+static void grid(oracle_model* m, const umwm_grid_input* g) {
+  const umwm_params* P = &m->P;
+  const int mm = m->mm, nm = m->nm;
+  const double pi = 3.141592653589793, twopi = 2 * pi;
+  const double r_earth = 6.371009e6;
+  if (P->gridfromfile) {
+    double* rlon = ALLOCD((size_t)mm * nm);
+    double* rlat = ALLOCD((size_t)mm * nm);
+    for (size_t q = 0; q < (size_t)mm * nm; ++q) {
+      m->lon[q] = g->lon[q];
+      m->lat[q] = g->lat[q];
+      rlon[q] = m->lon[q] * twopi / 360.;
+      rlat[q] = m->lat[q] * twopi / 360.;
+    }
+    for (int n = 1; n <= nm; ++n) {
+      for (int i = 2; i <= mm - 1; ++i)
+        M2(m->dx_2d, i, n) =
+            r_earth * oracle_distance_haversine(
+                          0.5 * (M2(rlon, i - 1, n) + M2(rlon, i, n)),
+                          0.5 * (M2(rlon, i, n) + M2(rlon, i + 1, n)),
+                          0.5 * (M2(rlat, i - 1, n) + M2(rlat, i, n)),
+                          0.5 * (M2(rlat, i, n) + M2(rlat, i + 1, n)));
+      M2(m->dx_2d, 1, n) = 2 * M2(m->dx_2d, 2, n) - M2(m->dx_2d, 3, n);
+      M2(m->dx_2d, mm, n) = 2 * M2(m->dx_2d, mm - 1, n) - M2(m->dx_2d, mm - 2, n);
+    }
+    for (int n = 2; n <= nm - 1; ++n)
+      for (int i = 1; i <= mm; ++i)
+        M2(m->dy_2d, i, n) =
+            r_earth * oracle_distance_haversine(
+                          0.5 * (M2(rlon, i, n - 1) + M2(rlon, i, n)),
+                          0.5 * (M2(rlon, i, n) + M2(rlon, i, n + 1)),
+                          0.5 * (M2(rlat, i, n - 1) + M2(rlat, i, n)),
+                          0.5 * (M2(rlat, i, n) + M2(rlat, i, n + 1)));
+    for (int i = 1; i <= mm; ++i) {
+      M2(m->dy_2d, i, 1) = 2 * M2(m->dy_2d, i, 2) - M2(m->dy_2d, i, 3);
+      M2(m->dy_2d, i, nm) = 2 * M2(m->dy_2d, i, nm - 1) - M2(m->dy_2d, i, nm - 2);
+    }
+    /* great-circle grid rotation, src/umwm_init.F90:489-513 */
+    for (int n = 1; n <= nm; ++n) {
+      for (int i = 1; i <= mm; ++i) M2(m->curv, i, n) = 0;
+      for (int i = 2; i <= mm - 1; ++i) {
+        double lo1 = M2(rlon, i - 1, n), lo2 = M2(rlon, i + 1, n);
+        double la1 = M2(rlat, i - 1, n), la2 = M2(rlat, i + 1, n);
+        M2(m->curv, i, n) =
+            atan2(sin(lo2 - lo1) * cos(la2),
+                  cos(la1) * sin(la2) - sin(la1) * cos(la2) * cos(lo2 - lo1));
+      }
+      if (P->isglobal) {
+        double lo1 = M2(rlon, mm, n), lo2 = M2(rlon, 2, n);
+        double la1 = M2(rlat, mm, n), la2 = M2(rlat, 2, n);
+        M2(m->curv, 1, n) =
+            atan2(sin(lo2 - lo1) * cos(la2),
+                  cos(la1) * sin(la2) - sin(la1) * cos(la2) * cos(lo2 - lo1));
+        lo1 = M2(rlon, mm - 1, n);
+        lo2 = M2(rlon, 1, n);
+        la1 = M2(rlat, mm - 1, n);
+        la2 = M2(rlat, 1, n);
+        M2(m->curv, mm, n) =
+            atan2(sin(lo2 - lo1) * cos(la2),
+                  cos(la1) * sin(la2) - sin(la1) * cos(la2) * cos(lo2 - lo1));
+      } else {
+        M2(m->curv, 1, n) = M2(m->curv, 2, n);
+        M2(m->curv, mm, n) = M2(m->curv, mm - 1, n);
+      }
+      for (int i = 1; i <= mm; ++i) M2(m->curv, i, n) = M2(m->curv, i, n) - 0.5 * pi;
+    }
+    free(rlon);
+    free(rlat);
+  } else {
+    for (size_t q = 0; q < (size_t)mm * nm; ++q) {
+      m->dx_2d[q] = P->delx;
+      m->dy_2d[q] = P->dely;
+      m->curv[q] = 0;
+      m->lon[q] = 0;
+      m->lat[q] = 0;
+    }
+  }
+  for (size_t q = 0; q < (size_t)mm * nm; ++q) {
+    m->d_2d[q] = P->topofromfile ? g->z[q] : P->dpt;
+    m->ar_2d[q] = m->dx_2d[q] * m->dy_2d[q];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_init.F90:571-718 masks and fill                            */
+/* ------------------------------------------------------------------ */
+static void fill_lake(oracle_model* m, int m0, int n0) {
+  /* src/umwm_init.F90:698-718, recursion replaced by an explicit stack;
+   * the set of cells zeroed is the same. */
+  size_t cap = 1024, top = 0;
+  int* st = (int*)xcalloc(2 * cap, sizeof(int));
+  st[0] = m0;
+  st[1] = n0;
+  top = 1;
+  while (top) {
+    --top;
+    int a = st[2 * top], b = st[2 * top + 1];
+    if (a == 1 || a == m->mm || b == 1 || b == m->nm) continue;
+    if (M2(m->mask, a, b) == 0 && !(a == m0 && b == n0)) continue;
+    M2(m->mask, a, b) = 0;
+    int nb[4][2] = {{a - 1, b}, {a + 1, b}, {a, b - 1}, {a, b + 1}};
+    for (int q = 3; q >= 0; --q) {
+      if (M2(m->mask, nb[q][0], nb[q][1]) != 1) continue;
+      if (top + 1 >= cap) {
+        cap *= 2;
+        st = (int*)realloc(st, 2 * cap * sizeof(int));
+      }
+      st[2 * top] = nb[q][0];
+      st[2 * top + 1] = nb[q][1];
+      ++top;
+    }
+  }
+  free(st);
+}
+
+static void masks(oracle_model* m, const umwm_grid_input* g) {
+  const umwm_params* P = &m->P;
+  const int mm = m->mm, nm = m->nm;
+  for (int n = 1; n <= nm; ++n)
+    for (int i = 1; i <= mm; ++i) M2(m->mask, i, n) = 1;
+  for (int i = 1; i <= mm; ++i) {
+    M2(m->mask, i, 1) = 0;
+    M2(m->mask, i, nm) = 0;
+  }
+  if (!P->isglobal)
+    for (int n = 1; n <= nm; ++n) {
+      M2(m->mask, 1, n) = 0;
+      M2(m->mask, mm, n) = 0;
+    }
+  if (P->topofromfile) {
+    for (size_t q = 0; q < (size_t)mm * nm; ++q) {
+      if (m->d_2d[q] >= 0) {
+        m->mask[q] = 0;
+        m->d_2d[q] = P->dmin;
+      }
+      m->d_2d[q] = fabs(m->d_2d[q]);
+      if (m->d_2d[q] < P->dmin) m->d_2d[q] = P->dmin;
+    }
+    if (P->fillestuaries) {
+      int iterate = 1;
+      while (iterate) {
+        iterate = 0;
+        for (int n = 2; n <= nm - 1; ++n)
+          for (int i = 2; i <= mm - 1; ++i) {
+            if (M2(m->mask, i, n) != 1) continue;
+            int cnt = 0;
+            if (M2(m->mask, i - 1, n) == 1) cnt++;
+            if (M2(m->mask, i + 1, n) == 1) cnt++;
+            if (M2(m->mask, i, n - 1) == 1) cnt++;
+            if (M2(m->mask, i, n + 1) == 1) cnt++;
+            if (cnt <= 1) {
+              M2(m->mask, i, n) = 0;
+              iterate = 1;
+            }
+          }
+      }
+    }
+    if (P->filllakes && g && g->exclude)
+      for (int q = 0; q < g->nexclude; ++q)
+        fill_lake(m, g->exclude[2 * q], g->exclude[2 * q + 1]);
+  } else {
+    for (size_t q = 0; q < (size_t)mm * nm; ++q)
+      if (m->mask[q] == 0) m->d_2d[q] = P->dmin;
+  }
+  int im = 0;
+  for (size_t q = 0; q < (size_t)mm * nm; ++q) im += m->mask[q] == 1;
+  m->im = im;
+  m->imm = mm * nm;
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_init.F90:875-1237 remap (serial, remap_dir = 'v')          */
+/* ------------------------------------------------------------------ */
+static void remap(oracle_model* m) {
+  const int mm = m->mm, nm = m->nm, imm = m->imm, im = m->im;
+  int i = 0;
+  for (int pass = 1; pass >= 0; --pass) /* sea points first, then land */
+    for (int a = 1; a <= mm; ++a)
+      for (int b = 1; b <= nm; ++b)
+        if (M2(m->mask, a, b) == pass) {
+          ++i;
+          M2(m->ii, a, b) = i;
+          m->dx[i] = M2(m->dx_2d, a, b);
+          m->dy[i] = M2(m->dy_2d, a, b);
+          m->d[i] = M2(m->d_2d, a, b);
+        }
+  for (int b = 1; b <= nm; ++b)
+    for (int a = 1; a <= mm; ++a) {
+      i = M2(m->ii, a, b);
+      m->mi[i] = a;
+      m->ni[i] = b;
+    }
+  for (int q = 0; q <= imm; ++q) m->is_[q] = m->in_[q] = m->iw[q] = m->ie[q] = 0;
+  for (int b = 2; b <= nm - 1; ++b)
+    for (int a = 2; a <= mm - 1; ++a) {
+      i = M2(m->ii, a, b);
+      m->is_[i] = M2(m->ii, a, b - 1);
+      m->in_[i] = M2(m->ii, a, b + 1);
+      m->iw[i] = M2(m->ii, a - 1, b);
+      m->ie[i] = M2(m->ii, a + 1, b);
+    }
+  if (m->P.isglobal)
+    for (int b = 2; b <= nm - 1; ++b) {
+      i = M2(m->ii, 1, b);
+      m->is_[i] = M2(m->ii, 1, b - 1);
+      m->in_[i] = M2(m->ii, 1, b + 1);
+      m->iw[i] = M2(m->ii, mm, b);
+      m->ie[i] = M2(m->ii, 2, b);
+      i = M2(m->ii, mm, b);
+      m->is_[i] = M2(m->ii, mm, b - 1);
+      m->in_[i] = M2(m->ii, mm, b + 1);
+      m->iw[i] = M2(m->ii, mm - 1, b);
+      m->ie[i] = M2(m->ii, 1, b);
+    }
+  for (int q = 0; q <= imm; ++q) {
+    m->iis[q] = m->is_[q];
+    m->iin[q] = m->in_[q];
+    m->iiw[q] = m->iw[q];
+    m->iie[q] = m->ie[q];
+  }
+  for (i = 1; i <= im; ++i) {
+    m->dxn[i] = 0.5 * (m->dx[i] + m->dx[m->iin[i]]);
+    m->dxs[i] = 0.5 * (m->dx[i] + m->dx[m->iis[i]]);
+    m->dye[i] = 0.5 * (m->dy[i] + m->dy[m->iie[i]]);
+    m->dyw[i] = 0.5 * (m->dy[i] + m->dy[m->iiw[i]]);
+  }
+  /* e = tiny(e) everywhere, ghost included (src/umwm_init.F90:1182) */
+  size_t ne = (size_t)m->om * m->pm * (im + 1);
+  for (size_t q = 0; q < ne; ++q) m->e[q] = TINY;
+  /* aliasing of off-tile neighbours to the ghost (:1195-1200) */
+  for (i = 1; i <= im; ++i) {
+    if (m->is_[i] < 1 || m->is_[i] > im) m->is_[i] = 0;
+    if (m->in_[i] < 1 || m->in_[i] > im) m->in_[i] = 0;
+    if (m->ie[i] < 1 || m->ie[i] > im) m->ie[i] = 0;
+    if (m->iw[i] < 1 || m->iw[i] > im) m->iw[i] = 0;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_io.F90:13-93 input_nc (constant-forcing semantics kept)    */
+/* ------------------------------------------------------------------ */
+static void input_nc(oracle_model* m, const umwm_forcing_input* f) {
+  const umwm_params* P = &m->P;
+  const size_t n2 = (size_t)m->mm * m->nm;
+  const int imm = m->imm;
+  if (P->winds) {
+    memcpy(m->uwf, f->uw, n2 * sizeof(double));
+    memcpy(m->vwf, f->vw, n2 * sizeof(double));
+  } else {
+    for (int i = 1; i <= imm; ++i) {
+      m->wspd[i] = P->wspd0;
+      m->wdir[i] = P->wdir0;
+    }
+  }
+  if (P->currents) {
+    memcpy(m->ucf, f->uc, n2 * sizeof(double));
+    memcpy(m->vcf, f->vc, n2 * sizeof(double));
+  } else {
+    for (size_t q = 0; q < n2; ++q) {
+      m->ucf[q] = P->uc0;
+      m->vcf[q] = P->vc0;
+    }
+    for (int i = 1; i <= imm; ++i) {
+      m->uc[i] = P->uc0;
+      m->vc[i] = P->vc0;
+    }
+  }
+  if (P->seaice) {
+    memcpy(m->ficef, f->fice, n2 * sizeof(double));
+  } else {
+    for (size_t q = 0; q < n2; ++q) m->fice_2d[q] = P->fice0;
+    for (int i = 1; i <= imm; ++i) m->fice[i] = P->fice0;
+  }
+  for (size_t q = 0; q < n2; ++q)
+    if (m->mask[q] == 0) m->ucf[q] = m->vcf[q] = 0;
+  if (P->air_density) {
+    memcpy(m->rhoa_2d, f->rhoa, n2 * sizeof(double));
+  } else {
+    for (size_t q = 0; q < n2; ++q) m->rhoa_2d[q] = P->rhoa0;
+    for (int i = 1; i <= imm; ++i) m->rhoa[i] = P->rhoa0;
+  }
+  if (P->water_density) {
+    memcpy(m->rhow_2d, f->rhow, n2 * sizeof(double));
+  } else {
+    for (size_t q = 0; q < n2; ++q) m->rhow_2d[q] = P->rhow0;
+    for (int i = 1; i <= imm; ++i) m->rhow[i] = P->rhow0;
+  }
+  remap_mn2i(m, m->rhoa_2d, m->rhoaf);
+  remap_mn2i(m, m->rhow_2d, m->rhowf);
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_init.F90:1422-1658 dispersion                              */
+/* ------------------------------------------------------------------ */
+static void dispersion(oracle_model* m, double tol) {
+  const umwm_params* P = &m->P;
+  const int om = m->om, im = m->im;
+  const double twopi = 2 * 3.141592653589793, g = P->g;
+  double* f_nd = ALLOCD((size_t)om * (im + 1));
+  double* kd = ALLOCD((size_t)om * (im + 1));
+  for (int i = 1; i <= im; ++i)
+    for (int o = 1; o <= om; ++o) {
+      TB(m->cp0, o, i) = twopi * sqrt(m->d[i] / g);
+      TB(f_nd, o, i) = TB(m->cp0, o, i) * m->f[o - 1];
+      TB(m->k, o, i) = TB(f_nd, o, i) * TB(f_nd, o, i);
+    }
+  for (int i = 1; i <= im; ++i) {
+    double b = P->sfct / (P->rhow0 * g * (m->d[i] * m->d[i]));
+    for (int o = 1; o <= om; ++o) {
+      int counter = 1;
+      double dk = 2. * tol, kk = TB(m->k, o, i), fn = TB(f_nd, o, i);
+      while (fabs(dk) > tol) {
+        double t = tanh(kk);
+        dk = -(fn * fn - kk * t * (1. + b * kk * kk)) /
+             (3. * b * kk * kk * t + t + kk * (1. + b * kk * kk) * (1. - t * t));
+        kk = kk - dk;
+        if (counter == 1000) break;
+        counter = counter + 1;
+      }
+      TB(m->k, o, i) = fabs(kk) / m->d[i];
+    }
+  }
+  for (int i = 1; i <= im; ++i)
+    for (int o = 1; o <= om; ++o) {
+      double v = TB(m->k, o, i) * m->d[i];
+      TB(kd, o, i) = v > 20. ? 20. : v;
+    }
+  for (size_t q = 0; q < (size_t)om * (im + 1); ++q) m->cp0[q] = m->cg0[q] = TINY;
+  for (int i = 1; i <= im; ++i)
+    for (int o = 1; o <= om; ++o) {
+      double kk = TB(m->k, o, i);
+      double cp = twopi * m->f[o - 1] / kk;
+      TB(m->cp0, o, i) = cp;
+      TB(m->cg0, o, i) =
+          cp * (0.5 + kk * m->d[i] / sinh(2. * TB(kd, o, i)) +
+                P->sfct * kk * kk / (P->rhow0 * g + P->sfct * kk * kk));
+    }
+  for (int i = 1; i <= im; ++i)
+    for (int o = 1; o <= om; ++o) {
+      double kk = TB(m->k, o, i), kdv = TB(kd, o, i), fo = m->f[o - 1];
+      TB(m->dwn, o, i) = m->dom[o - 1] / fabs(TB(m->cg0, o, i));
+      TB(m->l2, o, i) = 0.5 * fabs(TB(m->cp0, o, i)) / fo;
+      TB(m->k4, o, i) = pow(kk, 4.);
+      TB(m->oneoverk4, o, i) = 1. / TB(m->k4, o, i);
+      TB(m->kdk, o, i) = kk * TB(m->dwn, o, i);
+      TB(m->k3dk, o, i) = pow(kk, 3.) * TB(m->dwn, o, i);
+      TB(m->fkovg, o, i) = fo * kk / g;
+      TB(m->cothkd, o, i) = cosh(0.2 * kdv) / sinh(0.2 * kdv);
+      TB(m->invcp0, o, i) = 1. / TB(m->cp0, o, i);
+      TB(m->sbf, o, i) = P->sbf_fac * kk / (sinh(2. * kdv)) +
+                         P->sbp_fac * kk / (cosh(kdv) * cosh(kdv));
+      TB(m->sdv, o, i) = 4. * P->nu_water * pow(kk, 2.);
+    }
+  for (int i = 1; i <= im; ++i)
+    for (int o = 1; o <= om; ++o) {
+      TB(m->bf1r, o, i) = 0.;
+      TB(m->bf2r, o, i) = 0.;
+      TB(m->snl_arg, o, i) = 0.;
+    }
+  for (int i = 1; i <= im; ++i)
+    for (int o = 1; o <= om - 2; ++o) {
+      TB(m->bf1r, o, i) = P->snl_fac * m->bf1 * TB(m->kdk, o + 1, i) / TB(m->kdk, o, i);
+      TB(m->bf2r, o, i) = P->snl_fac * m->bf2 * TB(m->kdk, o + 2, i) / TB(m->kdk, o, i);
+      TB(m->snl_arg, o, i) = 1. - (TB(m->bf1r, o, i) + TB(m->bf2r, o, i));
+    }
+  for (int i = 1; i <= im; ++i)
+    for (int o = 1; o <= om; ++o) {
+      TB(m->logl2overz, o, i) = log(TB(m->l2, o, i) / P->z);
+      if (TB(m->l2, o, i) > 20.) TB(m->logl2overz, o, i) = log(20. / P->z);
+    }
+  /* ghost cp0/cg0 = per-frequency minimum over sea points (:1652-1656) */
+  for (int o = 1; o <= om; ++o) {
+    double a = INFINITY, b = INFINITY;
+    for (int i = 1; i <= im; ++i) {
+      a = dmin(a, TB(m->cp0, o, i));
+      b = dmin(b, TB(m->cg0, o, i));
+    }
+    TB(m->cp0, o, 0) = a;
+    TB(m->cg0, o, 0) = b;
+  }
+  free(f_nd);
+  free(kd);
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_init.F90:1240-1419 init                                    */
+/* ------------------------------------------------------------------ */
+static void init(oracle_model* m) {
+  const umwm_params* P = &m->P;
+  const int om = m->om, pm = m->pm, im = m->im;
+  const double pi = 3.141592653589793, twopi = 2 * pi;
+  m->dlnf = (log(P->fmax) - log(P->fmin)) / (double)(om - 1);
+  for (int o = 1; o <= om; ++o) m->f[o - 1] = exp(log(P->fmin) + (o - 1) * m->dlnf);
+  m->dth = twopi / (double)pm;
+  m->dthg = m->dth * P->g;
+  m->oneovdth = 1. / m->dth;
+  m->log10overz = log(10. / P->z);
+  m->twopisds_fac = twopi * P->sds_fac;
+  m->twonu = 2. * P->nu_water;
+  m->fieldscale1 = P->sin_diss1 / P->sin_fac;
+  m->fieldscale2 = P->sin_diss2 / P->sin_diss1;
+  m->inv_sds_power = 1. / P->sds_power;
+  if (pm % 8 == 0)
+    m->cfllim = cos(0.25 * pi - 0.5 * m->dth);
+  else
+    m->cfllim = 1. / sqrt(2.);
+  for (int i = 1; i <= im; ++i) {
+    m->ar[i] = m->dx[i] * m->dy[i];
+    m->oneovdx[i] = 1. / m->dx[i];
+    m->oneovdy[i] = 1. / m->dy[i];
+    m->oneovar[i] = 1. / m->ar[i];
+  }
+  double bf1 = exp(-16 * m->dlnf * m->dlnf);
+  double bf2 = exp(-64 * m->dlnf * m->dlnf);
+  double bf1a = bf1 / (bf1 + bf2);
+  bf2 = bf2 / (bf1 + bf2);
+  m->bf1 = bf1a;
+  m->bf2 = bf2;
+  for (int p = 1; p <= pm; ++p) {
+    m->th[p - 1] = (p - 0.5 * (pm + 1)) * m->dth;
+    m->cth[p - 1] = cos(m->th[p - 1]);
+    m->sth[p - 1] = sin(m->th[p - 1]);
+  }
+  for (int i = 1; i <= im; ++i)
+    for (int p = 1; p <= pm; ++p) {
+      double cv = M2(m->curv, m->mi[i], m->ni[i]);
+      PI_(m->cth_curv, p, i) = cos(m->th[p - 1] + cv);
+      PI_(m->sth_curv, p, i) = sin(m->th[p - 1] + cv);
+    }
+  for (int p = 1; p <= pm; ++p) {
+    m->pl[p - 1] = p + 1;
+    m->pr[p - 1] = p - 1;
+  }
+  m->pl[pm - 1] = 1;
+  m->pr[0] = pm;
+  for (int o = 1; o <= om; ++o) m->dom[o - 1] = twopi * m->dlnf * m->f[o - 1];
+  for (int p = 1; p <= pm; ++p) {
+    double c = cos(m->dth * (p - 1));
+    m->cth2[p - 1] = c * c;
+  }
+  for (int p = 1; p <= pm; ++p)
+    for (int pp = 1; pp <= pm; ++pp) {
+      int ind = pp - p + 1;
+      if (ind <= 0) ind = pm + ind;
+      m->cth2pp[(size_t)(p - 1) * pm + pp - 1] = m->cth2[ind - 1] * m->dth;
+    }
+  dispersion(m, 1e-2);
+  double mind = INFINITY;
+  for (size_t q = 0; q < (size_t)m->mm * m->nm; ++q)
+    if (m->mask[q] == 1) {
+      mind = dmin(mind, m->dx_2d[q]);
+      mind = dmin(mind, m->dy_2d[q]);
+    }
+  m->mindelx = mind;
+  double cgmax = -INFINITY;
+  for (int i = 1; i <= im; ++i)
+    for (int o = 1; o <= om; ++o) cgmax = dmax(cgmax, TB(m->cg0, o, i));
+  m->cgmax = cgmax;
+  m->dtamin = 0.98 * m->cfllim * m->mindelx / m->cgmax;
+  m->first = 1;
+  m->firstdtg = !P->restart;
+  if (P->winds) {
+    double* w2 = ALLOCD((size_t)m->mm * m->nm);
+    for (size_t q = 0; q < (size_t)m->mm * m->nm; ++q)
+      w2[q] = sqrt(m->uwf[q] * m->uwf[q] + m->vwf[q] * m->vwf[q]);
+    remap_mn2i(m, w2, m->wspd);
+    free(w2);
+  }
+  if (P->seaice) remap_mn2i(m, m->ficef, m->fice);
+  /* Large and Pond (1981) first guess, :1356-1365 */
+  for (int i = 1; i <= im; ++i) {
+    m->cd[i] = 1.2e-3;
+    if (m->wspd[i] > 11) m->cd[i] = (0.49 + 0.065 * m->wspd[i]) * 1e-3;
+    m->ustar[i] = sqrt(m->cd[i]) * m->wspd[i];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* field table                                                         */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  const char* name;
+  size_t off;
+  int is_int;
+  int kind; /* 0 scalar-len per point(imm+1), 1 im+1, 2 table om*(im+1),
+               3 spectrum, 4 2-D, 5 pm*(im+1), 6 im*lm, 7 om, 8 pm, 9 pm*pm */
+} fdesc;
+
+#define FD(nm_, k_) {#nm_, offsetof(oracle_model, nm_), 0, k_}
+#define FI(nm_, k_) {#nm_, offsetof(oracle_model, nm_), 1, k_}
+static const fdesc FIELDS[] = {
+    FD(lon, 4), FD(lat, 4), FD(dx_2d, 4), FD(dy_2d, 4), FD(curv, 4),
+    FD(d_2d, 4), FD(ar_2d, 4), FI(mask, 4), FI(ii, 4),
+    FI(mi, 0), FI(ni, 0), FI(iw, 0), FI(ie, 0), {"is", offsetof(oracle_model, is_), 1, 0},
+    {"in", offsetof(oracle_model, in_), 1, 0}, FI(iiw, 0), FI(iie, 0), FI(iis, 0), FI(iin, 0),
+    FD(dx, 0), FD(dy, 0), FD(d, 0), FD(wspd, 0), FD(wdir, 0), FD(uc, 0),
+    FD(vc, 0), FD(rhoa, 0), FD(rhow, 0), FD(rhorat, 0), FD(fice, 0),
+    FD(ar, 1), FD(oneovar, 1), FD(oneovdx, 1), FD(oneovdy, 1), FD(dxn, 1),
+    FD(dxs, 1), FD(dye, 1), FD(dyw, 1), FD(cd, 1), FD(ustar, 1), FD(shelt, 1),
+    FD(fcutoff, 1), {"physics_time_step", offsetof(oracle_model, pts), 0, 1},
+    FI(oc, 1), FD(taux, 1), FD(tauy, 1), FD(taux_form, 1), FD(tauy_form, 1),
+    FD(taux_skin, 1), FD(tauy_skin, 1), FD(taux_diag, 1), FD(tauy_diag, 1),
+    FD(tailatmx, 1), FD(tailatmy, 1), FD(ht, 1), FD(mwp, 1), FD(mwd, 1),
+    FD(mss, 1), FD(mwl, 1), FD(momx, 1), FD(momy, 1), FD(cgmxx, 1),
+    FD(cgmxy, 1), FD(cgmyy, 1), FD(dwd, 1), FD(dwl, 1), FD(dwp, 1),
+    FD(dcp0, 1), FD(dcg0, 1), FD(dcp, 1), FD(dcg, 1), FD(ds, 1),
+    FD(k, 2), FD(kdk, 2), FD(k3dk, 2), FD(k4, 2), FD(oneoverk4, 2),
+    FD(fkovg, 2), FD(logl2overz, 2), FD(cothkd, 2), FD(invcp0, 2), FD(sbf, 2),
+    FD(sdv, 2), FD(sdt, 2), FD(snl_arg, 2), {"bf1_renorm", offsetof(oracle_model, bf1r), 0, 2},
+    {"bf2_renorm", offsetof(oracle_model, bf2r), 0, 2}, FD(dwn, 2), FD(l2, 2),
+    FD(sice, 2), FD(cp0, 2), FD(cg0, 2),
+    FD(e, 3), FD(ef, 3), FD(ssin, 3), FD(sds, 3), FD(snl, 3), FD(dummy, 3),
+    FD(rotl, 3), FD(rotr, 3), FD(cth_curv, 5), FD(sth_curv, 5), FD(us, 6),
+    FD(vs, 6), FD(f, 7), FD(dom, 7), FD(th, 8), FD(cth, 8), FD(sth, 8),
+    FD(cth2, 8), FI(pl, 8), FI(pr, 8), FD(cth2pp, 9),
+};
+
+static size_t kind_len(const oracle_model* m, int kind) {
+  size_t im1 = (size_t)m->im + 1;
+  switch (kind) {
+    case 0: return (size_t)m->imm + 1;
+    case 1: return im1;
+    case 2: return (size_t)m->om * im1;
+    case 3: return (size_t)m->om * m->pm * im1;
+    case 4: return (size_t)m->mm * m->nm;
+    case 5: return (size_t)m->pm * im1;
+    case 6: return (size_t)m->im * (m->lm > 0 ? m->lm : 1);
+    case 7: return (size_t)m->om;
+    case 8: return (size_t)m->pm;
+    case 9: return (size_t)m->pm * m->pm;
+  }
+  return 0;
+}
+
+void* oracle_field(oracle_model* m, const char* name, long* n, int* is_int) {
+  for (size_t q = 0; q < sizeof(FIELDS) / sizeof(FIELDS[0]); ++q)
+    if (!strcmp(FIELDS[q].name, name)) {
+      void** slot = (void**)((char*)m + FIELDS[q].off);
+      if (n) *n = (long)kind_len(m, FIELDS[q].kind);
+      if (is_int) *is_int = FIELDS[q].is_int;
+      return *slot;
+    }
+  return NULL;
+}
+
+double oracle_get_scalar(oracle_model* m, const char* s) {
+#define S_(x) if (!strcmp(s, #x)) return (double)m->x;
+  S_(dlnf) S_(dth) S_(dthg) S_(oneovdth) S_(cfllim) S_(bf1) S_(bf2)
+  S_(mindelx) S_(cgmax) S_(dtamin) S_(dts) S_(dta) S_(dtr) S_(dtr_temp)
+  S_(sumt) S_(first) S_(firstdtg) S_(twopisds_fac) S_(fieldscale1)
+  S_(fieldscale2) S_(inv_sds_power) S_(log10overz) S_(zerocurrents)
+#undef S_
+  return NAN;
+}
+
+int oracle_set_scalar(oracle_model* m, const char* s, double v) {
+  if (!strcmp(s, "sumt")) { m->sumt = v; return 0; }
+  if (!strcmp(s, "dts")) { m->dts = v; return 0; }
+  if (!strcmp(s, "first")) { m->first = (int)v; return 0; }
+  if (!strcmp(s, "dtr_temp")) { m->dtr_temp = v; return 0; }
+  return -1;
+}
+
+void oracle_dims(const oracle_model* m, int dims[8]) {
+  dims[0] = m->im; dims[1] = m->imm; dims[2] = m->om; dims[3] = m->pm;
+  dims[4] = m->mm; dims[5] = m->nm; dims[6] = m->lm; dims[7] = 8;
+}
+
+/* ------------------------------------------------------------------ */
+/* construction                                                        */
+/* ------------------------------------------------------------------ */
+static int check_params(const umwm_params* P, char* err, int errlen) {
+  /* src/umwm_init.F90:95-187 */
+  const char* msg = NULL;
+  if (P->mm < 3 || P->nm < 3) msg = "mm and nm must be > 2";
+  else if (P->pm % 4 != 0) msg = "pm must be divisible by 4";
+  else if (P->om < 3) msg = "om must be > 2";
+  else if (P->fmin <= 0 || P->fmax <= 0 || P->fprog <= 0) msg = "fmin, fmax and fprog must be > 0";
+  else if (P->fmin >= P->fmax) msg = "fmin must be < fmax";
+  else if (P->dtg <= 0) msg = "dtg must be > 0";
+  else if (P->z <= 0) msg = "z must be > 0";
+  else if (P->gustiness != 0) msg = "gustiness must be 0 (random gusts are not reproducible)";
+  else if (P->dmin <= 0) msg = "dmin must be > 0";
+  else if (!P->gridfromfile && (P->delx <= 0 || P->dely <= 0)) msg = "delx and dely must be > 0";
+  else if (!P->topofromfile && P->dpt <= 0) msg = "dpt must be > 0";
+  if (msg) {
+    if (err && errlen > 0) snprintf(err, errlen, "%s", msg);
+    return -1;
+  }
+  return 0;
+}
+
+oracle_model* oracle_create(const umwm_params* P, const umwm_grid_input* g,
+                            const umwm_forcing_input* f0, char* err,
+                            int errlen) {
+  if (check_params(P, err, errlen)) return NULL;
+  if ((P->gridfromfile && (!g || !g->lon || !g->lat)) || (P->topofromfile && (!g || !g->z))) {
+    if (err) snprintf(err, errlen, "grid input missing");
+    return NULL;
+  }
+  int anyforcing = P->winds || P->currents || P->air_density || P->water_density || P->seaice;
+  if (anyforcing && !f0) {
+    if (err) snprintf(err, errlen, "forcing input missing");
+    return NULL;
+  }
+  oracle_model* m = (oracle_model*)xcalloc(1, sizeof(oracle_model));
+  m->P = *P;
+  m->mm = P->mm; m->nm = P->nm; m->om = P->om; m->pm = P->pm;
+  int lm = 0;
+  for (int q = 0; q < 100; ++q) lm += P->depths[q] >= 0;
+  m->lm = lm;
+  size_t n2 = (size_t)m->mm * m->nm;
+  m->lon = ALLOCD(n2); m->lat = ALLOCD(n2); m->dx_2d = ALLOCD(n2);
+  m->dy_2d = ALLOCD(n2); m->curv = ALLOCD(n2); m->d_2d = ALLOCD(n2);
+  m->ar_2d = ALLOCD(n2); m->mask = ALLOCI(n2); m->ii = ALLOCI(n2);
+  m->uwb = ALLOCD(n2); m->uwf = ALLOCD(n2); m->vwb = ALLOCD(n2); m->vwf = ALLOCD(n2);
+  m->ucb = ALLOCD(n2); m->ucf = ALLOCD(n2); m->vcb = ALLOCD(n2); m->vcf = ALLOCD(n2);
+  m->rhoa_2d = ALLOCD(n2); m->rhow_2d = ALLOCD(n2); m->fice_2d = ALLOCD(n2);
+  m->ficeb = ALLOCD(n2); m->ficef = ALLOCD(n2);
+  grid(m, g);
+  masks(m, g);
+  if (m->im < 1) {
+    if (err) snprintf(err, errlen, "no sea points");
+    oracle_free(m);
+    return NULL;
+  }
+  const int om = m->om, pm = m->pm, im = m->im;
+  size_t nu = (size_t)m->imm + 1, ns = (size_t)im + 1;
+  m->mi = ALLOCI(nu); m->ni = ALLOCI(nu); m->iw = ALLOCI(nu); m->ie = ALLOCI(nu);
+  m->is_ = ALLOCI(nu); m->in_ = ALLOCI(nu); m->iiw = ALLOCI(nu); m->iie = ALLOCI(nu);
+  m->iis = ALLOCI(nu); m->iin = ALLOCI(nu);
+  m->dx = ALLOCD(nu); m->dy = ALLOCD(nu); m->d = ALLOCD(nu);
+  m->wspd = ALLOCD(nu); m->wdir = ALLOCD(nu); m->uc = ALLOCD(nu); m->vc = ALLOCD(nu);
+  m->rhoa = ALLOCD(nu); m->rhow = ALLOCD(nu); m->rhorat = ALLOCD(nu); m->fice = ALLOCD(nu);
+  m->rhoab = ALLOCD(nu); m->rhoaf = ALLOCD(nu); m->rhowb = ALLOCD(nu); m->rhowf = ALLOCD(nu);
+  double** pp_[] = {&m->ar, &m->oneovar, &m->oneovdx, &m->oneovdy, &m->dxn, &m->dxs,
+                    &m->dye, &m->dyw, &m->cd, &m->ustar, &m->shelt, &m->fcutoff, &m->pts,
+                    &m->taux, &m->tauy, &m->taux_form, &m->tauy_form, &m->taux_skin,
+                    &m->tauy_skin, &m->taux_diag, &m->tauy_diag, &m->tailatmx, &m->tailatmy,
+                    &m->ht, &m->mwp, &m->mwd, &m->mss, &m->mwl, &m->momx, &m->momy,
+                    &m->cgmxx, &m->cgmxy, &m->cgmyy, &m->dwd, &m->dwl, &m->dwp, &m->dcp0,
+                    &m->dcg0, &m->dcp, &m->dcg, &m->ds};
+  for (size_t q = 0; q < sizeof(pp_) / sizeof(pp_[0]); ++q) *pp_[q] = ALLOCD(ns);
+  m->oc = ALLOCI(ns);
+  double** tb_[] = {&m->k, &m->kdk, &m->k3dk, &m->k4, &m->oneoverk4, &m->fkovg,
+                    &m->logl2overz, &m->cothkd, &m->invcp0, &m->sbf, &m->sdv, &m->sdt,
+                    &m->snl_arg, &m->bf1r, &m->bf2r, &m->dwn, &m->l2, &m->sice, &m->cp0,
+                    &m->cg0};
+  for (size_t q = 0; q < sizeof(tb_) / sizeof(tb_[0]); ++q) *tb_[q] = ALLOCD((size_t)om * ns);
+  double** sp_[] = {&m->e, &m->ef, &m->ssin, &m->sds, &m->snl, &m->dummy, &m->rotl, &m->rotr};
+  for (size_t q = 0; q < sizeof(sp_) / sizeof(sp_[0]); ++q)
+    *sp_[q] = ALLOCD((size_t)om * pm * ns);
+  m->f = ALLOCD(om); m->dom = ALLOCD(om);
+  m->th = ALLOCD(pm); m->cth = ALLOCD(pm); m->sth = ALLOCD(pm); m->cth2 = ALLOCD(pm);
+  m->cth2pp = ALLOCD((size_t)pm * pm); m->pl = ALLOCI(pm); m->pr = ALLOCI(pm);
+  m->cth_curv = ALLOCD((size_t)pm * ns); m->sth_curv = ALLOCD((size_t)pm * ns);
+  m->depth = ALLOCD(lm > 0 ? lm : 1);
+  m->us = ALLOCD((size_t)im * (lm > 0 ? lm : 1));
+  m->vs = ALLOCD((size_t)im * (lm > 0 ? lm : 1));
+  m->util = ALLOCD((size_t)om * im * (lm > 0 ? lm : 1));
+  remap(m);
+  input_nc(m, f0);
+  init(m);
+  return m;
+}
+
+void oracle_free(oracle_model* m) {
+  if (!m) return;
+  void** all[] = {
+      (void**)&m->lon, (void**)&m->lat, (void**)&m->dx_2d, (void**)&m->dy_2d,
+      (void**)&m->curv, (void**)&m->d_2d, (void**)&m->ar_2d, (void**)&m->mask,
+      (void**)&m->ii, (void**)&m->mi, (void**)&m->ni, (void**)&m->iw, (void**)&m->ie,
+      (void**)&m->is_, (void**)&m->in_, (void**)&m->iiw, (void**)&m->iie,
+      (void**)&m->iis, (void**)&m->iin, (void**)&m->dx, (void**)&m->dy, (void**)&m->d,
+      (void**)&m->wspd, (void**)&m->wdir, (void**)&m->uc, (void**)&m->vc,
+      (void**)&m->rhoa, (void**)&m->rhow, (void**)&m->rhorat, (void**)&m->fice,
+      (void**)&m->uwb, (void**)&m->uwf, (void**)&m->vwb, (void**)&m->vwf,
+      (void**)&m->ucb, (void**)&m->ucf, (void**)&m->vcb, (void**)&m->vcf,
+      (void**)&m->rhoa_2d, (void**)&m->rhow_2d, (void**)&m->fice_2d,
+      (void**)&m->ficeb, (void**)&m->ficef, (void**)&m->rhoab, (void**)&m->rhoaf,
+      (void**)&m->rhowb, (void**)&m->rhowf, (void**)&m->ar, (void**)&m->oneovar,
+      (void**)&m->oneovdx, (void**)&m->oneovdy, (void**)&m->dxn, (void**)&m->dxs,
+      (void**)&m->dye, (void**)&m->dyw, (void**)&m->cd, (void**)&m->ustar,
+      (void**)&m->shelt, (void**)&m->fcutoff, (void**)&m->pts, (void**)&m->oc,
+      (void**)&m->taux, (void**)&m->tauy, (void**)&m->taux_form, (void**)&m->tauy_form,
+      (void**)&m->taux_skin, (void**)&m->tauy_skin, (void**)&m->taux_diag,
+      (void**)&m->tauy_diag, (void**)&m->tailatmx, (void**)&m->tailatmy,
+      (void**)&m->ht, (void**)&m->mwp, (void**)&m->mwd, (void**)&m->mss,
+      (void**)&m->mwl, (void**)&m->momx, (void**)&m->momy, (void**)&m->cgmxx,
+      (void**)&m->cgmxy, (void**)&m->cgmyy, (void**)&m->dwd, (void**)&m->dwl,
+      (void**)&m->dwp, (void**)&m->dcp0, (void**)&m->dcg0, (void**)&m->dcp,
+      (void**)&m->dcg, (void**)&m->k, (void**)&m->kdk, (void**)&m->k3dk,
+      (void**)&m->k4, (void**)&m->oneoverk4, (void**)&m->fkovg, (void**)&m->logl2overz,
+      (void**)&m->cothkd, (void**)&m->invcp0, (void**)&m->sbf, (void**)&m->sdv,
+      (void**)&m->sdt, (void**)&m->snl_arg, (void**)&m->bf1r, (void**)&m->bf2r,
+      (void**)&m->dwn, (void**)&m->l2, (void**)&m->sice, (void**)&m->cp0,
+      (void**)&m->cg0, (void**)&m->e, (void**)&m->ef, (void**)&m->ssin, (void**)&m->sds,
+      (void**)&m->snl, (void**)&m->dummy, (void**)&m->rotl, (void**)&m->rotr,
+      (void**)&m->f, (void**)&m->th, (void**)&m->cth, (void**)&m->sth, (void**)&m->dom,
+      (void**)&m->cth2, (void**)&m->cth2pp, (void**)&m->pl, (void**)&m->pr,
+      (void**)&m->cth_curv, (void**)&m->sth_curv, (void**)&m->depth, (void**)&m->us,
+      (void**)&m->vs, (void**)&m->ds, (void**)&m->util};
+  for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); ++q) free(*all[q]);
+  free(m);
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_forcing.F90:11-37 forcinginput                             */
+/* ------------------------------------------------------------------ */
+void oracle_forcinginput(oracle_model* m, const umwm_forcing_input* nx) {
+  const umwm_params* P = &m->P;
+  const size_t n2 = (size_t)m->mm * m->nm;
+  if (P->winds) {
+    memcpy(m->uwb, m->uwf, n2 * sizeof(double));
+    memcpy(m->vwb, m->vwf, n2 * sizeof(double));
+  }
+  if (P->currents) {
+    memcpy(m->ucb, m->ucf, n2 * sizeof(double));
+    memcpy(m->vcb, m->vcf, n2 * sizeof(double));
+  }
+  if (P->air_density) memcpy(m->rhoab, m->rhoaf, ((size_t)m->imm + 1) * sizeof(double));
+  if (P->water_density) memcpy(m->rhowb, m->rhowf, ((size_t)m->imm + 1) * sizeof(double));
+  if (P->seaice) memcpy(m->ficeb, m->ficef, n2 * sizeof(double));
+  /* input_nc(timestr) for the next level (only the switched fields) */
+  if (P->winds) {
+    memcpy(m->uwf, nx->uw, n2 * sizeof(double));
+    memcpy(m->vwf, nx->vw, n2 * sizeof(double));
+  }
+  if (P->currents) {
+    memcpy(m->ucf, nx->uc, n2 * sizeof(double));
+    memcpy(m->vcf, nx->vc, n2 * sizeof(double));
+    for (size_t q = 0; q < n2; ++q)
+      if (m->mask[q] == 0) m->ucf[q] = m->vcf[q] = 0;
+  }
+  if (P->seaice) memcpy(m->ficef, nx->fice, n2 * sizeof(double));
+  if (P->air_density) {
+    memcpy(m->rhoa_2d, nx->rhoa, n2 * sizeof(double));
+    remap_mn2i(m, m->rhoa_2d, m->rhoaf);
+  }
+  if (P->water_density) {
+    memcpy(m->rhow_2d, nx->rhow, n2 * sizeof(double));
+    remap_mn2i(m, m->rhow_2d, m->rhowf);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_forcing.F90:40-99 forcinginterpolate (gustiness = 0)       */
+/* ------------------------------------------------------------------ */
+void oracle_forcinginterpolate(oracle_model* m) {
+  const umwm_params* P = &m->P;
+  const size_t n2 = (size_t)m->mm * m->nm;
+  const double s = m->sumt, dtg = P->dtg;
+  if (P->winds) {
+    double* w1 = ALLOCD(n2);
+    double* w2 = ALLOCD(n2);
+    for (size_t q = 0; q < n2; ++q) {
+      double uw = m->uwb[q] * (1 - s / dtg) + m->uwf[q] * s / dtg;
+      double vw = m->vwb[q] * (1 - s / dtg) + m->vwf[q] * s / dtg;
+      w1[q] = sqrt(uw * uw + vw * vw);
+      w2[q] = atan2(vw, uw);
+    }
+    remap_mn2i(m, w1, m->wspd);
+    remap_mn2i(m, w2, m->wdir);
+    free(w1);
+    free(w2);
+  }
+  if (P->seaice) {
+    for (size_t q = 0; q < n2; ++q)
+      m->fice_2d[q] = m->ficeb[q] * (1 - s / dtg) + m->ficef[q] * s / dtg;
+    remap_mn2i(m, m->fice_2d, m->fice);
+  }
+  if (P->currents) {
+    double* w1 = ALLOCD(n2);
+    double* w2 = ALLOCD(n2);
+    for (size_t q = 0; q < n2; ++q) {
+      w1[q] = m->ucb[q] * (1 - s / dtg) + m->ucf[q] * s / dtg;
+      w2[q] = m->vcb[q] * (1 - s / dtg) + m->vcf[q] * s / dtg;
+    }
+    remap_mn2i(m, w1, m->uc);
+    remap_mn2i(m, w2, m->vc);
+    free(w1);
+    free(w2);
+  }
+  for (int i = 1; i <= m->imm; ++i) {
+    if (P->air_density) m->rhoa[i] = m->rhoab[i] * (1 - s / dtg) + m->rhoaf[i] * s / dtg;
+    if (P->water_density) m->rhow[i] = m->rhowb[i] * (1 - s / dtg) + m->rhowf[i] * s / dtg;
+    m->rhorat[i] = m->rhoa[i] / m->rhow[i];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_source_functions.F90:16-82 sin_d12                         */
+/* ------------------------------------------------------------------ */
+void oracle_sin_d12(oracle_model* m) {
+  const umwm_params* P = &m->P;
+  const int om = m->om, pm = m->pm, im = m->im;
+  const double twopi = 2 * 3.141592653589793;
+  for (int i = 1; i <= m->imm; ++i) m->wspd[i] = dmax(m->wspd[i], 1e-2);
+  for (int i = 1; i <= im; ++i) {
+    m->fcutoff[i] = 0.53 * P->g / m->wspd[i];
+    if (m->fcutoff[i] > P->fprog) m->fcutoff[i] = P->fprog;
+    for (int o = om - 2; o >= 1; --o) {
+      m->oc[i] = o;
+      if (m->fcutoff[i] > m->f[o - 1]) break;
+    }
+    m->shelt[i] = oracle_sheltering_coare35(m->wspd[i]);
+  }
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 1; i <= im; ++i)
+    for (int p = 1; p <= pm; ++p) {
+      double cwt = cos(m->wdir[i] - m->th[p - 1]);
+      for (int o = 1; o <= om; ++o) {
+        double s = (m->wspd[i] + 2.5 * m->ustar[i] * TB(m->logl2overz, o, i)) * cwt -
+                   TB(m->cp0, o, i) - m->uc[i] * m->cth[p - 1] - m->vc[i] * m->sth[p - 1];
+        s = P->sin_fac * fabs(s) * s;
+        if (s > 0) s = s * m->shelt[i] / P->sin_fac;
+        if (s < 0) s = s * m->fieldscale1;
+        if (s < 0 && cwt > 0) s = s * m->fieldscale2;
+        s = (1 - m->fice[i]) * twopi * m->rhorat[i] * s * TB(m->fkovg, o, i);
+        if (o > m->oc[i]) s = dmax(s, 0.);
+        SP(m->ssin, o, p, i) = s;
+      }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_source_functions.F90:85-105 sds_d12                        */
+/* ------------------------------------------------------------------ */
+void oracle_sds_d12(oracle_model* m) {
+  const umwm_params* P = &m->P;
+  const int om = m->om, pm = m->pm, im = m->im;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 1; i <= im; ++i)
+    for (int p = 1; p <= pm; ++p) {
+      SP(m->dummy, 1, p, i) = 0;
+      for (int o = 2; o <= om; ++o) {
+        double acc = 0;
+        if (P->mss_fac > 0) {
+          /* sum(e(o-1,:,i) * cth2pp(:,p)), sequential over pp */
+          for (int pp = 1; pp <= pm; ++pp)
+            acc += SP(m->e, o - 1, pp, i) * m->cth2pp[(size_t)(p - 1) * pm + pp - 1];
+          SP(m->dummy, o, p, i) = SP(m->dummy, o - 1, p, i) + acc * TB(m->k3dk, o - 1, i);
+        } else {
+          SP(m->dummy, o, p, i) = 0;
+        }
+      }
+      for (int o = 1; o <= om; ++o) {
+        double dm = 1 + P->mss_fac * SP(m->dummy, o, p, i);
+        SP(m->dummy, o, p, i) = dm * dm;
+        SP(m->sds, o, p, i) = m->twopisds_fac * m->f[o - 1] * SP(m->dummy, o, p, i) *
+                              pow(SP(m->e, o, p, i) * TB(m->k4, o, i), P->sds_power);
+      }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_source_functions.F90:108-153 s_ice                         */
+/* ------------------------------------------------------------------ */
+void oracle_s_ice(oracle_model* m) {
+  const umwm_params* P = &m->P;
+  const int om = m->om, pm = m->pm, im = m->im;
+  const double H_th = 3.0, C1 = -5.35e-6, C2 = C1 * H_th;
+  for (int i = 1; i <= im; ++i) {
+    for (int o = 1; o <= om; ++o) TB(m->sice, o, i) = 0.0;
+    if (m->fice[i] > P->fice_lth) {
+      double ht = 0;
+      for (int p = 1; p <= pm; ++p)
+        for (int o = 1; o <= om; ++o) ht += SP(m->e, o, p, i) * TB(m->kdk, o, i);
+      ht = 4 * sqrt(ht * m->dth);
+      double s = ht < H_th ? C1 * ht : C2;
+      for (int o = 1; o <= om; ++o) TB(m->sice, o, i) = 2 * TB(m->cg0, o, i) * s;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_source_functions.F90:156-183 snl_d12                       */
+/* ------------------------------------------------------------------ */
+void oracle_snl_d12(oracle_model* m) {
+  const umwm_params* P = &m->P;
+  const int om = m->om, pm = m->pm, im = m->im;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 1; i <= im; ++i) {
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= om; ++o) SP(m->snl, o, p, i) = 0;
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= m->oc[i]; ++o)
+        SP(m->snl, o, p, i) =
+            TB(m->bf1r, o, i) * SP(m->sds, o + 1, p, i) * SP(m->e, o + 1, p, i) +
+            TB(m->bf2r, o, i) * SP(m->sds, o + 2, p, i) * SP(m->e, o + 2, p, i) -
+            P->snl_fac * SP(m->sds, o, p, i) * SP(m->e, o, p, i);
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= om; ++o) SP(m->sds, o, p, i) = SP(m->sds, o, p, i) * TB(m->cothkd, o, i);
+    for (int o = 1; o <= om; ++o)
+      TB(m->sdt, o, i) = P->sdt_fac * sqrt(m->rhorat[i]) * m->ustar[i] * TB(m->k, o, i);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_physics.F90:9-86 source                                    */
+/* ------------------------------------------------------------------ */
+void oracle_source(oracle_model* m) {
+  const umwm_params* P = &m->P;
+  const int om = m->om, pm = m->pm, im = m->im;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 1; i <= im; ++i) {
+    double mx = 0;
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= om; ++o) {
+        double r = 0;
+        if (o <= m->oc[i])
+          r = SP(m->ssin, o, p, i) - SP(m->sds, o, p, i) * TB(m->snl_arg, o, i) -
+              TB(m->sbf, o, i) - TB(m->sdt, o, i) - TB(m->sdv, o, i) + TB(m->sice, o, i);
+        SP(m->ef, o, p, i) = r;
+        mx = dmax(mx, fabs(r));
+      }
+    m->pts[i] = P->explim / mx;
+  }
+  double mn = INFINITY;
+  for (int i = 1; i <= im; ++i) mn = dmin(mn, m->pts[i]);
+  m->dts = dmin(dmin(mn, m->dtamin), P->dtg - m->sumt);
+  if (!P->restart && m->first) m->dts = 0;
+  m->dta = m->dts;
+  m->sumt = m->sumt + m->dts;
+  const double dts = m->dts;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 1; i <= im; ++i)
+    for (int p = 1; p <= pm; ++p) {
+      for (int o = 1; o <= m->oc[i]; ++o)
+        SP(m->ef, o, p, i) =
+            SP(m->e, o, p, i) *
+                exp(dts * (SP(m->ssin, o, p, i) - SP(m->sds, o, p, i) - TB(m->sbf, o, i) -
+                           TB(m->sdt, o, i) - TB(m->sdv, o, i) + TB(m->sice, o, i))) +
+            dts * SP(m->snl, o, p, i);
+      for (int o = m->oc[i] + 1; o <= om; ++o) {
+        double num = SP(m->ssin, o, p, i) - TB(m->sdt, o, i) - TB(m->sdv, o, i) + TB(m->sice, o, i);
+        if (num >= 0)
+          SP(m->ef, o, p, i) =
+              TB(m->oneoverk4, o, i) *
+              pow(num / (m->twopisds_fac * m->f[o - 1] * SP(m->dummy, o, p, i) * TB(m->cothkd, o, i)),
+                  m->inv_sds_power);
+      }
+      for (int o = 1; o <= om; ++o)
+        SP(m->e, o, p, i) = 0.5 * (SP(m->e, o, p, i) + SP(m->ef, o, p, i));
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_advection.F90:18-99 propagation                            */
+/* ------------------------------------------------------------------ */
+void oracle_propagation(oracle_model* m) {
+  const umwm_params* P = &m->P;
+  const int pm = m->pm, im = m->im;
+  int anyc = 0;
+  int lo = 1, hi = P->isglobal ? m->imm : im;
+  for (int i = lo; i <= hi; ++i)
+    if (m->uc[i] != 0 || m->vc[i] != 0) anyc = 1;
+  m->zerocurrents = !anyc;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 1; i <= im; ++i) {
+    const int ie = m->ie[i], iw = m->iw[i], is = m->is_[i], in = m->in_[i];
+    double feup = 0, fedn = 0, fwup = 0, fwdn = 0, fnup = 0, fndn = 0, fsup = 0, fsdn = 0;
+    if (!m->zerocurrents) {
+      double ue = m->uc[i] + m->uc[m->iie[i]], uw = m->uc[i] + m->uc[m->iiw[i]];
+      double vn = m->vc[i] + m->vc[m->iin[i]], vs = m->vc[i] + m->vc[m->iis[i]];
+      feup = (ue + fabs(ue)) * m->dye[i];
+      fedn = (ue - fabs(ue)) * m->dye[i];
+      fwup = (uw + fabs(uw)) * m->dyw[i];
+      fwdn = (uw - fabs(uw)) * m->dyw[i];
+      fnup = (vn + fabs(vn)) * m->dxn[i];
+      fndn = (vn - fabs(vn)) * m->dxn[i];
+      fsup = (vs + fabs(vs)) * m->dxs[i];
+      fsdn = (vs - fabs(vs)) * m->dxs[i];
+    }
+    for (int p = 1; p <= pm; ++p) {
+      const double cc = PI_(m->cth_curv, p, i), sc = PI_(m->sth_curv, p, i);
+      for (int o = 1; o <= m->oc[i]; ++o) {
+        double cge = TB(m->cg0, o, i) * cc + TB(m->cg0, o, ie) * cc;
+        double cgw = TB(m->cg0, o, i) * cc + TB(m->cg0, o, iw) * cc;
+        double cgs = TB(m->cg0, o, i) * sc + TB(m->cg0, o, is) * sc;
+        double cgn = TB(m->cg0, o, i) * sc + TB(m->cg0, o, in) * sc;
+        double e0 = SP(m->e, o, p, i);
+        double flux = (cge + fabs(cge)) * m->dye[i] * e0 +
+                      (cge - fabs(cge)) * m->dye[i] * SP(m->e, o, p, ie) -
+                      (cgw + fabs(cgw)) * m->dyw[i] * SP(m->e, o, p, iw) -
+                      (cgw - fabs(cgw)) * m->dyw[i] * e0 +
+                      (cgn + fabs(cgn)) * m->dxn[i] * e0 +
+                      (cgn - fabs(cgn)) * m->dxn[i] * SP(m->e, o, p, in) -
+                      (cgs + fabs(cgs)) * m->dxs[i] * SP(m->e, o, p, is) -
+                      (cgs - fabs(cgs)) * m->dxs[i] * e0;
+        if (!m->zerocurrents)
+          flux = flux + (feup * e0 + fedn * SP(m->e, o, p, ie) - fwup * SP(m->e, o, p, iw) -
+                         fwdn * e0 + fnup * e0 + fndn * SP(m->e, o, p, in) -
+                         fsup * SP(m->e, o, p, is) - fsdn * e0);
+        double v = SP(m->ef, o, p, i) - 0.25 * m->dta * flux * m->oneovar[i];
+        if (m->fice[i] > P->fice_uth) v = 0.0;
+        SP(m->ef, o, p, i) = v;
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_advection.F90:102-180 refraction                           */
+/* ------------------------------------------------------------------ */
+void oracle_refraction(oracle_model* m) {
+  const umwm_params* P = &m->P;
+  const int om = m->om, pm = m->pm, im = m->im;
+  if (P->currents || m->first) {
+    /* rotation tendency at cell centres ("flux" in the reference) */
+    double* rot = (double*)xcalloc((size_t)om * pm, sizeof(double));
+    double mx = -INFINITY;
+    for (int i = 1; i <= im; ++i) {
+      const int ie = m->ie[i], iw = m->iw[i], is = m->is_[i], in = m->in_[i];
+      for (int p = 1; p <= pm; ++p)
+        for (int o = 1; o <= om; ++o) {
+          double r = 0;
+          if (o <= m->oc[i])
+            r = 0.5 * (((TB(m->cp0, o, ie) - TB(m->cp0, o, iw)) * m->sth[p - 1] +
+                        m->vc[m->iie[i]] - m->vc[m->iiw[i]]) * m->oneovdx[i] -
+                       ((TB(m->cp0, o, in) - TB(m->cp0, o, is)) * m->cth[p - 1] +
+                        m->uc[m->iin[i]] - m->uc[m->iis[i]]) * m->oneovdy[i]);
+          rot[(size_t)(p - 1) * om + o - 1] = r;
+          mx = dmax(mx, r); /* signed maxval over the whole array */
+        }
+      /* edge values; zero above oc (rotl = rotr = 0 first) */
+      for (int p = 1; p <= pm; ++p)
+        for (int o = 1; o <= om; ++o) {
+          double a = 0, b = 0;
+          if (o <= m->oc[i]) {
+            double r0 = rot[(size_t)(p - 1) * om + o - 1];
+            a = 0.5 * (r0 + rot[(size_t)(m->pl[p - 1] - 1) * om + o - 1]);
+            b = 0.5 * (r0 + rot[(size_t)(m->pr[p - 1] - 1) * om + o - 1]);
+          }
+          SP(m->rotl, o, p, i) = a;
+          SP(m->rotr, o, p, i) = b;
+        }
+    }
+    free(rot);
+    if (mx <= TINY)
+      m->dtr_temp = m->dts;
+    else
+      m->dtr_temp = m->dth / mx;
+  }
+  m->dtr = dmin(m->dtr_temp, m->dts);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 1; i <= im; ++i)
+    for (int p = 1; p <= pm; ++p) {
+      const int pl = m->pl[p - 1], pr = m->pr[p - 1];
+      for (int o = 1; o <= m->oc[i]; ++o) {
+        double rl = SP(m->rotl, o, p, i), rr = SP(m->rotr, o, p, i);
+        double flux = 0.5 *
+                      ((rl + fabs(rl)) * SP(m->e, o, p, i) + (rl - fabs(rl)) * SP(m->e, o, pl, i) -
+                       (rr + fabs(rr)) * SP(m->e, o, pr, i) - (rr - fabs(rr)) * SP(m->e, o, p, i)) *
+                      m->oneovdth;
+        double v = SP(m->ef, o, p, i) - m->dtr * flux;
+        if (m->fice[i] > P->fice_uth) v = 0.0;
+        SP(m->ef, o, p, i) = v;
+      }
+    }
+}
+
+/* src/umwm_top.F90:113  e = ef */
+void oracle_update(oracle_model* m) {
+  size_t n = (size_t)m->om * m->pm * m->im;
+  memcpy(m->e + (size_t)m->om * m->pm, m->ef + (size_t)m->om * m->pm, n * sizeof(double));
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_stress.F90:216-264 stress('atm') (+ helpers :268-344)      */
+/* ------------------------------------------------------------------ */
+void oracle_stress_atm(oracle_model* m) {
+  const umwm_params* P = &m->P;
+  const int om = m->om, pm = m->pm, im = m->im;
+  const double *us1 = m->us, *vs1 = m->vs; /* level l = 1 */
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 1; i <= im; ++i) {
+    double tx[512], ty[512]; /* om <= 512 */
+    for (int o = 1; o <= om; ++o) tx[o - 1] = ty[o - 1] = 0;
+    /* stress_vector_integral: cp(o,i) is cp0(o,i-1) because the dummy
+     * argument cp(:,:) has lower bound 1 over cp0(om,0:im) (:219,:316) */
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= om; ++o) {
+        double es = SP(m->e, o, p, i) * SP(m->ssin, o, p, i);
+        tx[o - 1] = tx[o - 1] + es * m->cth[p - 1] / TB(m->cp0, o, i - 1);
+        ty[o - 1] = ty[o - 1] + es * m->sth[p - 1] / TB(m->cp0, o, i - 1);
+      }
+    double tail = oracle_stress_tail(m->wspd[i], TB(m->k, om, i));
+    double rw = m->rhow[i];
+    m->tailatmx[i] = tx[om - 1] * TB(m->k, om, i) * tail * rw * m->dthg;
+    m->tailatmy[i] = ty[om - 1] * TB(m->k, om, i) * tail * rw * m->dthg;
+    double sx = 0, sy = 0, dx_ = 0, dy_ = 0;
+    for (int o = 1; o <= om; ++o) {
+      sx += tx[o - 1] * TB(m->kdk, o, i);
+      sy += ty[o - 1] * TB(m->kdk, o, i);
+    }
+    for (int o = m->oc[i]; o <= om; ++o) {
+      dx_ += tx[o - 1] * TB(m->kdk, o, i);
+      dy_ += ty[o - 1] * TB(m->kdk, o, i);
+    }
+    m->taux_form[i] = sx * rw * m->dthg + m->tailatmx[i];
+    m->tauy_form[i] = sy * rw * m->dthg + m->tailatmy[i];
+    m->taux_diag[i] = dx_ * rw * m->dthg;
+    m->tauy_diag[i] = dy_ * rw * m->dthg;
+    /* wind_relative (:333-344) with the surface Stokes drift */
+    double urel = m->wspd[i] * cos(m->wdir[i]) - (m->uc[i] + us1[i - 1]);
+    double vrel = m->wspd[i] * sin(m->wdir[i]) - (m->vc[i] + vs1[i - 1]);
+    double wspdrel = sqrt(urel * urel + vrel * vrel);
+    double wdirrel = atan2(vrel, urel);
+    double rhoa = m->rhoa[i], w = m->wspd[i];
+    double cd_form = drag_coefficient(m->taux_form[i], m->tauy_form[i], rhoa, w);
+    double cd_skin = oracle_drag_coefficient_skin(cd_form, w, wspdrel, P->z, P->nu_air, P->kappa);
+    m->taux_skin[i] = rhoa * cd_skin * (wspdrel * wspdrel) * cos(wdirrel);
+    m->tauy_skin[i] = rhoa * cd_skin * (wspdrel * wspdrel) * sin(wdirrel);
+    m->taux[i] = m->taux_form[i] + m->taux_skin[i];
+    m->tauy[i] = m->tauy_form[i] + m->tauy_skin[i];
+    m->cd[i] = drag_coefficient(m->taux[i], m->tauy[i], rhoa, w);
+    m->ustar[i] = sqrt(sqrt(m->taux[i] * m->taux[i] + m->tauy[i] * m->tauy[i]) / rhoa);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_physics.F90:89-205 diag                                    */
+/* ------------------------------------------------------------------ */
+void oracle_diag(oracle_model* m) {
+  const int om = m->om, pm = m->pm, im = m->im;
+  const double twopi = 2 * 3.141592653589793;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 1; i <= im; ++i) {
+    double m0 = 0, m2 = 0;
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= om; ++o) {
+        double ek = SP(m->e, o, p, i) * TB(m->kdk, o, i);
+        m0 = m0 + ek;
+        m2 = m2 + (m->f[o - 1] * m->f[o - 1]) * SP(m->e, o, p, i) * TB(m->kdk, o, i);
+      }
+    m->mwp[i] = sqrt(m0 / (m2 + TINY));
+    double mx = 0, my = 0, gxx = 0, gxy = 0, gyy = 0;
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= m->oc[i]; ++o) {
+        double ekc = SP(m->e, o, p, i) * TB(m->kdk, o, i) * TB(m->invcp0, o, i);
+        double c = m->cth[p - 1], s = m->sth[p - 1], cg = TB(m->cg0, o, i);
+        mx = mx + ekc * c;
+        my = my + ekc * s;
+        gxx = gxx + cg * ekc * (c * c);
+        gxy = gxy + cg * ekc * c * s;
+        gyy = gyy + cg * ekc * (s * s);
+      }
+    m->momx[i] = mx * m->rhow[i] * m->dthg;
+    m->momy[i] = my * m->rhow[i] * m->dthg;
+    m->cgmxx[i] = gxx * m->rhow[i] * m->dthg;
+    m->cgmxy[i] = gxy * m->rhow[i] * m->dthg;
+    m->cgmyy[i] = gyy * m->rhow[i] * m->dthg;
+    double h = 0;
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= om; ++o) h = h + SP(m->e, o, p, i) * TB(m->kdk, o, i);
+    m->ht[i] = 4 * sqrt(h * m->dth);
+    double xc = 0, yc = 0;
+    for (int p = 1; p <= pm; ++p) {
+      double mag = 0;
+      for (int o = 1; o <= om; ++o) mag += SP(m->e, o, p, i) * TB(m->kdk, o, i);
+      xc = xc + mag * m->cth[p - 1];
+      yc = yc + mag * m->sth[p - 1];
+    }
+    m->mwd[i] = atan2(yc, xc);
+    m0 = 0;
+    m2 = 0;
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= om; ++o) {
+        m0 = m0 + SP(m->e, o, p, i) * TB(m->kdk, o, i);
+        m2 = m2 + SP(m->e, o, p, i) * TB(m->k3dk, o, i);
+      }
+    m->mss[i] = m2 * m->dth;
+    m->mwl[i] = twopi * sqrt(m0 / (m2 + TINY));
+    /* maxloc: first maximum in column-major (o fastest) order */
+    int op = 1, ppk = 1;
+    double best = -INFINITY;
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= om; ++o) {
+        double v = SP(m->e, o, p, i) * TB(m->kdk, o, i);
+        if (v > best) {
+          best = v;
+          op = o;
+          ppk = p;
+        }
+      }
+    m->dwd[i] = m->th[ppk - 1];
+    m->dwl[i] = twopi / TB(m->k, op, i);
+    m->dwp[i] = 1. / m->f[op - 1];
+    m->dcp0[i] = TB(m->cp0, op, i);
+    m->dcg0[i] = TB(m->cg0, op, i);
+    m->dcp[i] = m->dcp0[i] + m->uc[i] * m->cth[ppk - 1] + m->vc[i] * m->sth[ppk - 1];
+    m->dcg[i] = m->dcg0[i] + m->uc[i] * m->cth[ppk - 1] + m->vc[i] * m->sth[ppk - 1];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_stokes.F90:13-132 stokes_drift                             */
+/* ------------------------------------------------------------------ */
+void oracle_stokes_drift(oracle_model* m, int init) {
+  const int om = m->om, pm = m->pm, im = m->im, lm = m->lm;
+  const double twopi = 2 * 3.141592653589793, eulerinv = 1 / exp(1.0);
+  if (lm < 1) return;
+#define UT(o, i, l) m->util[(((size_t)(l) - 1) * im + (i) - 1) * om + (o) - 1]
+  if (init) {
+    for (int l = 1; l <= lm; ++l) m->depth[l - 1] = -m->P.depths[l - 1];
+    for (int l = 1; l <= lm; ++l)
+      for (int i = 1; i <= im; ++i) {
+        for (int o = 1; o <= om; ++o) {
+          double kk = TB(m->k, o, i), kd = kk * m->d[i];
+          double arg = 2 * kk * (m->depth[l - 1] + m->d[i]);
+          if (fabs(arg) > 50 || kd > 50)
+            UT(o, i, l) = twopi * m->f[o - 1] * 2 * (kk * kk) * exp(2 * kk * m->depth[l - 1]) *
+                          TB(m->dwn, o, i) * m->dth;
+          else {
+            double sh = sinh(kd);
+            UT(o, i, l) = twopi * m->f[o - 1] * (kk * kk) * cosh(2 * kk * (m->depth[l - 1] + m->d[i])) /
+                          (sh * sh) * TB(m->dwn, o, i) * m->dth;
+          }
+        }
+        if (fabs(m->depth[l - 1]) > m->d[i])
+          for (int o = 1; o <= om; ++o) UT(o, i, l) = 0;
+      }
+  }
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 1; i <= im; ++i) {
+    for (int l = 1; l <= lm; ++l) {
+      double u = 0, v = 0;
+      for (int p = 1; p <= pm; ++p)
+        for (int o = 1; o <= om; ++o) {
+          u = u + UT(o, i, l) * SP(m->e, o, p, i) * m->cth[p - 1];
+          v = v + UT(o, i, l) * SP(m->e, o, p, i) * m->sth[p - 1];
+        }
+      m->us[(size_t)(l - 1) * im + i - 1] = u;
+      m->vs[(size_t)(l - 1) * im + i - 1] = v;
+    }
+    double ds = 0;
+    double u1 = m->us[i - 1], v1 = m->vs[i - 1];
+    double mag1 = sqrt(u1 * u1 + v1 * v1);
+    if (mag1 != 0) {
+      double ef = mag1 * eulerinv;
+      for (int l = 2; l <= lm; ++l) {
+        double ua = m->us[(size_t)(l - 2) * im + i - 1], va = m->vs[(size_t)(l - 2) * im + i - 1];
+        double ub = m->us[(size_t)(l - 1) * im + i - 1], vb = m->vs[(size_t)(l - 1) * im + i - 1];
+        double ma = sqrt(ua * ua + va * va), mb = sqrt(ub * ub + vb * vb);
+        if (mb < ef) {
+          ds = (fabs(ma - ef) * m->depth[l - 1] + fabs(mb - ef) * m->depth[l - 2]) / (ma - mb);
+          break;
+        }
+      }
+    }
+    m->ds[i] = -ds;
+  }
+#undef UT
+}
+
+/* ------------------------------------------------------------------ */
+/* JONSWAP x cos^2 initial spectrum (build-defined; the reference has  */
+/* no initializer).  e = S(f) f dlnf D(theta) / kdk so that            */
+/* Hs = 4 sqrt(sum_o S(f_o) f_o dlnf) when the directional weights     */
+/* sum to one.                                                         */
+/* ------------------------------------------------------------------ */
+void oracle_init_jonswap(oracle_model* m, const umwm_jonswap* js) {
+  const int om = m->om, pm = m->pm, im = m->im;
+  const double pi = 3.141592653589793, twopi = 2 * pi, g = m->P.g;
+  for (int i = 1; i <= im; ++i) {
+    double fp = js->fp;
+    if (js->fp_from_wind) {
+      fp = 0.13 * g / dmax(m->wspd[i], 3.0);
+      fp = dmin(dmax(fp, 0.05), 0.5);
+    }
+    double thw = js->dir_from_wind ? m->wdir[i] : js->thw;
+    for (int o = 1; o <= om; ++o) {
+      double fo = m->f[o - 1];
+      double sig = fo <= fp ? js->sigma_a : js->sigma_b;
+      double r = exp(-(fo - fp) * (fo - fp) / (2 * sig * sig * fp * fp));
+      double S = js->alpha * g * g * pow(twopi, -4.) * pow(fo, -5.) *
+                 exp(-1.25 * pow(fp / fo, 4.)) * pow(js->gamma, r);
+      for (int p = 1; p <= pm; ++p) {
+        double c = cos(m->th[p - 1] - thw);
+        double D = c > 0 ? (2 / pi) * c * c : 0;
+        SP(m->e, o, p, i) = S * fo * m->dlnf * D / TB(m->kdk, o, i);
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_top.F90:89-150 one substep, and a dtg                      */
+/* ------------------------------------------------------------------ */
+void oracle_substep(oracle_model* m) {
+  oracle_forcinginterpolate(m);
+  oracle_sin_d12(m);
+  oracle_sds_d12(m);
+  oracle_snl_d12(m);
+  oracle_s_ice(m);
+  oracle_source(m);
+  oracle_propagation(m);
+  oracle_refraction(m);
+  oracle_update(m);
+  oracle_stress_atm(m);
+  if (m->first) {
+    oracle_diag(m);
+    m->first = 0;
+  }
+}
+
+int oracle_run_dtg(oracle_model* m) {
+  int n = 0;
+  m->sumt = 0;
+  while (m->sumt < m->P.dtg) {
+    oracle_substep(m);
+    ++n;
+  }
+  if (m->firstdtg) m->firstdtg = 0;
+  if (m->P.stokes) oracle_stokes_drift(m, 0);
+  oracle_diag(m);
+  return n;
+}

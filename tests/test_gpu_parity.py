@@ -1,0 +1,183 @@
+"""GPU parity: the HIP library (operator path and fused path) against the
+FP64 CPU oracle on identical seeded inputs.  All calls go through the
+C-ABI of libumwm_hip.so."""
+import numpy as np
+import pytest
+
+from umwm_amd import configs
+from tests.harness import hs_rmse, make_pair, pointwise_relerr, relerr, tiny_case
+
+pytestmark = pytest.mark.gpu
+
+# FP64 tolerances: the device math library (exp, pow, cos) differs from
+# glibc by a few ulp and the fused kernels reassociate sums, so a single
+# operator agrees to ~1e-14 and a multi-substep run to ~1e-12.  The bars
+# below are the 1e-10 the FP64 plain path must meet, with room to spare
+# for single operators.
+TOL_OP = 1e-11
+TOL_RUN = 1e-10
+
+MAP_FIELDS = ["mask", "ii", "mi", "ni", "iw", "ie", "is", "in", "iiw", "iie", "iis", "iin"]
+TABLE_FIELDS = ["k", "kdk", "k3dk", "k4", "oneoverk4", "cp0", "cg0", "invcp0", "fkovg",
+                "logl2overz", "cothkd", "sbf", "sdv", "snl_arg", "bf1_renorm", "bf2_renorm",
+                "dwn"]
+
+
+@pytest.mark.parametrize("case", ["const_depth", "land_shelf", "global", "winds_currents"])
+def test_setup_maps_bit_exact(case, oracle_lib):
+    wl = tiny_case(case)
+    o, d = make_pair(wl, jonswap=False)
+    assert (o.im, o.imm) == (d.im, d.imm)
+    for name in MAP_FIELDS:
+        a, b = d.get(name), o.get(name)
+        np.testing.assert_array_equal(np.asarray(a).reshape(-1),
+                                      np.asarray(b).reshape(-1)[: np.asarray(a).size],
+                                      err_msg=name)
+    for name in TABLE_FIELDS:
+        assert relerr(d.get(name), o.get(name)) <= 1e-15, name
+    for name in ("dtamin", "cfllim", "dlnf", "dth", "bf1", "bf2"):
+        assert abs(d.scalar(name) - o.scalar(name)) <= 1e-15 * abs(o.scalar(name)), name
+
+
+def _op_compare(o, d, names, tol, tag):
+    for name in names:
+        err = relerr(d.get(name), o.get(name))
+        assert err <= tol, f"{tag}: {name} rel err {err:.3e}"
+
+
+@pytest.mark.parametrize("case", ["const_depth", "land_shelf", "global", "winds_currents"])
+def test_operators_fp64(case, oracle_lib):
+    """Each reference operator, called in the driver's order for two
+    substeps, against the oracle's module arrays."""
+    wl = tiny_case(case)
+    o, d = make_pair(wl)
+    for step in range(3):
+        tag = f"{case} substep {step}"
+        o.forcinginterpolate()
+        d.forcinginterpolate()
+        _op_compare(o, d, ["wspd", "wdir", "rhorat", "uc", "vc"], 1e-15, tag)
+        o.sin_d12()
+        d.sin_d12()
+        np.testing.assert_array_equal(d.get("oc"), o.get("oc"), err_msg=tag)
+        _op_compare(o, d, ["ssin", "shelt", "fcutoff"], TOL_OP, tag)
+        o.sds_d12()
+        d.sds_d12()
+        _op_compare(o, d, ["dummy", "sds"], TOL_OP, tag)
+        o.snl_d12()
+        d.snl_d12()
+        _op_compare(o, d, ["snl", "sds", "sdt"], TOL_OP, tag)
+        o.s_ice()
+        d.s_ice()
+        _op_compare(o, d, ["sice"], TOL_OP, tag)
+        o.source()
+        d.source()
+        for s in ("dts", "sumt"):
+            assert abs(d.scalar(s) - o.scalar(s)) <= TOL_OP * max(abs(o.scalar(s)), 1.0), (tag, s)
+        _op_compare(o, d, ["ef", "e"], TOL_OP, tag)
+        o.propagation()
+        d.propagation()
+        _op_compare(o, d, ["ef"], TOL_OP, tag)
+        o.refraction()
+        d.refraction()
+        _op_compare(o, d, ["rotl", "rotr", "ef"], TOL_OP, tag)
+        assert abs(d.scalar("dtr") - o.scalar("dtr")) <= TOL_OP * max(abs(o.scalar("dtr")), 1.0)
+        o.update()
+        d.update()
+        _op_compare(o, d, ["e"], TOL_OP, tag)
+        o.stress_atm()
+        d.stress("atm")
+        _op_compare(o, d, ["ustar", "cd", "taux", "tauy", "taux_form", "tauy_form",
+                           "taux_skin", "tauy_skin", "taux_diag", "tauy_diag"], TOL_OP, tag)
+        if step == 0:
+            o.diag()
+            d.diag()
+            _op_compare(o, d, ["ht", "mwp", "mss", "mwl", "dwl", "dwp"], TOL_OP, tag)
+            # vector components are judged against the vector's magnitude:
+            # a component of a spectrum symmetric about an axis is ~1e-15
+            for x, y in (("momx", "momy"), ("cgmxx", "cgmxy"), ("cgmyy", "cgmxy")):
+                ox, oy = o.get(x), o.get(y)
+                scale = max(np.max(np.abs(ox)), np.max(np.abs(oy)))
+                for name, ref in ((x, ox), (y, oy)):
+                    err = np.max(np.abs(d.get(name) - ref)) / scale
+                    assert err <= TOL_OP, f"{tag}: {name} rel err {err:.3e}"
+            o.set_scalar("first", 0)
+            d.end_first()
+
+
+@pytest.mark.parametrize("case", ["const_depth", "land_shelf", "global", "winds_currents"])
+def test_fused_fp64_substeps(case, oracle_lib):
+    """The fused three-kernel substep against the oracle's operator
+    sequence, substep by substep."""
+    wl = tiny_case(case)
+    o, d = make_pair(wl)
+    for step in range(6):
+        o.substep()
+        s_d, dts_d = d.substep()
+        assert abs(dts_d - o.scalar("dts")) <= TOL_RUN * max(o.scalar("dts"), 1.0), step
+        assert abs(s_d - o.scalar("sumt")) <= TOL_RUN * max(o.scalar("sumt"), 1.0), step
+        err = relerr(d.get("e"), o.get("e"))
+        assert err <= TOL_RUN, f"{case} step {step}: e rel err {err:.3e}"
+        for name in ("ustar", "cd", "taux", "tauy"):
+            err = pointwise_relerr(d.get(name), o.get(name), floor=1e-12)
+            assert err <= TOL_RUN, f"{case} step {step}: {name} rel err {err:.3e}"
+
+
+def test_fused_matches_operators_fp64(oracle_lib):
+    """Operator path and fused path of the library agree with each other."""
+    wl = tiny_case("land_shelf", seed=3)
+    _, a = make_pair(wl)
+    _, b = make_pair(wl)
+    for _ in range(4):
+        a.operator_substep()
+        b.substep()
+    assert relerr(b.get("e"), a.get("e")) <= TOL_RUN
+    assert relerr(b.get("ustar"), a.get("ustar")) <= TOL_RUN
+
+
+def test_run_dtg_fp64_tiny(oracle_lib):
+    """One dtg of the inner loop, Stokes drift and diag."""
+    wl = tiny_case("land_shelf", seed=5)
+    o, d = make_pair(wl)
+    n_o = o.run_dtg()
+    n_d = d.run_dtg()
+    assert n_o == n_d
+    for name in ("ht", "mwp", "ustar"):
+        err = pointwise_relerr(d.get(name), o.get(name), floor=1e-12)
+        assert err <= TOL_RUN, f"{name} {err:.3e}"
+    assert relerr(d.get("e"), o.get("e")) <= TOL_RUN
+    assert relerr(d.get("us"), o.get("us")) <= 1e-9
+    assert relerr(d.get("ds"), o.get("ds")) <= 1e-9
+
+
+def test_c1_fp64_one_hour(oracle_lib):
+    """Config 1 (64x48, 10 km, 4000 m, 15 m/s, JONSWAP), 1 h of dynamic
+    substeps: Hs, Tm and ustar to relative 1e-10."""
+    wl = configs.c1()
+    o, d = make_pair(wl, threads=8)
+    n_o = o.run_dtg()
+    n_d = d.run_dtg()
+    assert n_o == n_d
+    for name in ("ht", "mwp", "ustar"):
+        err = pointwise_relerr(d.get(name), o.get(name), floor=1e-12)
+        assert err <= TOL_RUN, f"{name} {err:.3e}"
+
+
+def test_c1_fp32_vs_fp64(oracle_lib):
+    """FP32 path on config 1: Hs within 0.5 %, ustar within 3 %."""
+    wl = configs.c1()
+    o, d = make_pair(wl, precision=4, threads=8)
+    o.run_dtg()
+    d.run_dtg()
+    hs, hs_ref = d.get("ht"), o.get("ht")
+    assert hs_rmse(hs, hs_ref) <= 0.005
+    assert pointwise_relerr(hs, hs_ref) <= 0.005
+    assert pointwise_relerr(d.get("ustar"), o.get("ustar")) <= 0.03
+
+
+@pytest.mark.parametrize("case", ["land_shelf", "winds_currents"])
+def test_fp32_tiny_tracks_fp64(case, oracle_lib):
+    wl = tiny_case(case, seed=7)
+    o, d = make_pair(wl, precision=4)
+    o.run_dtg()
+    d.run_dtg()
+    assert hs_rmse(d.get("ht"), o.get("ht")) <= 0.005

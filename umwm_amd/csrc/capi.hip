@@ -1,0 +1,1268 @@
+// C-ABI of the UMWM gfx950 library (include/umwm_hip.h): domain lifetime,
+// reference operator entry points, the fused substep and dtg loop, field
+// exchange in the reference's layout, timing, and batched ensembles.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "domain.hpp"
+#include "physics.hpp"
+
+namespace umwm {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+namespace {
+
+// ---------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T* p, size_t n, T v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_begin_dtg(Scalars* sc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->sumt = 0;
+    sc->nsub = 0;
+  }
+}
+
+// Gathers (sumt, dts, active, nsub) of every batch member into one array so
+// the host reads them with a single copy.
+struct ScalarsView {
+  double sumt, dts;
+  int32_t active, nsub;
+};
+__global__ void k_gather_scalars(Scalars* const* scs, int n, ScalarsView* out) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    const Scalars* s = scs[k];
+    out[k].sumt = s->sumt;
+    out[k].dts = s->dts;
+    out[k].active = s->active;
+    out[k].nsub = s->nsub;
+  }
+}
+
+template <typename T>
+int dfill(T* p, size_t n, T v, hipStream_t s) {
+  if (!n) return 0;
+  size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  UMWM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int upload(T* dst, const std::vector<double>& src, size_t n, hipStream_t s) {
+  std::vector<T> tmp(n, T(0));
+  for (size_t q = 0; q < std::min(n, src.size()); ++q) tmp[q] = T(src[q]);
+  UMWM_HIP_TRY(hipMemcpyAsync(dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+  UMWM_HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int upload_int(int32_t* dst, const std::vector<int>& src, size_t n, int pad, hipStream_t s) {
+  std::vector<int32_t> tmp(n, pad);
+  for (size_t q = 0; q < std::min(n, src.size()); ++q) tmp[q] = src[q];
+  UMWM_HIP_TRY(hipMemcpyAsync(dst, tmp.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  UMWM_HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+// XCD-aware block order: the hardware deals workgroups round-robin to the
+// 8 XCDs, so hand each XCD a contiguous range of point blocks; the x-
+// neighbours of a point (one grid column away in the unrolled index) then
+// sit in the same XCD's L2.
+constexpr int kXcd = 8;
+std::vector<int> xcd_order(int nb) {
+  std::vector<int> pb(nb);
+  const int per = nb / kXcd, rem = nb % kXcd;
+  for (int b = 0; b < nb; ++b) {
+    const int x = b % kXcd, k = b / kXcd;
+    pb[b] = x * per + std::min(x, rem) + k;
+  }
+  return pb;
+}
+
+Tile choose_tile(int pm) {
+  // Smallest register footprint (directions per thread D) that fits in at
+  // most 8 waves per workgroup; W = ceil(pm / D).
+  Tile t;
+  for (int D : {3, 6, 9, 12, 18})
+    if ((pm + D - 1) / D <= 8) {
+      t.D = D;
+      t.W = (pm + D - 1) / D;
+      return t;
+    }
+  t.D = 18;
+  t.W = (pm + 17) / 18;
+  return t;
+}
+
+}  // namespace
+
+template <typename T>
+DomainImpl<T>::~DomainImpl() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (sc_host) (void)hipHostFree(sc_host);
+  arena.release();
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+template <typename T>
+int ensure_ops(DomainImpl<T>& D) {
+  if (D.ops_ready) return 0;
+  const size_t ns = (size_t)D.h.om * D.h.pm * D.h.ldi, nt = (size_t)D.h.om * D.h.ldi;
+  DomainDev<T>& h = D.h;
+  h.ssin = D.arena.template alloc<T>(ns);
+  h.sds = D.arena.template alloc<T>(ns);
+  h.snl = D.arena.template alloc<T>(ns);
+  h.dummy = D.arena.template alloc<T>(ns);
+  h.rotl = D.arena.template alloc<T>(ns);
+  h.rotr = D.arena.template alloc<T>(ns);
+  h.sdt = D.arena.template alloc<T>(nt);
+  h.sice = D.arena.template alloc<T>(nt);
+  if (!h.ssin || !h.sds || !h.snl || !h.dummy || !h.rotl || !h.rotr || !h.sdt || !h.sice)
+    return fail(UMWM_ERR_NOMEM, "device allocation of operator arrays failed");
+  for (T* p : {h.ssin, h.sds, h.snl, h.dummy, h.rotl, h.rotr})
+    if (int rc = dfill<T>(p, ns, T(0), D.stream)) return rc;
+  for (T* p : {h.sdt, h.sice})
+    if (int rc = dfill<T>(p, nt, T(0), D.stream)) return rc;
+  UMWM_HIP_TRY(hipMemcpyAsync(D.d, &D.h, sizeof(DomainDev<T>), hipMemcpyHostToDevice, D.stream));
+  UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  D.ops_ready = true;
+  return 0;
+}
+
+template <typename T>
+int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forcing_input* f0,
+                DomainImpl<T>*& out) {
+  auto* D = new DomainImpl<T>();
+  std::string err;
+  int rc = build_setup(P, g, f0, D->S, err);
+  if (rc) {
+    delete D;
+    return fail(rc, err);
+  }
+  const HostSetup& S = D->S;
+  D->precision = sizeof(T);
+  D->first = true;
+  D->firstdtg = !P.restart;
+  if (hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete D;
+    return fail(UMWM_ERR_HIP, "hipStreamCreate failed");
+  }
+  hipStream_t s = D->stream;
+  DomainDev<T>& h = D->h;
+  DeviceArena& A = D->arena;
+  h.im = S.im; h.imm = S.imm; h.om = S.om; h.pm = S.pm; h.ldi = S.ldi; h.ldq = S.ldq;
+  h.mm = S.mm; h.nm = S.nm; h.lm = S.lm;
+  h.isglobal = P.isglobal; h.restart = P.restart; h.currents = P.currents;
+  h.g = P.g; h.explim = P.explim; h.fprog = P.fprog; h.dtg = P.dtg; h.dtamin = S.dtamin;
+  h.dth = S.dth; h.dthg = S.dthg; h.oneovdth = S.oneovdth; h.twopisds_fac = S.twopisds_fac;
+  h.sin_fac = P.sin_fac; h.fieldscale1 = S.fieldscale1; h.fieldscale2 = S.fieldscale2;
+  h.sds_power = P.sds_power; h.inv_sds_power = S.inv_sds_power; h.mss_fac = P.mss_fac;
+  h.snl_fac = P.snl_fac; h.sdt_fac = P.sdt_fac; h.z = P.z; h.nu_air = P.nu_air;
+  h.kappa = P.kappa; h.fice_lth = P.fice_lth; h.fice_uth = P.fice_uth;
+  h.tiny = sizeof(T) == 8 ? DBL_MIN : (double)FLT_MIN;
+
+  const int om = S.om, pm = S.pm, ldi = S.ldi, ldq = S.ldq;
+  const size_t ns = (size_t)om * pm * ldi, nt = (size_t)om * ldi;
+  h.e = A.template alloc<T>(ns);
+  h.ef = A.template alloc<T>(ns);
+  h.es = A.template alloc<T>(ns);
+  if (!h.e || !h.ef || !h.es) {
+    delete D;
+    return fail(UMWM_ERR_NOMEM, "device allocation of the spectrum failed");
+  }
+  // tables
+  struct TabUp { const T** dst; const std::vector<double>* src; };
+  T *k = A.template alloc<T>(nt), *kdk = A.template alloc<T>(nt), *k3dk = A.template alloc<T>(nt);
+  T *k4 = A.template alloc<T>(nt), *ok4 = A.template alloc<T>(nt), *cp0 = A.template alloc<T>(nt);
+  T *cg0 = A.template alloc<T>(nt), *icp = A.template alloc<T>(nt), *fkg = A.template alloc<T>(nt);
+  T *ll2 = A.template alloc<T>(nt), *cth = A.template alloc<T>(nt), *sbf = A.template alloc<T>(nt);
+  T *sdv = A.template alloc<T>(nt), *sarg = A.template alloc<T>(nt), *b1 = A.template alloc<T>(nt);
+  T *b2 = A.template alloc<T>(nt), *dwn = A.template alloc<T>(nt);
+  T *ccv = A.template alloc<T>((size_t)pm * ldi), *scv = A.template alloc<T>((size_t)pm * ldi);
+  T *f = A.template alloc<T>(om), *th = A.template alloc<T>(pm), *ct = A.template alloc<T>(pm);
+  T *st = A.template alloc<T>(pm), *c2 = A.template alloc<T>(pm), *s2 = A.template alloc<T>(pm);
+  T* c2pp = A.template alloc<T>((size_t)pm * pm);
+  int32_t *pl = A.template alloc<int32_t>(pm), *pr = A.template alloc<int32_t>(pm);
+  if (!dwn || !c2pp || !pr) {
+    delete D;
+    return fail(UMWM_ERR_NOMEM, "device allocation of tables failed");
+  }
+  const std::pair<T*, const std::vector<double>*> tabs[] = {
+      {k, &S.k}, {kdk, &S.kdk}, {k3dk, &S.k3dk}, {k4, &S.k4}, {ok4, &S.oneoverk4},
+      {cp0, &S.cp0}, {cg0, &S.cg0}, {icp, &S.invcp0}, {fkg, &S.fkovg}, {ll2, &S.logl2overz},
+      {cth, &S.cothkd}, {sbf, &S.sbf}, {sdv, &S.sdv}, {sarg, &S.snl_arg}, {b1, &S.bf1r},
+      {b2, &S.bf2r}, {dwn, &S.dwn}};
+  for (auto& tb : tabs)
+    if ((rc = upload<T>(tb.first, *tb.second, nt, s))) { delete D; return rc; }
+  if ((rc = upload<T>(ccv, S.cth_curv, (size_t)pm * ldi, s)) ||
+      (rc = upload<T>(scv, S.sth_curv, (size_t)pm * ldi, s)) || (rc = upload<T>(f, S.f, om, s)) ||
+      (rc = upload<T>(th, S.th, pm, s)) || (rc = upload<T>(ct, S.cth, pm, s)) ||
+      (rc = upload<T>(st, S.sth, pm, s)) || (rc = upload<T>(c2, S.c2th, pm, s)) ||
+      (rc = upload<T>(s2, S.s2th, pm, s)) || (rc = upload<T>(c2pp, S.cth2pp, (size_t)pm * pm, s)) ||
+      (rc = upload_int(pl, S.pl, pm, 0, s)) || (rc = upload_int(pr, S.pr, pm, 0, s))) {
+    delete D;
+    return rc;
+  }
+  h.k = k; h.kdk = kdk; h.k3dk = k3dk; h.k4 = k4; h.oneoverk4 = ok4; h.cp0 = cp0; h.cg0 = cg0;
+  h.invcp0 = icp; h.fkovg = fkg; h.logl2overz = ll2; h.cothkd = cth; h.sbf = sbf; h.sdv = sdv;
+  h.snl_arg = sarg; h.bf1r = b1; h.bf2r = b2; h.dwn = dwn; h.cth_curv = ccv; h.sth_curv = scv;
+  h.f = f; h.th = th; h.cth = ct; h.sth = st; h.c2th = c2; h.s2th = s2; h.cth2pp = c2pp;
+  h.pl = pl; h.pr = pr;
+  // per unrolled point
+  T** unr[] = {&h.wspd, &h.wdir, &h.uc, &h.vc, &h.rhoa, &h.rhow, &h.rhorat, &h.fice};
+  for (T** p : unr) *p = A.template alloc<T>(ldq);
+  std::vector<double> rhorat(ldq, 0.0);
+  for (int q = 0; q < S.imm; ++q) rhorat[q] = S.rhoa[q] / S.rhow[q];
+  if ((rc = upload<T>(h.wspd, S.wspd, ldq, s)) || (rc = upload<T>(h.wdir, S.wdir, ldq, s)) ||
+      (rc = upload<T>(h.uc, S.uc, ldq, s)) || (rc = upload<T>(h.vc, S.vc, ldq, s)) ||
+      (rc = upload<T>(h.rhoa, S.rhoa, ldq, s)) || (rc = upload<T>(h.rhow, S.rhow, ldq, s)) ||
+      (rc = upload<T>(h.rhorat, rhorat, ldq, s)) || (rc = upload<T>(h.fice, S.fice, ldq, s))) {
+    delete D;
+    return rc;
+  }
+  // per sea point
+  T *ovar = A.template alloc<T>(ldi), *ovdx = A.template alloc<T>(ldi), *ovdy = A.template alloc<T>(ldi);
+  T *dxn = A.template alloc<T>(ldi), *dxs = A.template alloc<T>(ldi), *dye = A.template alloc<T>(ldi);
+  T *dyw = A.template alloc<T>(ldi), *dep = A.template alloc<T>(ldi);
+  T *ccu = A.template alloc<T>(ldi), *scu = A.template alloc<T>(ldi);
+  std::vector<double> dsea(S.d.begin(), S.d.begin() + S.im);
+  if ((rc = upload<T>(ovar, S.oneovar, ldi, s)) || (rc = upload<T>(ovdx, S.oneovdx, ldi, s)) ||
+      (rc = upload<T>(ovdy, S.oneovdy, ldi, s)) || (rc = upload<T>(dxn, S.dxn, ldi, s)) ||
+      (rc = upload<T>(dxs, S.dxs, ldi, s)) || (rc = upload<T>(dye, S.dye, ldi, s)) ||
+      (rc = upload<T>(dyw, S.dyw, ldi, s)) || (rc = upload<T>(dep, dsea, ldi, s)) ||
+      (rc = upload<T>(ccu, S.ccurv, ldi, s)) || (rc = upload<T>(scu, S.scurv, ldi, s))) {
+    delete D;
+    return rc;
+  }
+  h.oneovar = ovar; h.oneovdx = ovdx; h.oneovdy = ovdy; h.dxn = dxn; h.dxs = dxs; h.dye = dye;
+  h.dyw = dyw; h.depth = dep; h.ccurv = ccu; h.scurv = scu;
+  int32_t *ie = A.template alloc<int32_t>(ldi), *iw = A.template alloc<int32_t>(ldi);
+  int32_t *in = A.template alloc<int32_t>(ldi), *is = A.template alloc<int32_t>(ldi);
+  int32_t *iie = A.template alloc<int32_t>(ldi), *iiw = A.template alloc<int32_t>(ldi);
+  int32_t *iin = A.template alloc<int32_t>(ldi), *iis = A.template alloc<int32_t>(ldi);
+  // padding lanes point at the ghost (aliased) or at point 0 (true maps)
+  if ((rc = upload_int(ie, S.ie, ldi, S.im, s)) || (rc = upload_int(iw, S.iw, ldi, S.im, s)) ||
+      (rc = upload_int(in, S.in, ldi, S.im, s)) || (rc = upload_int(is, S.is, ldi, S.im, s)) ||
+      (rc = upload_int(iie, S.iie, ldi, 0, s)) || (rc = upload_int(iiw, S.iiw, ldi, 0, s)) ||
+      (rc = upload_int(iin, S.iin, ldi, 0, s)) || (rc = upload_int(iis, S.iis, ldi, 0, s))) {
+    delete D;
+    return rc;
+  }
+  h.ie = ie; h.iw = iw; h.in_ = in; h.is_ = is; h.iie = iie; h.iiw = iiw; h.iin = iin; h.iis = iis;
+  h.oc = A.template alloc<int32_t>(ldi);
+  h.oc_rot = A.template alloc<int32_t>(ldi);
+  h.shelt = A.template alloc<T>(ldi);
+  h.fcutoff = A.template alloc<T>(ldi);
+  h.pts = A.template alloc<double>(ldi);
+  h.hs_ice = A.template alloc<double>(ldi);
+  double** pd[] = {&h.ustar, &h.cd, &h.taux, &h.tauy, &h.taux_form, &h.tauy_form, &h.taux_skin,
+                   &h.tauy_skin, &h.taux_diag, &h.tauy_diag, &h.tailatmx, &h.tailatmy,
+                   &h.us1, &h.vs1};
+  for (double** p : pd) *p = A.template alloc<double>(ldi);
+  if (!h.vs1 || !h.hs_ice) {
+    delete D;
+    return fail(UMWM_ERR_NOMEM, "device allocation of point arrays failed");
+  }
+  UMWM_HIP_TRY(hipMemsetAsync(h.oc, 0, ldi * sizeof(int32_t), s));
+  UMWM_HIP_TRY(hipMemsetAsync(h.oc_rot, 0, ldi * sizeof(int32_t), s));
+  for (double** p : pd) UMWM_HIP_TRY(hipMemsetAsync(*p, 0, ldi * sizeof(double), s));
+  UMWM_HIP_TRY(hipMemsetAsync(h.pts, 0, ldi * sizeof(double), s));
+  UMWM_HIP_TRY(hipMemsetAsync(h.hs_ice, 0, ldi * sizeof(double), s));
+  {
+    std::vector<double> u(ldi, 0.0), c(ldi, 0.0);
+    for (int j = 0; j < S.im; ++j) { u[j] = S.ustar[j]; c[j] = S.cd[j]; }
+    UMWM_HIP_TRY(hipMemcpyAsync(h.ustar, u.data(), ldi * sizeof(double), hipMemcpyHostToDevice, s));
+    UMWM_HIP_TRY(hipMemcpyAsync(h.cd, c.data(), ldi * sizeof(double), hipMemcpyHostToDevice, s));
+    UMWM_HIP_TRY(hipStreamSynchronize(s));
+  }
+  if ((rc = dfill<T>(h.shelt, ldi, T(0), s)) || (rc = dfill<T>(h.fcutoff, ldi, T(0), s))) {
+    delete D;
+    return rc;
+  }
+  // spectrum: e = tiny(e) everywhere, ghost included (src/umwm_init.F90:1182)
+  const T tiny = sizeof(T) == 8 ? T(DBL_MIN) : T(FLT_MIN);
+  if ((rc = dfill<T>(h.e, ns, tiny, s)) || (rc = dfill<T>(h.ef, ns, T(0), s)) ||
+      (rc = dfill<T>(h.es, ns, tiny, s))) {
+    delete D;
+    return rc;
+  }
+  // reduction scratch and scalars
+  h.nblocks = (S.im + kLanes - 1) / kLanes;
+  h.block0 = 0;
+  h.part_min = A.template alloc<double>(std::max(h.nblocks, 1));
+  h.part_max = A.template alloc<double>(std::max(h.nblocks, 1));
+  h.sc = A.template alloc<Scalars>(1);
+  if (!h.sc) {
+    delete D;
+    return fail(UMWM_ERR_NOMEM, "device allocation failed");
+  }
+  Scalars sc0{};
+  sc0.anycur = (!P.currents && (P.uc0 != 0 || P.vc0 != 0)) ? 1 : 0;
+  if (P.currents) {
+    for (int q = 0; q < S.imm; ++q)
+      if (S.uc[q] != 0 || S.vc[q] != 0) sc0.anycur = 1;
+  }
+  UMWM_HIP_TRY(hipMemcpyAsync(h.sc, &sc0, sizeof(Scalars), hipMemcpyHostToDevice, s));
+  UMWM_HIP_TRY(hipHostMalloc((void**)&D->sc_host, sizeof(Scalars), hipHostMallocDefault));
+  // diag and Stokes outputs
+  double** dg[] = {&D->diag.ht, &D->diag.mwp, &D->diag.mwd, &D->diag.mss, &D->diag.mwl,
+                   &D->diag.momx, &D->diag.momy, &D->diag.cgmxx, &D->diag.cgmxy, &D->diag.cgmyy,
+                   &D->diag.dwd, &D->diag.dwl, &D->diag.dwp, &D->diag.dcp0, &D->diag.dcg0,
+                   &D->diag.dcp, &D->diag.dcg};
+  for (double** p : dg) {
+    *p = A.template alloc<double>(ldi);
+    UMWM_HIP_TRY(hipMemsetAsync(*p, 0, ldi * sizeof(double), s));
+  }
+  const int lm = std::max(S.lm, 1);
+  double* dl = A.template alloc<double>(lm);
+  std::vector<double> dlh(lm, 0.0);
+  for (int l = 0; l < S.lm; ++l) dlh[l] = S.depth_l[l];
+  UMWM_HIP_TRY(hipMemcpyAsync(dl, dlh.data(), lm * sizeof(double), hipMemcpyHostToDevice, s));
+  D->stokes.depth = dl;
+  // us/vs carry om extra rows used as scratch for directional sums
+  const size_t nus = (size_t)(lm + std::max(om, 1)) * ldi;
+  D->stokes.us = A.template alloc<double>(nus);
+  D->stokes.vs = A.template alloc<double>(nus);
+  D->stokes.ds = A.template alloc<double>(ldi);
+  if (!D->stokes.ds) {
+    delete D;
+    return fail(UMWM_ERR_NOMEM, "device allocation of Stokes arrays failed");
+  }
+  UMWM_HIP_TRY(hipMemsetAsync(D->stokes.us, 0, nus * sizeof(double), s));
+  UMWM_HIP_TRY(hipMemsetAsync(D->stokes.vs, 0, nus * sizeof(double), s));
+  UMWM_HIP_TRY(hipMemsetAsync(D->stokes.ds, 0, ldi * sizeof(double), s));
+  // forcing levels
+  ForcingDev<T>& F = D->fh;
+  const size_t n2 = (size_t)S.mm * S.nm;
+  F.winds = P.winds; F.currents = P.currents; F.air_density = P.air_density;
+  F.water_density = P.water_density; F.seaice = P.seaice;
+  int32_t *mi = A.template alloc<int32_t>(ldq), *ni = A.template alloc<int32_t>(ldq);
+  if ((rc = upload_int(mi, S.mi, ldq, 1, s)) || (rc = upload_int(ni, S.ni, ldq, 1, s))) {
+    delete D;
+    return rc;
+  }
+  F.mi = mi; F.ni = ni;
+  const bool anyf = P.winds || P.currents || P.air_density || P.water_density || P.seaice;
+  if (anyf) {
+    D->uwb = A.template alloc<T>(n2); D->uwf = A.template alloc<T>(n2);
+    D->vwb = A.template alloc<T>(n2); D->vwf = A.template alloc<T>(n2);
+    D->ucb = A.template alloc<T>(n2); D->ucf = A.template alloc<T>(n2);
+    D->vcb = A.template alloc<T>(n2); D->vcf = A.template alloc<T>(n2);
+    D->ficeb = A.template alloc<T>(n2); D->ficef = A.template alloc<T>(n2);
+    D->rhoab = A.template alloc<T>(ldq); D->rhoaf = A.template alloc<T>(ldq);
+    D->rhowb = A.template alloc<T>(ldq); D->rhowf = A.template alloc<T>(ldq);
+    if (!D->rhowf) {
+      delete D;
+      return fail(UMWM_ERR_NOMEM, "device allocation of forcing levels failed");
+    }
+    if ((rc = upload<T>(D->uwf, S.uwf, n2, s)) || (rc = upload<T>(D->vwf, S.vwf, n2, s)) ||
+        (rc = upload<T>(D->ucf, S.ucf, n2, s)) || (rc = upload<T>(D->vcf, S.vcf, n2, s)) ||
+        (rc = upload<T>(D->ficef, S.ficef, n2, s)) || (rc = upload<T>(D->rhoaf, S.rhoaf, ldq, s)) ||
+        (rc = upload<T>(D->rhowf, S.rhowf, ldq, s)) || (rc = upload<T>(D->uwb, S.uwf, n2, s)) ||
+        (rc = upload<T>(D->vwb, S.vwf, n2, s)) || (rc = upload<T>(D->ucb, S.ucf, n2, s)) ||
+        (rc = upload<T>(D->vcb, S.vcf, n2, s)) || (rc = upload<T>(D->ficeb, S.ficef, n2, s)) ||
+        (rc = upload<T>(D->rhoab, S.rhoaf, ldq, s)) || (rc = upload<T>(D->rhowb, S.rhowf, ldq, s))) {
+      delete D;
+      return rc;
+    }
+    F.uwb = D->uwb; F.uwf = D->uwf; F.vwb = D->vwb; F.vwf = D->vwf;
+    F.ucb = D->ucb; F.ucf = D->ucf; F.vcb = D->vcb; F.vcf = D->vcf;
+    F.ficeb = D->ficeb; F.ficef = D->ficef;
+    F.rhoab = D->rhoab; F.rhoaf = D->rhoaf; F.rhowb = D->rhowb; F.rhowf = D->rhowf;
+  }
+  // device copy of the domain struct and the block task list
+  D->d = A.template alloc<DomainDev<T>>(1);
+  const int nb = h.nblocks;
+  D->tasks = A.template alloc<BlockTask>(std::max(nb, 1));
+  D->ntasks = nb;
+  {
+    std::vector<BlockTask> tk(nb);
+    std::vector<int> ord = xcd_order(nb);
+    for (int b = 0; b < nb; ++b) tk[b] = BlockTask{0, ord[b] * kLanes, ord[b], 0};
+    UMWM_HIP_TRY(hipMemcpyAsync(D->tasks, tk.data(), nb * sizeof(BlockTask), hipMemcpyHostToDevice, s));
+  }
+  UMWM_HIP_TRY(hipMemcpyAsync(D->d, &h, sizeof(DomainDev<T>), hipMemcpyHostToDevice, s));
+  D->tile = choose_tile(pm);
+  for (auto& e : D->ev) UMWM_HIP_TRY(hipEventCreate(&e));
+  UMWM_HIP_TRY(hipStreamSynchronize(s));
+  // stokes_drift('init') on the cold-start spectrum, as umwm_initialize does
+  if ((rc = launch_stokes(*D))) { delete D; return rc; }
+  if ((rc = launch_hs_ice(*D))) { delete D; return rc; }
+  UMWM_HIP_TRY(hipStreamSynchronize(s));
+  out = D;
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// substeps
+// ---------------------------------------------------------------------
+template <typename T>
+int read_scalars(DomainImpl<T>& D) {
+  UMWM_HIP_TRY(hipMemcpyAsync(D.sc_host, D.h.sc, sizeof(Scalars), hipMemcpyDeviceToHost, D.stream));
+  UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  return 0;
+}
+
+template <typename T>
+bool any_forcing(const DomainImpl<T>& D) {
+  const umwm_params& P = D.S.P;
+  return P.winds || P.currents || P.air_density || P.water_density || P.seaice;
+}
+
+template <typename T>
+int fused_substep(DomainImpl<T>& D) {
+  int rc;
+  if (any_forcing(D) && (rc = launch_forcinginterpolate(D))) return rc;
+  const int rotation = (D.S.P.currents || D.first || !D.rot_valid) ? 1 : 0;
+  rc = launch_fused<T>(D.d, D.tasks, D.ntasks, D.first ? 1 : 0, rotation, D.tile, D.stream,
+                       D.timing ? D.ev : nullptr);
+  if (rc) return rc;
+  D.rot_valid = true;
+  if (D.first) {
+    if ((rc = launch_diag(D))) return rc;
+    D.first = false;
+  }
+  if (D.timing) {
+    UMWM_HIP_TRY(hipEventSynchronize(D.ev[3]));
+    for (int q = 0; q < 3; ++q) {
+      float ms = 0;
+      UMWM_HIP_TRY(hipEventElapsedTime(&ms, D.ev[q], D.ev[q + 1]));
+      D.ms[q] += ms;
+      D.launches[q] += 1;
+    }
+  }
+  return 0;
+}
+
+template <typename T>
+int run_dtg_impl(DomainImpl<T>& D, int32_t* nsub) {
+  hipLaunchKernelGGL(k_begin_dtg, dim3(1), dim3(64), 0, D.stream, D.h.sc);
+  UMWM_HIP_TRY(hipGetLastError());
+  int n = 0, rc;
+  for (;;) {
+    if ((rc = fused_substep(D))) return rc;
+    ++n;
+    if ((rc = read_scalars(D))) return rc;
+    if (!(D.sc_host->sumt < D.S.P.dtg)) break;
+    if (n > 10000000) return fail(UMWM_ERR_STATE, "run_dtg: substep count runaway");
+  }
+  if (D.firstdtg) D.firstdtg = false;
+  if (D.S.P.stokes && (rc = launch_stokes(D))) return rc;
+  if ((rc = launch_diag(D))) return rc;
+  UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  if (nsub) *nsub = n;
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// field exchange
+// ---------------------------------------------------------------------
+enum FieldKind { F_SPEC, F_TAB, F_PT_T, F_PT_D, F_PT_I, F_UNR_T, F_HOST_MAP, F_HOST_2D, F_HOST_UNR, F_STOKES, F_HOST_SEA };
+
+struct FieldRef {
+  FieldKind kind;
+  void* ptr = nullptr;
+  const std::vector<int>* imap = nullptr;
+  const std::vector<double>* dmap = nullptr;
+};
+
+template <typename T>
+bool find_field(DomainImpl<T>& D, const std::string& n, FieldRef& r) {
+  DomainDev<T>& h = D.h;
+  const HostSetup& S = D.S;
+  auto set = [&](FieldKind k, void* p) { r.kind = k; r.ptr = p; return true; };
+  // spectra
+  if (n == "e") return set(F_SPEC, h.e);
+  if (n == "ef") return set(F_SPEC, h.ef);
+  if (n == "es") return set(F_SPEC, h.es);
+  if (n == "ssin" || n == "sds" || n == "snl" || n == "dummy" || n == "rotl" || n == "rotr" ||
+      n == "sdt" || n == "sice") {
+    if (ensure_ops(D)) return false;
+    if (n == "ssin") return set(F_SPEC, h.ssin);
+    if (n == "sds") return set(F_SPEC, h.sds);
+    if (n == "snl") return set(F_SPEC, h.snl);
+    if (n == "dummy") return set(F_SPEC, h.dummy);
+    if (n == "rotl") return set(F_SPEC, h.rotl);
+    if (n == "rotr") return set(F_SPEC, h.rotr);
+    if (n == "sdt") return set(F_TAB, h.sdt);
+    if (n == "sice") return set(F_TAB, h.sice);
+  }
+  // tables
+  const std::pair<const char*, const T*> tabs[] = {
+      {"k", h.k}, {"kdk", h.kdk}, {"k3dk", h.k3dk}, {"k4", h.k4}, {"oneoverk4", h.oneoverk4},
+      {"cp0", h.cp0}, {"cg0", h.cg0}, {"invcp0", h.invcp0}, {"fkovg", h.fkovg},
+      {"logl2overz", h.logl2overz}, {"cothkd", h.cothkd}, {"sbf", h.sbf}, {"sdv", h.sdv},
+      {"snl_arg", h.snl_arg}, {"bf1_renorm", h.bf1r}, {"bf2_renorm", h.bf2r}, {"dwn", h.dwn}};
+  for (auto& t : tabs)
+    if (n == t.first) return set(F_TAB, (void*)t.second);
+  const std::pair<const char*, T*> unr[] = {
+      {"wspd", h.wspd}, {"wdir", h.wdir}, {"uc", h.uc}, {"vc", h.vc}, {"rhoa", h.rhoa},
+      {"rhow", h.rhow}, {"rhorat", h.rhorat}, {"fice", h.fice}};
+  for (auto& t : unr)
+    if (n == t.first) return set(F_UNR_T, t.second);
+  const std::pair<const char*, const T*> ptt[] = {
+      {"shelt", h.shelt}, {"fcutoff", h.fcutoff}, {"oneovar", h.oneovar}, {"oneovdx", h.oneovdx},
+      {"oneovdy", h.oneovdy}, {"dxn", h.dxn}, {"dxs", h.dxs}, {"dye", h.dye}, {"dyw", h.dyw}};
+  for (auto& t : ptt)
+    if (n == t.first) return set(F_PT_T, (void*)t.second);
+  const std::pair<const char*, double*> ptd[] = {
+      {"ustar", h.ustar}, {"cd", h.cd}, {"taux", h.taux}, {"tauy", h.tauy},
+      {"taux_form", h.taux_form}, {"tauy_form", h.tauy_form}, {"taux_skin", h.taux_skin},
+      {"tauy_skin", h.tauy_skin}, {"taux_diag", h.taux_diag}, {"tauy_diag", h.tauy_diag},
+      {"tailatmx", h.tailatmx}, {"tailatmy", h.tailatmy}, {"physics_time_step", h.pts},
+      {"ht", D.diag.ht}, {"mwp", D.diag.mwp}, {"mwd", D.diag.mwd}, {"mss", D.diag.mss},
+      {"mwl", D.diag.mwl}, {"momx", D.diag.momx}, {"momy", D.diag.momy},
+      {"cgmxx", D.diag.cgmxx}, {"cgmxy", D.diag.cgmxy}, {"cgmyy", D.diag.cgmyy},
+      {"dwd", D.diag.dwd}, {"dwl", D.diag.dwl}, {"dwp", D.diag.dwp}, {"dcp0", D.diag.dcp0},
+      {"dcg0", D.diag.dcg0}, {"dcp", D.diag.dcp}, {"dcg", D.diag.dcg}, {"ds", D.stokes.ds},
+      {"hs_ice", h.hs_ice}};
+  for (auto& t : ptd)
+    if (n == t.first) return set(F_PT_D, t.second);
+  if (n == "oc") return set(F_PT_I, h.oc);
+  if (n == "oc_rot") return set(F_PT_I, h.oc_rot);
+  if (n == "us") return set(F_STOKES, D.stokes.us);
+  if (n == "vs") return set(F_STOKES, D.stokes.vs);
+  // host-side setup products
+  const std::pair<const char*, const std::vector<int>*> maps[] = {
+      {"iw", &S.iw}, {"ie", &S.ie}, {"is", &S.is}, {"in", &S.in},
+      {"iiw", &S.iiw}, {"iie", &S.iie}, {"iis", &S.iis}, {"iin", &S.iin}};
+  for (auto& t : maps)
+    if (n == t.first) { r.kind = F_HOST_MAP; r.imap = t.second; return true; }
+  if (n == "mi") { r.kind = F_HOST_UNR; r.imap = &S.mi; return true; }
+  if (n == "ni") { r.kind = F_HOST_UNR; r.imap = &S.ni; return true; }
+  if (n == "mask") { r.kind = F_HOST_2D; r.imap = &S.mask; return true; }
+  if (n == "ii") { r.kind = F_HOST_2D; r.imap = &S.ii; return true; }
+  const std::pair<const char*, const std::vector<double>*> d2[] = {
+      {"dx_2d", &S.dx2}, {"dy_2d", &S.dy2}, {"curv", &S.curv2}, {"d_2d", &S.d2},
+      {"lon", &S.lon}, {"lat", &S.lat}};
+  for (auto& t : d2)
+    if (n == t.first) { r.kind = F_HOST_2D; r.dmap = t.second; return true; }
+  const std::pair<const char*, const std::vector<double>*> du[] = {
+      {"dx", &S.dx}, {"dy", &S.dy}, {"d", &S.d}};
+  for (auto& t : du)
+    if (n == t.first) { r.kind = F_HOST_UNR; r.dmap = t.second; return true; }
+  if (n == "ar") { r.kind = F_HOST_SEA; r.dmap = &S.ar; return true; }
+  return false;
+}
+
+template <typename T>
+int64_t field_size(const DomainImpl<T>& D, FieldKind k) {
+  const auto& h = D.h;
+  switch (k) {
+    case F_SPEC: return (int64_t)h.om * h.pm * h.im;
+    case F_TAB: return (int64_t)h.om * h.im;
+    case F_PT_T: case F_PT_D: case F_PT_I: case F_HOST_MAP: case F_HOST_SEA: return h.im;
+    case F_UNR_T: case F_HOST_UNR: return h.imm;
+    case F_HOST_2D: return (int64_t)h.mm * h.nm;
+    case F_STOKES: return (int64_t)h.im * std::max(h.lm, 1);
+  }
+  return 0;
+}
+
+template <typename T>
+int get_field_impl(DomainImpl<T>& D, const std::string& name, double* buf, int64_t n) {
+  FieldRef r;
+  if (!find_field(D, name, r)) return fail(UMWM_ERR_UNKNOWN_FIELD, "unknown field: " + name);
+  const int64_t want = field_size(D, r.kind);
+  if (n != want)
+    return fail(UMWM_ERR_SIZE, "field " + name + " has " + std::to_string(want) + " values");
+  const auto& h = D.h;
+  const size_t ldi = h.ldi, im = h.im;
+  UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  switch (r.kind) {
+    case F_SPEC: {
+      // device [o][p][ldi] -> reference (om,pm,im) column-major
+      const size_t np = (size_t)h.om * h.pm;
+      std::vector<T> tmp(np * ldi);
+      UMWM_HIP_TRY(hipMemcpy(tmp.data(), r.ptr, tmp.size() * sizeof(T), hipMemcpyDeviceToHost));
+      // device op = o*pm + p; reference index o + om*(p + pm*j)
+      for (size_t o = 0; o < (size_t)h.om; ++o)
+        for (size_t p = 0; p < (size_t)h.pm; ++p) {
+          const T* src = tmp.data() + (o * h.pm + p) * ldi;
+          for (size_t j = 0; j < im; ++j) buf[j * np + p * h.om + o] = (double)src[j];
+        }
+      return 0;
+    }
+    case F_TAB: {
+      std::vector<T> tmp((size_t)h.om * ldi);
+      UMWM_HIP_TRY(hipMemcpy(tmp.data(), r.ptr, tmp.size() * sizeof(T), hipMemcpyDeviceToHost));
+      for (size_t o = 0; o < (size_t)h.om; ++o)
+        for (size_t j = 0; j < im; ++j) buf[j * h.om + o] = (double)tmp[o * ldi + j];
+      return 0;
+    }
+    case F_PT_T: case F_UNR_T: {
+      std::vector<T> tmp(n);
+      UMWM_HIP_TRY(hipMemcpy(tmp.data(), r.ptr, n * sizeof(T), hipMemcpyDeviceToHost));
+      for (int64_t q = 0; q < n; ++q) buf[q] = (double)tmp[q];
+      return 0;
+    }
+    case F_PT_D:
+      UMWM_HIP_TRY(hipMemcpy(buf, r.ptr, n * sizeof(double), hipMemcpyDeviceToHost));
+      return 0;
+    case F_PT_I: {
+      std::vector<int32_t> tmp(n);
+      UMWM_HIP_TRY(hipMemcpy(tmp.data(), r.ptr, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+      for (int64_t q = 0; q < n; ++q) buf[q] = tmp[q] + 1.0;  // 1-based bin index
+      return 0;
+    }
+    case F_STOKES: {
+      for (int l = 0; l < std::max(h.lm, 1); ++l)
+        UMWM_HIP_TRY(hipMemcpy(buf + (size_t)l * im, (double*)r.ptr + (size_t)l * ldi,
+                               im * sizeof(double), hipMemcpyDeviceToHost));
+      return 0;
+    }
+    case F_HOST_MAP: {
+      // aliased maps: ghost -> 0; true maps: 1-based
+      const bool aliased = name.size() == 2;
+      for (int64_t q = 0; q < n; ++q) {
+        const int v = (*r.imap)[q];
+        buf[q] = aliased ? (v == (int)im ? 0.0 : v + 1.0) : v + 1.0;
+      }
+      return 0;
+    }
+    case F_HOST_UNR: case F_HOST_2D: case F_HOST_SEA:
+      for (int64_t q = 0; q < n; ++q) buf[q] = r.imap ? (double)(*r.imap)[q] : (*r.dmap)[q];
+      return 0;
+  }
+  return fail(UMWM_ERR_UNKNOWN_FIELD, name);
+}
+
+template <typename T>
+int set_field_impl(DomainImpl<T>& D, const std::string& name, const double* buf, int64_t n) {
+  FieldRef r;
+  if (!find_field(D, name, r)) return fail(UMWM_ERR_UNKNOWN_FIELD, "unknown field: " + name);
+  const int64_t want = field_size(D, r.kind);
+  if (n != want)
+    return fail(UMWM_ERR_SIZE, "field " + name + " has " + std::to_string(want) + " values");
+  const auto& h = D.h;
+  const size_t ldi = h.ldi, im = h.im;
+  UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  switch (r.kind) {
+    case F_SPEC: {
+      const size_t np = (size_t)h.om * h.pm;
+      std::vector<T> tmp(np * ldi);
+      UMWM_HIP_TRY(hipMemcpy(tmp.data(), r.ptr, tmp.size() * sizeof(T), hipMemcpyDeviceToHost));
+      for (size_t o = 0; o < (size_t)h.om; ++o)
+        for (size_t p = 0; p < (size_t)h.pm; ++p) {
+          T* dst = tmp.data() + (o * h.pm + p) * ldi;
+          for (size_t j = 0; j < im; ++j) dst[j] = T(buf[j * np + p * h.om + o]);
+        }
+      UMWM_HIP_TRY(hipMemcpy(r.ptr, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+      break;
+    }
+    case F_PT_T: case F_UNR_T: {
+      std::vector<T> tmp(n);
+      for (int64_t q = 0; q < n; ++q) tmp[q] = T(buf[q]);
+      UMWM_HIP_TRY(hipMemcpy(r.ptr, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+      break;
+    }
+    case F_PT_D:
+      UMWM_HIP_TRY(hipMemcpy(r.ptr, buf, n * sizeof(double), hipMemcpyHostToDevice));
+      break;
+    default:
+      return fail(UMWM_ERR_ARGUMENT, "field " + name + " is read-only");
+  }
+  if (name == "e") {
+    int rc = launch_hs_ice(D);
+    if (rc) return rc;
+    UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// batch
+// ---------------------------------------------------------------------
+template <typename T>
+struct BatchImpl {
+  std::vector<DomainImpl<T>*> doms;
+  DeviceArena arena;
+  DomainDev<T>* d_doms = nullptr;
+  BlockTask* tasks = nullptr;
+  int ntasks = 0;
+  Tile tile;
+  hipStream_t stream = nullptr;
+  Scalars** d_scs = nullptr;
+  ScalarsView* d_view = nullptr;
+  ScalarsView* h_view = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool rot_every = false;
+  ~BatchImpl() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (h_view) (void)hipHostFree(h_view);
+    arena.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+}  // namespace umwm
+
+// =====================================================================
+// extern "C"
+// =====================================================================
+using namespace umwm;
+
+struct umwm_domain {
+  DomainBase* base = nullptr;
+};
+
+struct umwm_batch {
+  int precision = 8;
+  void* impl = nullptr;
+};
+
+namespace {
+
+template <typename Fn>
+int dispatch(umwm_domain* dom, Fn&& fn) {
+  if (!dom || !dom->base) return fail(UMWM_ERR_ARGUMENT, "null domain");
+  if (dom->base->precision == 4) return fn(*static_cast<DomainImpl<float>*>(dom->base));
+  return fn(*static_cast<DomainImpl<double>*>(dom->base));
+}
+
+template <typename T>
+int sync_and_check(DomainImpl<T>& D) {
+  UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* umwm_hip_error_string(int status) {
+  switch (status) {
+    case UMWM_OK: return "ok";
+    case UMWM_ERR_ARGUMENT: return "bad argument";
+    case UMWM_ERR_HIP: return "HIP runtime error";
+    case UMWM_ERR_NOMEM: return "out of memory";
+    case UMWM_ERR_STATE: return "call made in the wrong state";
+    case UMWM_ERR_UNKNOWN_FIELD: return "unknown field";
+    case UMWM_ERR_SIZE: return "buffer size mismatch";
+  }
+  return "unknown status";
+}
+
+const char* umwm_hip_last_error(void) { return g_last_error.c_str(); }
+
+int umwm_hip_device_count(int* count) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  if (count) *count = n;
+  return 0;
+}
+
+int umwm_hip_create(const umwm_params* params, const umwm_grid_input* grid,
+                    const umwm_forcing_input* forcing0, int precision, umwm_domain** out) {
+  if (!params || !out) return fail(UMWM_ERR_ARGUMENT, "null argument");
+  if (precision != UMWM_FP32 && precision != UMWM_FP64)
+    return fail(UMWM_ERR_ARGUMENT, "precision must be 4 or 8");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+    return fail(UMWM_ERR_HIP, "no HIP device visible");
+  auto* dom = new umwm_domain();
+  int rc;
+  if (precision == UMWM_FP32) {
+    DomainImpl<float>* D = nullptr;
+    rc = create_impl<float>(*params, grid, forcing0, D);
+    dom->base = D;
+  } else {
+    DomainImpl<double>* D = nullptr;
+    rc = create_impl<double>(*params, grid, forcing0, D);
+    dom->base = D;
+  }
+  if (rc) {
+    delete dom;
+    return rc;
+  }
+  *out = dom;
+  return 0;
+}
+
+int umwm_hip_destroy(umwm_domain* dom) {
+  if (!dom) return 0;
+  delete dom->base;
+  delete dom;
+  return 0;
+}
+
+int umwm_hip_dims(const umwm_domain* dom, int32_t out[8]) {
+  if (!dom || !dom->base || !out) return fail(UMWM_ERR_ARGUMENT, "null argument");
+  const HostSetup& S = dom->base->S;
+  out[0] = S.im; out[1] = S.imm; out[2] = S.om; out[3] = S.pm;
+  out[4] = S.mm; out[5] = S.nm; out[6] = S.lm; out[7] = dom->base->precision;
+  return 0;
+}
+
+int umwm_hip_init_jonswap(umwm_domain* dom, const umwm_jonswap* js) {
+  if (!js) return fail(UMWM_ERR_ARGUMENT, "null jonswap");
+  return dispatch(dom, [&](auto& D) {
+    if (js->fp <= 0 && !js->fp_from_wind) return fail(UMWM_ERR_ARGUMENT, "fp must be > 0");
+    int rc = launch_jonswap(D, *js);
+    if (!rc) rc = launch_hs_ice(D);
+    if (!rc) rc = sync_and_check(D);
+    return rc;
+  });
+}
+
+int umwm_hip_stokes_drift(umwm_domain* dom, int init) {
+  (void)init;  // transfer coefficients are evaluated on the fly
+  return dispatch(dom, [&](auto& D) {
+    int rc = launch_stokes(D);
+    return rc ? rc : sync_and_check(D);
+  });
+}
+
+int umwm_hip_forcinginterpolate(umwm_domain* dom) {
+  return dispatch(dom, [&](auto& D) {
+    int rc = launch_forcinginterpolate(D);
+    return rc ? rc : sync_and_check(D);
+  });
+}
+
+int umwm_hip_forcinginput(umwm_domain* dom, const umwm_forcing_input* nx) {
+  if (!nx) return fail(UMWM_ERR_ARGUMENT, "null forcing");
+  return dispatch(dom, [&](auto& D) {
+    using T = std::remove_reference_t<decltype(*D.h.e)>;
+    const umwm_params& P = D.S.P;
+    if (!any_forcing(D)) return 0;
+    if ((P.winds && (!nx->uw || !nx->vw)) || (P.currents && (!nx->uc || !nx->vc)) ||
+        (P.air_density && !nx->rhoa) || (P.water_density && !nx->rhow) ||
+        (P.seaice && !nx->fice))
+      return fail(UMWM_ERR_ARGUMENT, "forcing snapshot lacks a switched field");
+    UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+    const size_t n2 = (size_t)D.S.mm * D.S.nm, ldq = D.S.ldq;
+    // level n <- level n+1 (src/umwm_forcing.F90:11-37)
+    std::swap(D.uwb, D.uwf); std::swap(D.vwb, D.vwf); std::swap(D.ucb, D.ucf);
+    std::swap(D.vcb, D.vcf); std::swap(D.ficeb, D.ficef); std::swap(D.rhoab, D.rhoaf);
+    std::swap(D.rhowb, D.rhowf);
+    int rc = 0;
+    load_forcing_level(D.S, *nx);
+    if (P.winds && ((rc = upload<T>(D.uwf, D.S.uwf, n2, D.stream)) ||
+                    (rc = upload<T>(D.vwf, D.S.vwf, n2, D.stream))))
+      return rc;
+    if (P.currents && ((rc = upload<T>(D.ucf, D.S.ucf, n2, D.stream)) ||
+                       (rc = upload<T>(D.vcf, D.S.vcf, n2, D.stream))))
+      return rc;
+    if (P.seaice && (rc = upload<T>(D.ficef, D.S.ficef, n2, D.stream))) return rc;
+    if (P.air_density && (rc = upload<T>(D.rhoaf, D.S.rhoaf, ldq, D.stream))) return rc;
+    if (P.water_density && (rc = upload<T>(D.rhowf, D.S.rhowf, ldq, D.stream))) return rc;
+    ForcingDev<T>& F = D.fh;
+    F.uwb = D.uwb; F.uwf = D.uwf; F.vwb = D.vwb; F.vwf = D.vwf;
+    F.ucb = D.ucb; F.ucf = D.ucf; F.vcb = D.vcb; F.vcf = D.vcf;
+    F.ficeb = D.ficeb; F.ficef = D.ficef;
+    F.rhoab = D.rhoab; F.rhoaf = D.rhoaf; F.rhowb = D.rhowb; F.rhowf = D.rhowf;
+    return 0;
+  });
+}
+
+#define OP_ENTRY(name, launcher)                                   \
+  int name(umwm_domain* dom) {                                     \
+    return dispatch(dom, [&](auto& D) {                            \
+      int rc = ensure_ops(D);                                      \
+      if (!rc) rc = launcher(D);                                   \
+      if (!rc) rc = sync_and_check(D);                             \
+      return rc;                                                   \
+    });                                                            \
+  }
+
+OP_ENTRY(umwm_hip_sin_d12, launch_op_sin)
+OP_ENTRY(umwm_hip_sds_d12, launch_op_sds)
+OP_ENTRY(umwm_hip_snl_d12, launch_op_snl)
+OP_ENTRY(umwm_hip_s_ice, launch_op_sice)
+OP_ENTRY(umwm_hip_propagation, launch_op_propagation)
+OP_ENTRY(umwm_hip_update, launch_op_update)
+
+int umwm_hip_source(umwm_domain* dom) {
+  return dispatch(dom, [&](auto& D) {
+    int rc = ensure_ops(D);
+    if (!rc) rc = launch_op_source(D);
+    if (!rc) rc = sync_and_check(D);
+    return rc;
+  });
+}
+
+int umwm_hip_refraction(umwm_domain* dom) {
+  return dispatch(dom, [&](auto& D) {
+    int rc = ensure_ops(D);
+    if (!rc) rc = launch_op_refraction(D);
+    if (!rc) rc = sync_and_check(D);
+    return rc;
+  });
+}
+
+int umwm_hip_stress(umwm_domain* dom, const char* option) {
+  if (!option) return fail(UMWM_ERR_ARGUMENT, "null option");
+  const std::string opt(option, strnlen(option, 3));
+  if (opt == "ocn") return fail(UMWM_ERR_ARGUMENT, "stress('ocn') is not implemented");
+  if (opt != "atm") return fail(UMWM_ERR_ARGUMENT, "stress option must be 'atm' or 'ocn'");
+  return dispatch(dom, [&](auto& D) {
+    int rc = ensure_ops(D);
+    if (!rc) rc = launch_op_stress_atm(D);
+    if (!rc) rc = sync_and_check(D);
+    return rc;
+  });
+}
+
+int umwm_hip_diag(umwm_domain* dom) {
+  return dispatch(dom, [&](auto& D) {
+    int rc = launch_diag(D);
+    return rc ? rc : sync_and_check(D);
+  });
+}
+
+int umwm_hip_end_first(umwm_domain* dom) {
+  return dispatch(dom, [&](auto& D) {
+    D.first = false;
+    return 0;
+  });
+}
+
+int umwm_hip_substep(umwm_domain* dom, double* sumt, double* dts) {
+  return dispatch(dom, [&](auto& D) {
+    int rc = fused_substep(D);
+    if (!rc) rc = read_scalars(D);
+    if (rc) return rc;
+    if (sumt) *sumt = D.sc_host->sumt;
+    if (dts) *dts = D.sc_host->dts;
+    return 0;
+  });
+}
+
+int umwm_hip_run_dtg(umwm_domain* dom, int32_t* nsub) {
+  return dispatch(dom, [&](auto& D) { return run_dtg_impl(D, nsub); });
+}
+
+int umwm_hip_get_field(umwm_domain* dom, const char* name, double* buf, int64_t n) {
+  if (!name || !buf) return fail(UMWM_ERR_ARGUMENT, "null argument");
+  return dispatch(dom, [&](auto& D) { return get_field_impl(D, name, buf, n); });
+}
+
+int umwm_hip_set_field(umwm_domain* dom, const char* name, const double* buf, int64_t n) {
+  if (!name || !buf) return fail(UMWM_ERR_ARGUMENT, "null argument");
+  return dispatch(dom, [&](auto& D) { return set_field_impl(D, name, buf, n); });
+}
+
+int64_t umwm_hip_field_size(const umwm_domain* dom, const char* name) {
+  if (!dom || !dom->base || !name) return 0;
+  umwm_domain* m = const_cast<umwm_domain*>(dom);
+  int64_t out = 0;
+  dispatch(m, [&](auto& D) {
+    FieldRef r;
+    if (find_field(D, name, r)) out = field_size(D, r.kind);
+    return 0;
+  });
+  return out;
+}
+
+int umwm_hip_get_scalar(umwm_domain* dom, const char* name, double* v) {
+  if (!name || !v) return fail(UMWM_ERR_ARGUMENT, "null argument");
+  return dispatch(dom, [&](auto& D) {
+    const std::string n(name);
+    const HostSetup& S = D.S;
+    if (n == "dtamin") { *v = S.dtamin; return 0; }
+    if (n == "cfllim") { *v = S.cfllim; return 0; }
+    if (n == "cgmax") { *v = S.cgmax; return 0; }
+    if (n == "mindelx") { *v = S.mindelx; return 0; }
+    if (n == "dlnf") { *v = S.dlnf; return 0; }
+    if (n == "dth") { *v = S.dth; return 0; }
+    if (n == "bf1") { *v = S.bf1; return 0; }
+    if (n == "bf2") { *v = S.bf2; return 0; }
+    if (n == "first") { *v = D.first ? 1 : 0; return 0; }
+    int rc = read_scalars(D);
+    if (rc) return rc;
+    const Scalars& s = *D.sc_host;
+    if (n == "sumt") { *v = s.sumt; return 0; }
+    if (n == "dts") { *v = s.dts; return 0; }
+    if (n == "dta") { *v = s.dta; return 0; }
+    if (n == "dtr") { *v = s.dtr; return 0; }
+    if (n == "dtr_temp") { *v = s.dtr_temp; return 0; }
+    if (n == "maxrot") { *v = s.maxrot; return 0; }
+    if (n == "nsub") { *v = s.nsub; return 0; }
+    if (n == "zerocurrents") { *v = s.anycur ? 0 : 1; return 0; }
+    return fail(UMWM_ERR_UNKNOWN_FIELD, "unknown scalar: " + n);
+  });
+}
+
+int umwm_hip_set_scalar(umwm_domain* dom, const char* name, double v) {
+  if (!name) return fail(UMWM_ERR_ARGUMENT, "null argument");
+  return dispatch(dom, [&](auto& D) {
+    const std::string n(name);
+    if (n == "first") { D.first = v != 0; return 0; }
+    int rc = read_scalars(D);
+    if (rc) return rc;
+    Scalars s = *D.sc_host;
+    if (n == "sumt") s.sumt = v;
+    else if (n == "dts") s.dts = v;
+    else if (n == "dtr_temp") s.dtr_temp = v;
+    else return fail(UMWM_ERR_UNKNOWN_FIELD, "scalar not settable: " + n);
+    UMWM_HIP_TRY(hipMemcpy(D.h.sc, &s, sizeof(Scalars), hipMemcpyHostToDevice));
+    return 0;
+  });
+}
+
+int umwm_hip_set_timing(umwm_domain* dom, int enable) {
+  return dispatch(dom, [&](auto& D) {
+    D.timing = enable != 0;
+    return 0;
+  });
+}
+
+int umwm_hip_get_timing(umwm_domain* dom, double ms[3], int64_t counts[3], int reset) {
+  return dispatch(dom, [&](auto& D) {
+    for (int q = 0; q < 3; ++q) {
+      if (ms) ms[q] = D.ms[q];
+      if (counts) counts[q] = D.launches[q];
+      if (reset) {
+        D.ms[q] = 0;
+        D.launches[q] = 0;
+      }
+    }
+    return 0;
+  });
+}
+
+int umwm_hip_synchronize(umwm_domain* dom) {
+  return dispatch(dom, [&](auto& D) { return sync_and_check(D); });
+}
+
+int umwm_hip_set_tile(umwm_domain* dom, int32_t Dd, int32_t W) {
+  return dispatch(dom, [&](auto& D) {
+    const bool okD = Dd == 3 || Dd == 6 || Dd == 9 || Dd == 12 || Dd == 18;
+    if (!okD || W < 1 || W > 8 || Dd * W < D.S.pm)
+      return fail(UMWM_ERR_ARGUMENT, "tile needs D in {3,6,9,12,18}, W in 1..8, D*W >= pm");
+    D.tile.D = Dd;
+    D.tile.W = W;
+    return 0;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// batch entry points
+// ---------------------------------------------------------------------
+namespace {
+
+template <typename T>
+int batch_create_impl(umwm_domain* const* doms, int n, BatchImpl<T>*& out) {
+  auto* B = new BatchImpl<T>();
+  int pmmax = 0;
+  for (int k = 0; k < n; ++k) {
+    auto* D = static_cast<DomainImpl<T>*>(doms[k]->base);
+    B->doms.push_back(D);
+    pmmax = std::max(pmmax, D->S.pm);
+  }
+  B->tile = choose_tile(pmmax);
+  if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete B;
+    return fail(UMWM_ERR_HIP, "hipStreamCreate failed");
+  }
+  // domain structs and concatenated task list, largest domains first
+  std::vector<DomainDev<T>> hd(n);
+  std::vector<int> order(n);
+  for (int k = 0; k < n; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (size_t)B->doms[a]->S.im * B->doms[a]->S.om * B->doms[a]->S.pm >
+           (size_t)B->doms[b]->S.im * B->doms[b]->S.om * B->doms[b]->S.pm;
+  });
+  std::vector<BlockTask> tk;
+  for (int k : order) {
+    hd[k] = B->doms[k]->h;
+    const int nb = hd[k].nblocks;
+    for (int b = 0; b < nb; ++b) tk.push_back(BlockTask{k, b * kLanes, b, 0});
+  }
+  // XCD-aware interleave over the whole list
+  std::vector<int> ord = xcd_order((int)tk.size());
+  std::vector<BlockTask> tk2(tk.size());
+  for (size_t b = 0; b < tk.size(); ++b) tk2[b] = tk[ord[b]];
+  B->ntasks = (int)tk2.size();
+  B->d_doms = B->arena.template alloc<DomainDev<T>>(n);
+  B->tasks = B->arena.template alloc<BlockTask>(std::max(B->ntasks, 1));
+  B->d_scs = B->arena.template alloc<Scalars*>(n);
+  B->d_view = B->arena.template alloc<ScalarsView>(n);
+  if (!B->d_view) {
+    delete B;
+    return fail(UMWM_ERR_NOMEM, "batch allocation failed");
+  }
+  std::vector<Scalars*> scs(n);
+  for (int k = 0; k < n; ++k) scs[k] = hd[k].sc;
+  UMWM_HIP_TRY(hipMemcpy(B->d_doms, hd.data(), n * sizeof(DomainDev<T>), hipMemcpyHostToDevice));
+  UMWM_HIP_TRY(hipMemcpy(B->tasks, tk2.data(), tk2.size() * sizeof(BlockTask), hipMemcpyHostToDevice));
+  UMWM_HIP_TRY(hipMemcpy(B->d_scs, scs.data(), n * sizeof(Scalars*), hipMemcpyHostToDevice));
+  UMWM_HIP_TRY(hipHostMalloc((void**)&B->h_view, n * sizeof(ScalarsView), hipHostMallocDefault));
+  out = B;
+  return 0;
+}
+
+// Runs a per-domain launcher on another stream (batch members share the
+// batch stream so a whole substep can be captured as one graph).
+template <typename T>
+int on_stream(DomainImpl<T>& D, hipStream_t s, int (*fn)(DomainImpl<T>&)) {
+  hipStream_t keep = D.stream;
+  D.stream = s;
+  const int rc = fn(D);
+  D.stream = keep;
+  return rc;
+}
+
+template <typename T>
+int batch_launch_substep(BatchImpl<T>& B, int first, int rotation) {
+  for (auto* D : B.doms)
+    if (any_forcing(*D))
+      if (int rc = on_stream(*D, B.stream, launch_forcinginterpolate<T>)) return rc;
+  int rc = launch_fused<T>(B.d_doms, B.tasks, B.ntasks, first, rotation, B.tile, B.stream, nullptr);
+  if (rc) return rc;
+  const int n = (int)B.doms.size();
+  hipLaunchKernelGGL(k_gather_scalars, dim3((n + 63) / 64), dim3(64), 0, B.stream,
+                     (Scalars* const*)B.d_scs, n, B.d_view);
+  UMWM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int batch_run_dtg_impl(BatchImpl<T>& B, int use_graph, int32_t* nsub) {
+  const int n = (int)B.doms.size();
+  for (auto* D : B.doms) {
+    UMWM_HIP_TRY(hipStreamSynchronize(D->stream));
+    hipLaunchKernelGGL(k_begin_dtg, dim3(1), dim3(64), 0, B.stream, D->h.sc);
+  }
+  UMWM_HIP_TRY(hipGetLastError());
+  int rc;
+  // members still in their first substep take it eagerly (cold dts = 0,
+  // rotation tendency) before the captured steady-state substep is replayed
+  bool any_first = false;
+  for (auto* D : B.doms) any_first |= D->first;
+  if (any_first) {
+    for (auto* D : B.doms)
+      if (!D->first) return fail(UMWM_ERR_STATE, "batch members must start together");
+    if ((rc = batch_launch_substep(B, 1, 1))) return rc;
+    B.rot_every = false;
+    for (auto* D : B.doms) B.rot_every |= D->S.P.currents != 0;
+    for (auto* D : B.doms) {
+      D->first = false;
+      D->rot_valid = true;
+      if ((rc = on_stream(*D, B.stream, launch_diag<T>))) return rc;
+    }
+  }
+  if (use_graph && !B.exec) {
+    UMWM_HIP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+    rc = batch_launch_substep(B, 0, B.rot_every ? 1 : 0);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(B.stream, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(UMWM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    B.graph = g;
+    UMWM_HIP_TRY(hipGraphInstantiate(&B.exec, g, nullptr, nullptr, 0));
+  }
+  // replay in chunks; finished members exit their blocks at once
+  int chunk = 4, total = 0;
+  for (;;) {
+    for (int c = 0; c < chunk; ++c) {
+      if (use_graph) {
+        UMWM_HIP_TRY(hipGraphLaunch(B.exec, B.stream));
+      } else if ((rc = batch_launch_substep(B, 0, B.rot_every ? 1 : 0))) {
+        return rc;
+      }
+    }
+    total += chunk;
+    UMWM_HIP_TRY(hipMemcpyAsync(B.h_view, B.d_view, n * sizeof(ScalarsView), hipMemcpyDeviceToHost, B.stream));
+    UMWM_HIP_TRY(hipStreamSynchronize(B.stream));
+    bool done = true;
+    for (int k = 0; k < n; ++k) done &= !(B.h_view[k].sumt < B.doms[k]->S.P.dtg);
+    if (done) break;
+    if (total > 10000000) return fail(UMWM_ERR_STATE, "batch run_dtg: substep count runaway");
+    chunk = std::min(chunk * 2, 32);
+  }
+  for (auto* D : B.doms) {
+    if (D->firstdtg) D->firstdtg = false;
+    if (D->S.P.stokes && (rc = on_stream(*D, B.stream, launch_stokes<T>))) return rc;
+    if ((rc = on_stream(*D, B.stream, launch_diag<T>))) return rc;
+  }
+  UMWM_HIP_TRY(hipStreamSynchronize(B.stream));
+  if (nsub)
+    for (int k = 0; k < n; ++k) nsub[k] = B.h_view[k].nsub;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int umwm_hip_batch_create(umwm_domain* const* doms, int32_t n, umwm_batch** out) {
+  if (!doms || n < 1 || !out) return fail(UMWM_ERR_ARGUMENT, "bad batch arguments");
+  const int prec = doms[0] && doms[0]->base ? doms[0]->base->precision : 0;
+  for (int k = 0; k < n; ++k)
+    if (!doms[k] || !doms[k]->base || doms[k]->base->precision != prec)
+      return fail(UMWM_ERR_ARGUMENT, "batch members must exist and share the precision");
+  auto* b = new umwm_batch();
+  b->precision = prec;
+  int rc;
+  if (prec == 4) {
+    BatchImpl<float>* B = nullptr;
+    rc = batch_create_impl<float>(doms, n, B);
+    b->impl = B;
+  } else {
+    BatchImpl<double>* B = nullptr;
+    rc = batch_create_impl<double>(doms, n, B);
+    b->impl = B;
+  }
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+int umwm_hip_batch_destroy(umwm_batch* b) {
+  if (!b) return 0;
+  if (b->precision == 4) delete static_cast<BatchImpl<float>*>(b->impl);
+  else delete static_cast<BatchImpl<double>*>(b->impl);
+  delete b;
+  return 0;
+}
+
+int umwm_hip_batch_run_dtg(umwm_batch* b, int use_graph, int32_t* nsub) {
+  if (!b) return fail(UMWM_ERR_ARGUMENT, "null batch");
+  if (b->precision == 4) return batch_run_dtg_impl(*static_cast<BatchImpl<float>*>(b->impl), use_graph, nsub);
+  return batch_run_dtg_impl(*static_cast<BatchImpl<double>*>(b->impl), use_graph, nsub);
+}
+
+int umwm_hip_batch_tile(const umwm_batch* b, int32_t out[2]) {
+  if (!b || !out) return fail(UMWM_ERR_ARGUMENT, "null argument");
+  const Tile t = b->precision == 4 ? static_cast<BatchImpl<float>*>(b->impl)->tile
+                                   : static_cast<BatchImpl<double>*>(b->impl)->tile;
+  out[0] = t.D;
+  out[1] = t.W;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// Shared declarations of the UMWM gfx950 library: device-side domain
+// layout, per-domain scalars, launch descriptors and error plumbing.
+//
+// Device layout (one domain):
+//   spectra     T[o][p][ldi]  sea point j = 0..im-1 contiguous, ghost at
+//                             j = im (holds tiny(e), never updated; the
+//                             reference's e(:,:,0), src/umwm_init.F90:1176)
+//   tables      T[o][ldi]     per (frequency, sea point); cp0/cg0 carry the
+//                             ghost column (src/umwm_init.F90:1652-1656)
+//   per (p, j)  T[p][ldi]     cth_curv, sth_curv
+//   per point   [ldq]         all mm*nm points (sea first, then land) for
+//                             the forcing fields the reference keeps on imm
+//   per sea pt  [ldi]
+// Index maps are 0-based on the device: aliased neighbours point at the
+// ghost (im) where the reference points at 0, true neighbours index the
+// full unrolled grid (0..imm-1).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/umwm_hip.h"
+
+namespace umwm {
+
+// Sticky per-thread error message, surfaced by umwm_hip_last_error().
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define UMWM_HIP_TRY(expr)                                                     \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return ::umwm::fail(UMWM_ERR_HIP, std::string(#expr) + ": " +            \
+                                            hipGetErrorString(e_));            \
+  } while (0)
+
+// Scalars that the reference keeps in module globals and that the device
+// updates itself (src/umwm_module.F90:60,66; src/umwm_advection.F90:110).
+struct Scalars {
+  double sumt, dts, dta, dtr, dtr_temp, maxrot;
+  int32_t active;    // this substep runs (sumt < dtg at its start)
+  int32_t nsub;      // substeps taken since the last reset
+  uint32_t counter;  // arrival counter of the rates kernel's last block
+  uint32_t counter_rot;
+  int32_t anycur;    // any current != 0 (zerocurrents = !anycur)
+  int32_t pad;
+};
+
+// Everything a kernel needs to know about one domain.
+template <typename T>
+struct DomainDev {
+  int32_t im, imm, om, pm, ldi, ldq, mm, nm, lm;
+  int32_t isglobal, restart, currents, nblocks, block0;
+  // namelist and derived constants (double; kernels cast to T)
+  double g, explim, fprog, dtg, dtamin, dth, dthg, oneovdth, twopisds_fac;
+  double sin_fac, fieldscale1, fieldscale2, sds_power, inv_sds_power;
+  double mss_fac, snl_fac, sdt_fac, z, nu_air, kappa, fice_lth, fice_uth;
+  double tiny;  // tiny(e) of the working precision
+  // spectra
+  T *e, *ef, *es;
+  T *ssin, *sds, *snl, *dummy, *rotl, *rotr;  // operator path only
+  // (o, j) tables
+  const T *k, *kdk, *k3dk, *k4, *oneoverk4, *cp0, *cg0, *invcp0, *fkovg;
+  const T *logl2overz, *cothkd, *sbf, *sdv, *snl_arg, *bf1r, *bf2r, *dwn;
+  T *sdt, *sice;  // operator path only
+  // (p, j)
+  const T *cth_curv, *sth_curv;
+  // spectral constants
+  const T *f, *th, *cth, *sth, *c2th, *s2th, *cth2pp;
+  const int32_t *pl, *pr;
+  // per unrolled point (ldq)
+  T *wspd, *wdir, *uc, *vc, *rhoa, *rhow, *rhorat, *fice;
+  // per sea point (ldi)
+  const T *oneovar, *oneovdx, *oneovdy, *dxn, *dxs, *dye, *dyw, *depth;
+  const T *ccurv, *scurv;  // cos, sin of the grid rotation (src/umwm_init.F90:1304-1309)
+  const int32_t *ie, *iw, *in_, *is_, *iie, *iiw, *iin, *iis;
+  int32_t *oc, *oc_rot;
+  T *shelt, *fcutoff;
+  double *pts, *hs_ice;
+  // stress('atm') outputs, FP64 in both precisions
+  double *ustar, *cd, *taux, *tauy, *taux_form, *tauy_form, *taux_skin;
+  double *tauy_skin, *taux_diag, *tauy_diag, *tailatmx, *tailatmy;
+  double *us1, *vs1;  // surface Stokes drift used by stress('atm')
+  // reduction scratch
+  double *part_min, *part_max;
+  Scalars* sc;
+};
+
+// One workgroup of a fused launch: which domain, which 64 sea points.
+struct BlockTask {
+  int32_t dom;
+  int32_t i0;
+  int32_t local;  // block index within the domain
+  int32_t pad;
+};
+
+// Diagnostic outputs of diag() (src/umwm_physics.F90:89-205), FP64.
+struct DiagDev {
+  double *ht, *mwp, *mwd, *mss, *mwl, *momx, *momy, *cgmxx, *cgmxy, *cgmyy;
+  double *dwd, *dwl, *dwp, *dcp0, *dcg0, *dcp, *dcg;
+};
+
+// Stokes drift outputs (src/umwm_stokes.F90:13-132), FP64.
+struct StokesDev {
+  const double* depth;  // lm levels, negative down
+  double *us, *vs, *ds; // us/vs [l][ldi]
+};
+
+// Forcing levels kept on the device (src/umwm_forcing.F90:11-99).
+template <typename T>
+struct ForcingDev {
+  const T *uwb, *uwf, *vwb, *vwf;      // (mm,nm)
+  const T *ucb, *ucf, *vcb, *vcf;      // (mm,nm)
+  const T *ficeb, *ficef;              // (mm,nm)
+  const T *rhoab, *rhoaf, *rhowb, *rhowf;  // unrolled (ldq)
+  const int32_t *mi, *ni;              // unrolled -> 1-based (m,n)
+  int32_t winds, currents, air_density, water_density, seaice, pad;
+};
+
+constexpr int kLanes = 64;  // wavefront width on CDNA
+
+inline int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+}  // namespace umwm

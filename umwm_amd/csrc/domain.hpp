@@ -1,0 +1,103 @@
+// Host-side owner of one domain's device state (the opaque umwm_domain).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "setup.hpp"
+
+namespace umwm {
+
+// Tile shape of the fused kernels: W waves per workgroup, each thread owns
+// D consecutive directions of one sea point; 64 sea points per workgroup.
+struct Tile {
+  int D = 9;
+  int W = 4;
+};
+
+class DeviceArena {
+ public:
+  ~DeviceArena() { release(); }
+  template <typename U>
+  U* alloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    if (hipMalloc(&p, n * sizeof(U)) != hipSuccess) return nullptr;
+    ptrs_.push_back(p);
+    bytes_ += n * sizeof(U);
+    return static_cast<U*>(p);
+  }
+  void release() {
+    for (void* p : ptrs_) (void)hipFree(p);
+    ptrs_.clear();
+    bytes_ = 0;
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  std::vector<void*> ptrs_;
+  size_t bytes_ = 0;
+};
+
+struct DomainBase {
+  virtual ~DomainBase() = default;
+  int precision = 8;
+  HostSetup S;
+  hipStream_t stream = nullptr;
+  bool first = true;      // src/umwm_module.F90:47
+  bool firstdtg = true;
+  bool timing = false;
+  double ms[3] = {0, 0, 0};
+  int64_t launches[3] = {0, 0, 0};
+};
+
+template <typename T>
+struct DomainImpl : DomainBase {
+  DeviceArena arena;
+  DomainDev<T> h{};           // host mirror of the device struct
+  DomainDev<T>* d = nullptr;  // device copy (array of one)
+  ForcingDev<T> fh{};
+  DiagDev diag{};
+  StokesDev stokes{};
+  BlockTask* tasks = nullptr;
+  int ntasks = 0;
+  Tile tile;
+  Scalars* sc_host = nullptr;  // pinned mirror
+  bool ops_ready = false;      // operator-path arrays allocated
+  bool rot_valid = false;      // rotation computed at least once
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // forcing device buffers (level n and n+1)
+  T *uwb = nullptr, *uwf = nullptr, *vwb = nullptr, *vwf = nullptr;
+  T *ucb = nullptr, *ucf = nullptr, *vcb = nullptr, *vcf = nullptr;
+  T *ficeb = nullptr, *ficef = nullptr;
+  T *rhoab = nullptr, *rhoaf = nullptr, *rhowb = nullptr, *rhowf = nullptr;
+
+  ~DomainImpl() override;
+};
+
+// Kernel launchers (ops.hip, fused.hip).
+template <typename T> int launch_forcinginterpolate(DomainImpl<T>& D);
+template <typename T> int launch_op_sin(DomainImpl<T>& D);
+template <typename T> int launch_op_sds(DomainImpl<T>& D);
+template <typename T> int launch_op_snl(DomainImpl<T>& D);
+template <typename T> int launch_op_sice(DomainImpl<T>& D);
+template <typename T> int launch_op_source(DomainImpl<T>& D);
+template <typename T> int launch_op_propagation(DomainImpl<T>& D);
+template <typename T> int launch_op_refraction(DomainImpl<T>& D);
+template <typename T> int launch_op_update(DomainImpl<T>& D);
+template <typename T> int launch_op_stress_atm(DomainImpl<T>& D);
+template <typename T> int launch_diag(DomainImpl<T>& D);
+template <typename T> int launch_stokes(DomainImpl<T>& D);
+template <typename T> int launch_hs_ice(DomainImpl<T>& D);
+template <typename T> int launch_jonswap(DomainImpl<T>& D, const umwm_jonswap& js);
+
+// Fused substep kernels over a task list that may span many domains.
+template <typename T>
+int launch_fused(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int first,
+                 int rotation, Tile tile, hipStream_t s, hipEvent_t* ev);
+
+}  // namespace umwm
