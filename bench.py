@@ -20,7 +20,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from umwm_amd import configs, hip  # noqa: E402
+from umwm_amd import autotune, configs, hip  # noqa: E402
 
 METRIC = "spectral-bin updates/s (source+advect+refract), chosen vs plain at Hs RMSE ≤2%"
 HBM_PEAK_GBS = 8000.0
@@ -38,10 +38,13 @@ def make_domain(wl, precision):
     return d
 
 
-def timed_steps(d, warmup, steps):
-    """Cold substep + warmup, then `steps` timed substeps.  Returns wall
-    seconds, per-kernel mean ms, and the substeps taken."""
+def timed_steps(d, warmup, steps, tune=True):
+    """Cold substep, autotuning and warmup, then `steps` timed substeps.
+    Returns wall seconds, per-kernel mean ms, and the tile chosen."""
     d.substep()  # cold first substep (dts = 0), not counted
+    tile = None
+    if tune:
+        tile, _ = autotune.autotune(d, steps=2, verbose=log)
     for _ in range(warmup):
         d.substep()
     d.synchronize()
@@ -55,7 +58,7 @@ def timed_steps(d, warmup, steps):
     wall = time.perf_counter() - t0
     ms, cnt = d.timing(reset=True)
     d.set_timing(False)
-    return wall, [m / max(c, 1) for m, c in zip(ms, cnt)], sumt
+    return wall, [m / max(c, 1) for m, c in zip(ms, cnt)], tile
 
 
 def finish_run(d, wl, hours):
@@ -119,7 +122,25 @@ def main():
     ap.add_argument("--hours", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="quarter-size grid for smoke runs")
+    ap.add_argument("--only", choices=["fp32", "fp64"], default=None,
+                    help="run one path only (profiling runs); prints that path's line")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
+    ap.add_argument("--tile", type=int, nargs=5, default=None,
+                    metavar=("D1", "W1", "D2", "W2", "NB"),
+                    help="fixed tile shape (implies --no-tune)")
+    ap.add_argument("--no-finish", action="store_true",
+                    help="skip finishing the simulated hours (profiling runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device-to-device copy first (scales HBM counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("bench: copy probe 3 x 1 GiB done")
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note "
               "only (see DESIGN.md), so --gpus must be 1", file=sys.stderr)
@@ -130,22 +151,39 @@ def main():
     wl = configs.c2(seed=2, mm=mm, nm=nm, hours=args.hours)
 
     res = {}
-    for prec, tag in ((8, "fp64"), (4, "fp32")):
+    paths = ((8, "fp64"), (4, "fp32"))
+    if args.only:
+        paths = tuple(x for x in paths if x[1] == args.only)
+    for prec, tag in paths:
         log(f"bench: {tag} create")
         d = make_domain(wl, prec)
         bins = d.im * d.om * d.pm
         log(f"bench: {tag} timed {args.steps} substeps of {bins} bins")
-        wall, kms, sumt = timed_steps(d, args.warmup, args.steps)
+        if args.tile:
+            d.set_tile(*args.tile)
+        wall, kms, tile = timed_steps(d, args.warmup, args.steps,
+                                      tune=not (args.no_tune or args.tile))
+        tile = tile or (tuple(args.tile) if args.tile else None)
         rate = args.steps * bins / wall
         log(f"bench: {tag} {rate / 1e9:.2f} G bin-updates/s, kernels ms {kms}")
-        log(f"bench: {tag} finishing {args.hours} h")
-        t0 = time.perf_counter()
-        finish_run(d, wl, args.hours)
-        log(f"bench: {tag} run done in {time.perf_counter() - t0:.1f} s")
+        if not args.no_finish:
+            log(f"bench: {tag} finishing {args.hours} h")
+            t0 = time.perf_counter()
+            finish_run(d, wl, args.hours)
+            log(f"bench: {tag} run done in {time.perf_counter() - t0:.1f} s")
         res[tag] = dict(rate=rate, wall=wall, kms=kms, bins=bins, im=d.im,
                         hs=d.get("ht"), tm=d.get("mwp"), ust=d.get("ustar"),
-                        us=d.get("us")[0], vs=d.get("vs")[0], tile=None)
+                        us=d.get("us")[0], vs=d.get("vs")[0], tile=tile)
         d.close()
+
+    if args.only:
+        r = res[args.only]
+        word = 4 if args.only == "fp32" else 8
+        t = sum(r["kms"]) * 1e-3
+        print(json.dumps({"path": args.only, "value": r["rate"], "unit": "bin-updates/s",
+                          "kernel_ms": r["kms"], "tile": r["tile"],
+                          "achieved_GBs": 3 * word * r["bins"] / t / 1e9}), flush=True)
+        return
 
     r32, r64 = res["fp32"], res["fp64"]
     hs_ref = r64["hs"]
@@ -170,8 +208,8 @@ def main():
         ach = 3 * word * r["bins"] / t / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "fused substep (k_rates + k_integrate + k_advect)",
-                "kernel_ms": [round(x, 4) for x in r["kms"]]}
+                "kernel": "fused substep (k_rates + k_integrate + k_advect/k_stress)",
+                "kernel_ms": [round(x, 4) for x in r["kms"]], "tile": r["tile"]}
 
     out = {
         "metric": METRIC,

@@ -239,10 +239,13 @@ int umwm_hip_get_timing(umwm_domain* dom, double ms[3], int64_t counts[3],
                         int reset);
 /* Waits for all work queued on the domain's stream. */
 int umwm_hip_synchronize(umwm_domain* dom);
-/* Overrides the fused kernels' tile shape: D directions per thread (3, 6,
- * 9, 12 or 18) and W waves per workgroup (1..8, D*W >= pm).  Used by the
- * autotuner (umwm_amd/autotune.py). */
-int umwm_hip_set_tile(umwm_domain* dom, int32_t D, int32_t W);
+/* Tile shape of the fused kernels, tile[5] = {D1, W1, D2, W2, NB}:
+ * rates with D1 directions per thread (6, 9, 18 or 36) and W1 waves,
+ * integrate with D2 (3, 6, 9, 12 or 18) and W2 waves (W in 1..8,
+ * D*W >= pm), and advect prefetching NB directions ahead (2 or 4).
+ * Used by the autotuner (umwm_amd/autotune.py). */
+int umwm_hip_set_tile(umwm_domain* dom, const int32_t tile[5]);
+int umwm_hip_get_tile(const umwm_domain* dom, int32_t tile[5]);
 
 /* ---------------------------------------------------------------------- */
 /* Batched ensembles: many independent domains of uneven sizes advance in */
@@ -257,9 +260,11 @@ int umwm_hip_batch_destroy(umwm_batch* b);
  * hipGraph per substep when use_graph != 0.  nsub[k] receives member k's
  * substep count (array of n, may be NULL). */
 int umwm_hip_batch_run_dtg(umwm_batch* b, int use_graph, int32_t* nsub);
-/* Tile shape chosen by the autotuner for the batch (directions per thread
- * and waves per block): out[0]=D, out[1]=W. */
-int umwm_hip_batch_tile(const umwm_batch* b, int32_t out[2]);
+/* Tile shape of the batch's fused launches (layout of umwm_hip_set_tile). */
+int umwm_hip_batch_tile(const umwm_batch* b, int32_t out[5]);
+/* Sets the batch's tile shape (same rules as umwm_hip_set_tile over the
+ * largest pm of the batch). */
+int umwm_hip_batch_set_tile(umwm_batch* b, const int32_t tile[5]);
 
 #ifdef __cplusplus
 }

@@ -101,19 +101,50 @@ std::vector<int> xcd_order(int nb) {
   return pb;
 }
 
+// Default tile shape before autotuning: D directions per thread, W =
+// ceil(pm / D) waves (at most 8).
 Tile choose_tile(int pm) {
-  // Smallest register footprint (directions per thread D) that fits in at
-  // most 8 waves per workgroup; W = ceil(pm / D).
+  auto pick = [pm](std::initializer_list<int> ds, int& D, int& W) {
+    for (int c : ds)
+      if ((pm + c - 1) / c <= 8) {
+        D = c;
+        W = (pm + c - 1) / c;
+        return;
+      }
+    D = 36;
+    W = (pm + 35) / 36;
+  };
   Tile t;
-  for (int D : {3, 6, 9, 12, 18})
-    if ((pm + D - 1) / D <= 8) {
-      t.D = D;
-      t.W = (pm + D - 1) / D;
-      return t;
-    }
-  t.D = 18;
-  t.W = (pm + 17) / 18;
+  pick({9, 18, 36}, t.D1, t.W1);
+  pick({9, 12, 18}, t.D2, t.W2);
+  t.pmax = pm;
   return t;
+}
+
+bool tile_ok(const Tile& t, int pm) {
+  const bool d1 = t.D1 == 6 || t.D1 == 9 || t.D1 == 18 || t.D1 == 36;
+  const bool d2 = t.D2 == 3 || t.D2 == 6 || t.D2 == 9 || t.D2 == 12 || t.D2 == 18;
+  return d1 && d2 && t.W1 >= 1 && t.W1 <= 8 && t.W2 >= 1 && t.W2 <= 8 && t.D1 * t.W1 >= pm &&
+         t.D2 * t.W2 >= pm && (t.NB == 2 || t.NB == 4);
+}
+
+Tile tile_from(const int32_t* a, int pmax) {
+  Tile t;
+  t.D1 = a[0];
+  t.W1 = a[1];
+  t.D2 = a[2];
+  t.W2 = a[3];
+  t.NB = a[4];
+  t.pmax = pmax;
+  return t;
+}
+
+void tile_to(const Tile& t, int32_t* a) {
+  a[0] = t.D1;
+  a[1] = t.W1;
+  a[2] = t.D2;
+  a[3] = t.W2;
+  a[4] = t.NB;
 }
 
 }  // namespace
@@ -276,6 +307,8 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   h.oc = A.template alloc<int32_t>(ldi);
   h.oc_rot = A.template alloc<int32_t>(ldi);
   h.shelt = A.template alloc<T>(ldi);
+  h.cwdir = A.template alloc<T>(ldi);
+  h.swdir = A.template alloc<T>(ldi);
   h.fcutoff = A.template alloc<T>(ldi);
   h.pts = A.template alloc<double>(ldi);
   h.hs_ice = A.template alloc<double>(ldi);
@@ -299,7 +332,8 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
     UMWM_HIP_TRY(hipMemcpyAsync(h.cd, c.data(), ldi * sizeof(double), hipMemcpyHostToDevice, s));
     UMWM_HIP_TRY(hipStreamSynchronize(s));
   }
-  if ((rc = dfill<T>(h.shelt, ldi, T(0), s)) || (rc = dfill<T>(h.fcutoff, ldi, T(0), s))) {
+  if ((rc = dfill<T>(h.shelt, ldi, T(0), s)) || (rc = dfill<T>(h.fcutoff, ldi, T(0), s)) ||
+      (rc = dfill<T>(h.cwdir, ldi, T(1), s)) || (rc = dfill<T>(h.swdir, ldi, T(0), s))) {
     delete D;
     return rc;
   }
@@ -315,10 +349,16 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   h.block0 = 0;
   h.part_min = A.template alloc<double>(std::max(h.nblocks, 1));
   h.part_max = A.template alloc<double>(std::max(h.nblocks, 1));
+  h.pfx = A.template alloc<T>(3 * nt);
+  h.spart = A.template alloc<double>(3 * nt);
   h.sc = A.template alloc<Scalars>(1);
-  if (!h.sc) {
+  if (!h.sc || !h.pfx || !h.spart) {
     delete D;
     return fail(UMWM_ERR_NOMEM, "device allocation failed");
+  }
+  if ((rc = dfill<T>(h.pfx, 3 * nt, T(0), s)) || (rc = dfill<double>(h.spart, 3 * nt, 0.0, s))) {
+    delete D;
+    return rc;
   }
   Scalars sc0{};
   sc0.anycur = (!P.currents && (P.uc0 != 0 || P.vc0 != 0)) ? 1 : 0;
@@ -438,8 +478,8 @@ int fused_substep(DomainImpl<T>& D) {
   int rc;
   if (any_forcing(D) && (rc = launch_forcinginterpolate(D))) return rc;
   const int rotation = (D.S.P.currents || D.first || !D.rot_valid) ? 1 : 0;
-  rc = launch_fused<T>(D.d, D.tasks, D.ntasks, D.first ? 1 : 0, rotation, D.tile, D.stream,
-                       D.timing ? D.ev : nullptr);
+  rc = launch_fused<T>(D.d, D.tasks, D.ntasks, D.h.om, D.first ? 1 : 0, rotation, D.tile,
+                       D.stream, D.timing ? D.ev : nullptr);
   if (rc) return rc;
   D.rot_valid = true;
   if (D.first) {
@@ -712,6 +752,7 @@ struct BatchImpl {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool rot_every = false;
+  int omax = 0;
   ~BatchImpl() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -1053,13 +1094,23 @@ int umwm_hip_synchronize(umwm_domain* dom) {
   return dispatch(dom, [&](auto& D) { return sync_and_check(D); });
 }
 
-int umwm_hip_set_tile(umwm_domain* dom, int32_t Dd, int32_t W) {
+int umwm_hip_set_tile(umwm_domain* dom, const int32_t tile[5]) {
+  if (!tile) return fail(UMWM_ERR_ARGUMENT, "null tile");
   return dispatch(dom, [&](auto& D) {
-    const bool okD = Dd == 3 || Dd == 6 || Dd == 9 || Dd == 12 || Dd == 18;
-    if (!okD || W < 1 || W > 8 || Dd * W < D.S.pm)
-      return fail(UMWM_ERR_ARGUMENT, "tile needs D in {3,6,9,12,18}, W in 1..8, D*W >= pm");
-    D.tile.D = Dd;
-    D.tile.W = W;
+    Tile t = tile_from(tile, D.S.pm);
+    if (!tile_ok(t, D.S.pm))
+      return fail(UMWM_ERR_ARGUMENT, "tile needs D1 in {6,9,18,36}, D2 in {3,6,9,12,18}, "
+                                     "W in 1..8, D*W >= pm, NB in {2,4}");
+    D.tile = t;
+    return 0;
+  });
+}
+
+int umwm_hip_get_tile(const umwm_domain* dom, int32_t tile[5]) {
+  if (!dom || !dom->base || !tile) return fail(UMWM_ERR_ARGUMENT, "null argument");
+  umwm_domain* m = const_cast<umwm_domain*>(dom);
+  return dispatch(m, [&](auto& D) {
+    tile_to(D.tile, tile);
     return 0;
   });
 }
@@ -1079,6 +1130,7 @@ int batch_create_impl(umwm_domain* const* doms, int n, BatchImpl<T>*& out) {
     auto* D = static_cast<DomainImpl<T>*>(doms[k]->base);
     B->doms.push_back(D);
     pmmax = std::max(pmmax, D->S.pm);
+    B->omax = std::max(B->omax, D->S.om);
   }
   B->tile = choose_tile(pmmax);
   if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1138,7 +1190,8 @@ int batch_launch_substep(BatchImpl<T>& B, int first, int rotation) {
   for (auto* D : B.doms)
     if (any_forcing(*D))
       if (int rc = on_stream(*D, B.stream, launch_forcinginterpolate<T>)) return rc;
-  int rc = launch_fused<T>(B.d_doms, B.tasks, B.ntasks, first, rotation, B.tile, B.stream, nullptr);
+  int rc = launch_fused<T>(B.d_doms, B.tasks, B.ntasks, B.omax, first, rotation, B.tile,
+                           B.stream, nullptr);
   if (rc) return rc;
   const int n = (int)B.doms.size();
   hipLaunchKernelGGL(k_gather_scalars, dim3((n + 63) / 64), dim3(64), 0, B.stream,
@@ -1256,13 +1309,32 @@ int umwm_hip_batch_run_dtg(umwm_batch* b, int use_graph, int32_t* nsub) {
   return batch_run_dtg_impl(*static_cast<BatchImpl<double>*>(b->impl), use_graph, nsub);
 }
 
-int umwm_hip_batch_tile(const umwm_batch* b, int32_t out[2]) {
+int umwm_hip_batch_tile(const umwm_batch* b, int32_t out[5]) {
   if (!b || !out) return fail(UMWM_ERR_ARGUMENT, "null argument");
   const Tile t = b->precision == 4 ? static_cast<BatchImpl<float>*>(b->impl)->tile
                                    : static_cast<BatchImpl<double>*>(b->impl)->tile;
-  out[0] = t.D;
-  out[1] = t.W;
+  tile_to(t, out);
   return 0;
+}
+
+int umwm_hip_batch_set_tile(umwm_batch* b, const int32_t tile[5]) {
+  if (!b || !tile) return fail(UMWM_ERR_ARGUMENT, "null argument");
+  auto apply = [&](auto* B) {
+    int pmmax = 0;
+    for (auto* D : B->doms) pmmax = std::max(pmmax, D->S.pm);
+    Tile t = tile_from(tile, pmmax);
+    if (!tile_ok(t, pmmax)) return fail(UMWM_ERR_ARGUMENT, "tile does not cover pm");
+    B->tile = t;
+    if (B->exec) {  // the captured graph holds the old launch shape
+      (void)hipGraphExecDestroy(B->exec);
+      (void)hipGraphDestroy(B->graph);
+      B->exec = nullptr;
+      B->graph = nullptr;
+    }
+    return 0;
+  };
+  if (b->precision == 4) return apply(static_cast<BatchImpl<float>*>(b->impl));
+  return apply(static_cast<BatchImpl<double>*>(b->impl));
 }
 
 }  // extern "C"

@@ -79,11 +79,17 @@ struct DomainDev {
   const int32_t *ie, *iw, *in_, *is_, *iie, *iiw, *iin, *iis;
   int32_t *oc, *oc_rot;
   T *shelt, *fcutoff;
+  T *cwdir, *swdir;  // cos, sin of wdir, written by the fused rates kernel
   double *pts, *hs_ice;
   // stress('atm') outputs, FP64 in both precisions
   double *ustar, *cd, *taux, *tauy, *taux_form, *tauy_form, *taux_skin;
   double *tauy_skin, *taux_diag, *tauy_diag, *tailatmx, *tailatmy;
   double *us1, *vs1;  // surface Stokes drift used by stress('atm')
+  // fused-path scratch: exclusive frequency prefix of the cos^2 directional
+  // convolution times dth/2, [3][om][ldi] (written by k_rates), and the
+  // per-frequency stress / wave-height partial sums [3][om][ldi]
+  T* pfx;
+  double* spart;
   // reduction scratch
   double *part_min, *part_max;
   Scalars* sc;

@@ -12,12 +12,17 @@
 
 namespace umwm {
 
-// Tile shape of the fused kernels: W waves per workgroup, each thread owns
-// D consecutive directions of one sea point; 64 sea points per workgroup.
+// Tile shape of the frequency-sweeping fused kernels (k_rates: D1, W1;
+// k_integrate: D2, W2): W waves per workgroup, each thread owns D
+// consecutive directions of one sea point; 64 sea points per workgroup.
 struct Tile {
-  int D = 9;
-  int W = 4;
+  int D1 = 9, W1 = 4;
+  int D2 = 9, W2 = 4;
+  int NB = 2;     // advect: directions per prefetch block
+  int pmax = 36;  // largest pm of the domains launched together
 };
+
+bool tile_supported(int D);
 
 class DeviceArena {
  public:
@@ -97,7 +102,7 @@ template <typename T> int launch_jonswap(DomainImpl<T>& D, const umwm_jonswap& j
 
 // Fused substep kernels over a task list that may span many domains.
 template <typename T>
-int launch_fused(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int first,
-                 int rotation, Tile tile, hipStream_t s, hipEvent_t* ev);
+int launch_fused(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int omax,
+                 int first, int rotation, Tile tile, hipStream_t s, hipEvent_t* ev);
 
 }  // namespace umwm

@@ -12,10 +12,14 @@ namespace umwm {
 
 constexpr double kTwoPiD = 2 * 3.141592653589793;  // src/umwm_module.F90:87-89
 
-// x**y for x >= 0 (Fortran real power).  FP64 keeps the library pow; FP32
-// uses the hardware log2/exp2 pair, which is the FP32 path's documented
-// error budget (Hs within 0.5 % of FP64).
-__device__ inline double pw(double x, double y) { return pow(x, y); }
+// x**y for x >= 0 (Fortran real power).  FP64 evaluates exp(y log x) with
+// the device library's FP64 exp and log (relative error ~|y log x| ulp,
+// below 1e-14 for the spectra here); FP32 uses the hardware log2/exp2
+// pair, inside the FP32 path's error budget (Hs within 0.5 % of FP64).
+__device__ inline double pw(double x, double y) {
+  if (!(x > 0.0)) return x == 0.0 ? 0.0 : __builtin_nan("");
+  return exp(y * log(x));
+}
 __device__ inline float pw(float x, float y) {
   if (!(x > 0.f)) return x == 0.f ? 0.f : __builtin_nanf("");
   return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
@@ -63,6 +67,39 @@ __device__ inline T ssin_bin(const SinPoint<T>& s, T logl2, T cwt, T cp0, T cth,
   if (x < T(0) && cwt > T(0)) x = x * fs2;
   x = s.pre * x * fkovg;
   if (o > s.oc && x < T(0)) x = T(0);  // diagnostic tail: max(ssin, 0)
+  return x;
+}
+
+// ssin_bin with ba = wspd + 2.5 ustar logl2overz(o) evaluated once per
+// frequency.
+template <typename T>
+__device__ inline T ssin_bin2(const SinPoint<T>& s, T ba, T cwt, T cp0, T cth, T sth, T fkovg,
+                              T sin_fac, T fs1, T fs2, int o) {
+  T x = ba * cwt - cp0 - s.uc * cth - s.vc * sth;
+  x = sin_fac * fabs(x) * x;
+  if (x > T(0)) x = x * s.shelt / sin_fac;
+  if (x < T(0)) x = x * fs1;
+  if (x < T(0) && cwt > T(0)) x = x * fs2;
+  x = s.pre * x * fkovg;
+  if (o > s.oc && x < T(0)) x = T(0);
+  return x;
+}
+
+// ssin_bin with the wind projection expanded by the angle-sum identity:
+// (wspd + 2.5 ustar logl2overz) cos(wdir - th) - uc cth - vc sth
+//   = A cth + B sth,  A = ba cos(wdir) - uc,  B = ba sin(wdir) - vc,
+// A and B evaluated once per frequency; cwt = cos(wdir - th) is still
+// needed for the swell-overrunning-wind test.
+template <typename T>
+__device__ inline T ssin_ab(const SinPoint<T>& s, T A, T B, T cwt, T cp0, T cth, T sth, T fkovg,
+                            T sin_fac, T fs1, T fs2, int o) {
+  T x = A * cth + B * sth - cp0;
+  x = sin_fac * fabs(x) * x;
+  if (x > T(0)) x = x * s.shelt / sin_fac;
+  if (x < T(0)) x = x * fs1;
+  if (x < T(0) && cwt > T(0)) x = x * fs2;
+  x = s.pre * x * fkovg;
+  if (o > s.oc && x < T(0)) x = T(0);
   return x;
 }
 

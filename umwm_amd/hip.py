@@ -26,7 +26,7 @@ EXPORTS = [
     "umwm_hip_set_scalar", "umwm_hip_field_size", "umwm_hip_set_timing",
     "umwm_hip_get_timing", "umwm_hip_synchronize", "umwm_hip_batch_create",
     "umwm_hip_batch_destroy", "umwm_hip_batch_run_dtg", "umwm_hip_batch_tile",
-    "umwm_hip_set_tile",
+    "umwm_hip_set_tile", "umwm_hip_get_tile", "umwm_hip_batch_set_tile",
 ]
 
 
@@ -77,7 +77,9 @@ def lib():
     L.umwm_hip_batch_destroy.argtypes = [vp]
     L.umwm_hip_batch_run_dtg.argtypes = [vp, C.c_int, P(C.c_int32)]
     L.umwm_hip_batch_tile.argtypes = [vp, P(C.c_int32)]
-    L.umwm_hip_set_tile.argtypes = [vp, C.c_int32, C.c_int32]
+    L.umwm_hip_set_tile.argtypes = [vp, P(C.c_int32)]
+    L.umwm_hip_get_tile.argtypes = [vp, P(C.c_int32)]
+    L.umwm_hip_batch_set_tile.argtypes = [vp, P(C.c_int32)]
     _lib = L
     return L
 
@@ -239,8 +241,14 @@ class Domain:
         check(lib().umwm_hip_run_dtg(self.h, C.byref(n)))
         return n.value
 
-    def set_tile(self, D, W):
-        check(lib().umwm_hip_set_tile(self.h, int(D), int(W)))
+    def set_tile(self, D1, W1, D2, W2, NB=2):
+        t = (C.c_int32 * 5)(int(D1), int(W1), int(D2), int(W2), int(NB))
+        check(lib().umwm_hip_set_tile(self.h, t))
+
+    def tile(self):
+        t = (C.c_int32 * 5)()
+        check(lib().umwm_hip_get_tile(self.h, t))
+        return tuple(t)
 
     def set_timing(self, on=True):
         check(lib().umwm_hip_set_timing(self.h, int(on)))
@@ -282,6 +290,10 @@ class Batch:
         return list(n)
 
     def tile(self):
-        t = (C.c_int32 * 2)()
+        t = (C.c_int32 * 5)()
         check(lib().umwm_hip_batch_tile(self.h, t))
         return tuple(t)
+
+    def set_tile(self, D1, W1, D2, W2, NB=2):
+        t = (C.c_int32 * 5)(int(D1), int(W1), int(D2), int(W2), int(NB))
+        check(lib().umwm_hip_batch_set_tile(self.h, t))
