@@ -38,8 +38,7 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
+        build()  # no-op when the library is newer than its sources
         L = C.CDLL(LIB_PATH)
         P = C.POINTER
         L.oracle_create.restype = C.c_void_p

@@ -741,6 +741,7 @@ double oracle_get_scalar(oracle_model* m, const char* s) {
 int oracle_set_scalar(oracle_model* m, const char* s, double v) {
   if (!strcmp(s, "sumt")) { m->sumt = v; return 0; }
   if (!strcmp(s, "dts")) { m->dts = v; return 0; }
+  if (!strcmp(s, "dta")) { m->dta = v; return 0; }
   if (!strcmp(s, "first")) { m->first = (int)v; return 0; }
   if (!strcmp(s, "dtr_temp")) { m->dtr_temp = v; return 0; }
   return -1;

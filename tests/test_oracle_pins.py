@@ -1,0 +1,466 @@
+"""Pins of the FP64 oracle against what the mathematics fixes.
+
+The reference ships no tests, golden vectors or fixtures for this path, and
+its Fortran does not build here (it needs NetCDF-Fortran and an unvendored
+datetime module), so each oracle function is checked against a closed form,
+an invariant, a textbook reduction or a hand-enumerated case, chosen so that
+a plausible slip (a dropped term, a wrong sign, an off-by-one in a prefix,
+a transposed neighbour) fails a test.  CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, optimize
+
+from umwm_amd import abi
+
+G = 9.80665
+
+
+@pytest.fixture(scope="module")
+def O(oracle_lib):
+    return oracle_lib
+
+
+def regional(O, mm=12, nm=10, om=37, pm=36, depth=None, **kw):
+    p = abi.make_params(mm=mm, nm=nm, om=om, pm=pm, **kw)
+    grid = abi.make_grid_input(z=-depth) if depth is not None else None
+    if depth is not None:
+        p.topofromfile = 1
+    return O.Oracle(p, grid, None, threads=1)
+
+
+# ---------------------------------------------------------------- closed forms
+def test_sheltering_coare35_pieces():
+    """src/umwm_sheltering.F90:49-79: linear from 0.04 to 0.10 on [0, 15],
+    a quadratic on (15, 33] continuous at both ends, exponential decay
+    above 33 continuous at 33."""
+    from oracle import oracle_py as O
+    s = O.sheltering_coare35
+    assert s(0.0) == pytest.approx(0.04, rel=1e-15)
+    assert s(7.5) == pytest.approx(0.07, rel=1e-14)
+    assert s(15.0) == pytest.approx(0.10, rel=1e-14)
+    eps = 1e-9
+    for x in (15.0, 33.0):
+        assert abs(s(x + eps) - s(x)) < 1e-9
+    # quadratic through (15, 0.10) with the stated curvature; decay beyond 33
+    s33 = s(33.0)
+    for w in (40.0, 60.0, 80.0):
+        assert s(w) == pytest.approx(s33 * math.exp(-(w - 33.0) / (1.6 * w)), rel=1e-14)
+    # the quadratic's second difference is 2c with c = 0.65 (slope2 - slope1) / 18
+    slope1, slope2 = 0.06 / 15, (0.06 - 0.09) / 27
+    c = 0.65 * (slope2 - slope1) / 18
+    h = 1.0
+    d2 = s(20.0 + h) - 2 * s(20.0) + s(20.0 - h)
+    assert d2 == pytest.approx(2 * c * h * h, rel=1e-9)
+
+
+@pytest.mark.parametrize("w,kmax", [(5.0, 3.0), (15.0, 10.0), (40.0, 50.0)])
+def test_stress_tail_is_the_integral(w, kmax):
+    """src/umwm_stress.F90:314-324: the tail is the integral of (k/kmax)^t
+    from kmax to the capillary wavenumber 1e3, t = a w^2 + b w + c."""
+    from oracle import oracle_py as O
+    t = 0.000112 * w * w - 0.01451 * w - 1.0186
+    ref, _ = integrate.quad(lambda k: (k / kmax) ** t, kmax, 1e3, limit=200, epsrel=1e-12)
+    assert O.stress_tail(w, kmax) == pytest.approx(ref, rel=1e-9)
+
+
+def test_friction_velocity_skin_fixed_point():
+    """src/umwm_stress.F90:300-311: six iterates of u = kappa w / ln(z/z0),
+    z0 = 0.132 nu / u, from z0 = 1e-3.  The result is close to the fixed
+    point (found by root bracketing) and is the sixth iterate, not the
+    fifth or seventh."""
+    from oracle import oracle_py as O
+    w, z, nu, kap = 12.0, 10.0, 1.56e-5, 0.4
+    u = O.friction_velocity_skin(w, z, nu, kap)
+    fp = optimize.brentq(lambda x: x - kap * w / math.log(z * x / (0.132 * nu)), 1e-4, 5.0,
+                         xtol=1e-15)
+    assert u == pytest.approx(fp, rel=1e-6)
+    its, z0 = [], 1e-3
+    for _ in range(7):
+        x = kap * w / math.log(z / z0)
+        z0 = 0.132 * nu / x
+        its.append(x)
+    assert u == its[5]
+    assert u != its[4]
+
+
+def test_drag_coefficient_skin_cap():
+    """src/umwm_stress.F90:287-298: cd capped at 1e-2; cd = u*^2/w^2 reduced by
+    the form-drag share (1 + 2 cd/(cd + cd_form))/3."""
+    from oracle import oracle_py as O
+    assert O.drag_coefficient_skin(0.0, 0.05, 60.0, 10.0, 1.56e-5, 0.4) == 1e-2
+    us = O.friction_velocity_skin(10.0, 10.0, 1.56e-5, 0.4)
+    cd = us * us / 100.0
+    assert O.drag_coefficient_skin(0.0, 10.0, 10.0, 10.0, 1.56e-5, 0.4) == pytest.approx(cd, rel=1e-14)
+    # a large form drag leaves a third of the skin drag
+    big = O.drag_coefficient_skin(1e6, 10.0, 10.0, 10.0, 1.56e-5, 0.4)
+    assert big == pytest.approx(cd / 3, rel=1e-5)
+
+
+def test_haversine_against_vector_formula():
+    """src/umwm_util.F90:211-232 great-circle angle (radians in, unit sphere),
+    checked against the angle between unit vectors."""
+    from oracle import oracle_py as O
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        lon1, lon2 = rng.uniform(-180, 180, 2)
+        lat1, lat2 = rng.uniform(-80, 80, 2)
+        v = []
+        for lo, la in ((lon1, lat1), (lon2, lat2)):
+            lo, la = math.radians(lo), math.radians(la)
+            v.append(np.array([math.cos(la) * math.cos(lo), math.cos(la) * math.sin(lo),
+                               math.sin(la)]))
+        ang = math.atan2(np.linalg.norm(np.cross(v[0], v[1])), float(np.dot(v[0], v[1])))
+        d = O.distance_haversine(*map(math.radians, (lon1, lon2, lat1, lat2)))
+        assert d == pytest.approx(ang, rel=1e-9)
+
+
+def test_spectral_grid_constants(O):
+    """src/umwm_init.F90:1254-1335: f log-spaced, th centred on 0, cth2pp
+    circulant with weights cos^2 * dth, bf1 + bf2 = 1, dom = 2 pi dlnf f."""
+    o = regional(O, mm=6, nm=5, om=37, pm=36)
+    f = o.get("f")
+    dlnf = (math.log(2.0) - math.log(0.0313)) / 36
+    assert np.allclose(f, 0.0313 * np.exp(dlnf * np.arange(37)), rtol=1e-14)
+    th = o.get("th")
+    dth = 2 * math.pi / 36
+    assert np.allclose(th, (np.arange(1, 37) - 18.5) * dth, rtol=0, atol=1e-15)
+    c2 = o.get("cth2pp")  # [p-1, pp-1]
+    for p in range(36):
+        for pp in range(36):
+            assert c2[p, pp] == pytest.approx(math.cos(th[pp] - th[p]) ** 2 * dth, abs=1e-15)
+    # each row of cos^2 weights integrates to pi
+    assert np.allclose(c2.sum(axis=1), math.pi, rtol=1e-13)
+    assert np.allclose(o.get("dom"), 2 * math.pi * dlnf * f, rtol=1e-14)
+    # bf1, bf2 renormalised: bf1_renorm + bf2_renorm = snl_fac (1 - snl_arg)/..., and
+    # with kdk cancelled on a uniform-depth grid the two weights sum to one
+    b1 = math.exp(-16 * dlnf * dlnf)
+    b2 = math.exp(-64 * dlnf * dlnf)
+    kdk = o.get("kdk")
+    r1 = o.get("bf1_renorm")[:, :35] / (5.0 * kdk[:, 1:36] / kdk[:, :35])
+    r2 = o.get("bf2_renorm")[:, :35] / (5.0 * kdk[:, 2:37] / kdk[:, :35])
+    assert np.allclose(r1, b1 / (b1 + b2), rtol=1e-13)
+    assert np.allclose(r2, b2 / (b1 + b2), rtol=1e-13)
+    assert np.allclose(o.get("snl_arg")[:, :35], 1 - o.get("bf1_renorm")[:, :35]
+                       - o.get("bf2_renorm")[:, :35], rtol=1e-14)
+    assert np.all(o.get("snl_arg")[:, 35:] == 0)
+
+
+@pytest.mark.parametrize("pm,expect", [(36, 1 / math.sqrt(2)), (32, None)])
+def test_cfl_limit(O, pm, expect):
+    """src/umwm_init.F90:1277-1281, 1336-1340: cfllim = cos(pi/4 - dth/2)
+    when pm % 8 == 0 else 1/sqrt(2); dtamin = 0.98 cfllim min(dx, dy) / max cg."""
+    o = regional(O, mm=6, nm=5, om=25, pm=pm, delx=12000.0, dely=9000.0)
+    dth = 2 * math.pi / pm
+    cfl = expect if expect is not None else math.cos(math.pi / 4 - dth / 2)
+    cgmax = float(o.get("cg0").max())
+    dtamin = 0.98 * cfl * 9000.0 / cgmax
+    o.substep()  # cold substep: dts = 0
+    o.substep()
+    assert o.scalar("dts") <= dtamin * (1 + 1e-14)
+    # with a tiny explim bound disabled, dts is exactly the CFL limit
+    p = abi.make_params(mm=6, nm=5, om=25, pm=pm, delx=12000.0, dely=9000.0, explim=1e9)
+    o2 = O.Oracle(p, None, None, threads=1)
+    o2.substep()
+    o2.substep()
+    assert o2.scalar("dts") == pytest.approx(dtamin, rel=1e-14)
+
+
+# ------------------------------------------------------------- dispersion
+def test_dispersion_against_root_solve(O):
+    """src/umwm_init.F90:1422-1496: k solves w^2 = g k tanh(kd)(1 + sigma k^2/(rho g));
+    Newton stops at a 1e-2 nondimensional step, so k matches the exact root to
+    about 1e-4 relative; cp = w/k; cg from the closed form with the uncapped kd."""
+    depth = np.full((6, 5), 35.0)
+    depth[2, 2] = 4000.0
+    o = regional(O, mm=6, nm=5, om=37, pm=8, depth=depth)
+    f = o.get("f")
+    k = o.get("k")
+    d = o.get("d")[: o.im]
+    cp = o.get("cp0")
+    cg = o.get("cg0")
+    sig, rho = 0.07, 1030.0
+    for i in range(o.im):
+        for q in range(37):
+            w = 2 * math.pi * f[q]
+            fn = lambda x: G * x * math.tanh(x * d[i]) * (1 + sig * x * x / (rho * G)) - w * w
+            kx = optimize.brentq(fn, 1e-8, 1e5, xtol=1e-14, rtol=1e-15)
+            assert k[i, q] == pytest.approx(kx, rel=2e-4)
+            assert cp[i, q] == pytest.approx(w / k[i, q], rel=1e-13)
+            kd = k[i, q] * d[i]
+            cgx = cp[i, q] * (0.5 + kd / math.sinh(2 * min(kd, 20.0))
+                              + sig * k[i, q] ** 2 / (rho * G + sig * k[i, q] ** 2))
+            assert cg[i, q] == pytest.approx(cgx, rel=1e-12)
+    # deep water, gravity range: cp -> g/w, cg -> cp/2
+    ideep = int(np.argmax(d))
+    for q in range(8):
+        w = 2 * math.pi * f[q]
+        assert cp[ideep, q] == pytest.approx(G / w, rel=1e-3)
+        assert cg[ideep, q] / cp[ideep, q] == pytest.approx(0.5, rel=2e-3)
+
+
+# -------------------------------------------------------------- JONSWAP + diag
+def test_jonswap_hs_closed_form(O):
+    """Initial JONSWAP (alpha 0.01, gamma 3.3, fp 0.3 Hz) on the 37-bin grid with
+    cos^2 spreading: Hs = 4 sqrt(sum_o S(f_o) f_o dlnf) = 0.607 m, reproduced by
+    diag's Hs = 4 sqrt(sum e kdk dth) (src/umwm_physics.F90:148-154)."""
+    o = regional(O, mm=6, nm=5, om=37, pm=36, dpt=4000.0, wspd0=15.0)
+    o.init_jonswap(abi.make_jonswap(alpha=0.01, gamma=3.3, fp=0.3, dir_from_wind=True))
+    f = o.get("f")
+    dlnf = math.log(f[1] / f[0])
+    fp = 0.3
+    sig = np.where(f <= fp, 0.07, 0.09)
+    S = (0.01 * G * G * (2 * math.pi) ** -4 * f ** -5 * np.exp(-1.25 * (fp / f) ** 4)
+         * 3.3 ** np.exp(-(f - fp) ** 2 / (2 * sig * sig * fp * fp)))
+    hs = 4 * math.sqrt(float(np.sum(S * f * dlnf)))
+    assert hs == pytest.approx(0.607, abs=5e-4)
+    o.diag()
+    assert np.allclose(o.get("ht"), hs, rtol=1e-12)
+    # mean period sqrt(m0 / m2) of the same spectrum
+    tm = math.sqrt(np.sum(S * f * dlnf) / np.sum(f * f * S * f * dlnf))
+    assert np.allclose(o.get("mwp"), tm, rtol=1e-10)
+
+
+# ---------------------------------------------------------------- source terms
+def test_sds_prefix_convolution_isotropic(O):
+    """src/umwm_source_functions.F90:85-105: for a direction-independent spectrum
+    a(o), mss(o) = pi * sum_{o' < o} k3dk(o') a(o') (exclusive prefix), and
+    sds = 2 pi sds_fac f (1 + mss_fac mss)^2 (a k4)^sds_power."""
+    o = regional(O, mm=6, nm=5, om=25, pm=24, dpt=600.0)
+    rng = np.random.default_rng(3)
+    a = 1e-4 * rng.uniform(0.2, 1.0, size=(o.im, o.om)) * np.exp(-0.2 * np.arange(o.om))
+    e = o.get("e")
+    e[:] = a[:, None, :]
+    o.sds_d12()
+    k3dk, k4, f = o.get("k3dk"), o.get("k4"), o.get("f")
+    mss = math.pi * np.concatenate([np.zeros((o.im, 1)), np.cumsum(k3dk * a, axis=1)[:, :-1]],
+                                   axis=1)
+    dummy = (1 + 360.0 * mss) ** 2
+    sds = 2 * math.pi * 42.0 * f[None, :] * dummy * (a * k4) ** 2.4
+    for p in (0, 7, 23):
+        assert np.allclose(o.get("dummy")[:, p, :], dummy, rtol=1e-12)
+        assert np.allclose(o.get("sds")[:, p, :], sds, rtol=1e-12)
+
+
+def test_snl_energy_telescopes(O):
+    """src/umwm_source_functions.F90:156-183: with X = sds e taken before the
+    cothkd factor and bf1 + bf2 = 1, sum_{o <= oc} kdk snl telescopes to
+    snl_fac [bf1 (Y(oc+1) - Y(1)) + bf2 (Y(oc+1) + Y(oc+2) - Y(1) - Y(2))],
+    Y = kdk X (the energy-flux bookkeeping of src/umwm_stress.F90:100-101)."""
+    o = regional(O, mm=7, nm=6, om=37, pm=36, dpt=50.0, wspd0=9.0)
+    o.init_jonswap(abi.make_jonswap(fp=0.2))
+    o.sin_d12()
+    o.sds_d12()
+    o.snl_d12()
+    f = o.get("f")
+    dlnf = math.log(f[1] / f[0])
+    b1, b2 = math.exp(-16 * dlnf ** 2), math.exp(-64 * dlnf ** 2)
+    b1, b2 = b1 / (b1 + b2), b2 / (b1 + b2)
+    kdk, coth = o.get("kdk"), o.get("cothkd")
+    sds = o.get("sds") / coth[:, None, :]  # undo the cothkd factor applied after snl
+    X = sds * o.get("e")
+    Y = kdk[:, None, :] * X
+    snl = o.get("snl")
+    oc = o.get("oc")[: o.im]
+    for i in range(o.im):
+        c = int(oc[i])
+        lhs = np.sum(kdk[i, None, :c] * snl[i, :, :c], axis=1)
+        yy = Y[i]
+        rhs = 5.0 * (b1 * (yy[:, c] - yy[:, 0]) + b2 * (yy[:, c] + yy[:, c + 1] - yy[:, 0] - yy[:, 1]))
+        assert np.allclose(lhs, rhs, rtol=1e-11, atol=1e-12 * np.abs(yy).max())
+
+
+def test_sin_shelter_and_signs(O):
+    """src/umwm_source_functions.F90:16-82 in special cases: waves running
+    against the wind are damped (ssin < 0 up to oc), waves slower than the wind
+    and along it grow, waves faster than the wind along it are weakly damped,
+    bins above oc are clamped at >= 0, and growth rises with wind speed."""
+    o = regional(O, mm=6, nm=5, om=37, pm=36, dpt=4000.0, wspd0=12.0, wdir0=0.0)
+    o.init_jonswap(abi.make_jonswap())
+    o.forcinginterpolate()  # sets rhorat = rhoa / rhow
+    o.sin_d12()
+    s = o.get("ssin")
+    oc = int(o.get("oc")[0])
+    th = o.get("th")
+    down = int(np.argmin(np.abs(th)))          # along the wind
+    up = int(np.argmin(np.abs(np.abs(th) - math.pi)))  # against it
+    cp = o.get("cp0")[0]
+    slow = oc - 2          # cp < U: growth
+    fast = 2               # cp > U: x < 0 along the wind, sin_diss2 damping
+    assert cp[slow] < 12.0 < cp[fast]
+    assert np.all(s[:, down, slow] > 0)
+    assert np.all(s[:, down, fast] <= 0)
+    assert np.all(np.abs(s[:, down, fast]) < 1e-2 * np.abs(s[:, up, fast]))
+    assert np.all(s[:, up, : oc] < 0)
+    assert np.all(s[:, :, oc:] >= 0)
+    o2 = regional(O, mm=6, nm=5, om=37, pm=36, dpt=4000.0, wspd0=20.0, wdir0=0.0)
+    o2.init_jonswap(abi.make_jonswap())
+    o2.forcinginterpolate()
+    o2.sin_d12()
+    assert np.all(o2.get("ssin")[:, down, slow] > s[:, down, slow])
+
+
+# -------------------------------------------------------------- propagation
+def _blob(o, centre, width=0.8):
+    mi, ni = o.get("mi")[: o.im], o.get("ni")[: o.im]
+    r2 = (mi - centre[0]) ** 2 + (ni - centre[1]) ** 2
+    return np.exp(-r2 / (2 * width * width))
+
+
+def test_propagation_conserves_and_moves_at_cg(O):
+    """src/umwm_advection.F90:18-99 on a uniform all-sea grid: the upwind
+    update conserves sum(e ar) for a blob away from the coast, and the first
+    moment moves by exactly cg cos(th) dta in x and cg sin(th) dta in y."""
+    o = regional(O, mm=17, nm=17, om=25, pm=24, dpt=4000.0, delx=5000.0, dely=4000.0,
+                 wspd0=15.0)
+    o.sin_d12()  # sets oc
+    blob = _blob(o, (9.0, 9.0), width=0.7)
+    e, ef = o.get("e"), o.get("ef")
+    e[:] = 1e-3 * blob[:, None, None]
+    ef[:] = e
+    dta = 60.0
+    o.set_scalar("dta", dta)
+    o.propagation()
+    de = o.get("ef") - e
+    mi, ni = o.get("mi")[: o.im], o.get("ni")[: o.im]
+    x, y = (mi - 1) * 5000.0, (ni - 1) * 4000.0
+    cg = o.get("cg0")
+    th = o.get("th")
+    oc = int(o.get("oc")[0])
+    for p in (0, 5, 13):
+        for q in (2, oc - 2):
+            tot = float(np.sum(e[:, p, q]))
+            assert abs(float(np.sum(de[:, p, q]))) < 1e-12 * tot
+            dx_mean = float(np.sum(x * de[:, p, q])) / tot
+            dy_mean = float(np.sum(y * de[:, p, q])) / tot
+            c = cg[0, q]
+            assert dx_mean == pytest.approx(c * math.cos(th[p]) * dta, rel=1e-9, abs=1e-9)
+            assert dy_mean == pytest.approx(c * math.sin(th[p]) * dta, rel=1e-9, abs=1e-9)
+    # bins above oc are not propagated
+    assert np.all(de[:, :, oc:] == 0)
+
+
+def test_propagation_uniform_field_is_steady(O):
+    """A uniform spectrum with uniform cg has zero flux divergence away from the
+    coast."""
+    o = regional(O, mm=10, nm=9, om=25, pm=24, dpt=4000.0, wspd0=15.0)
+    o.sin_d12()
+    e, ef = o.get("e"), o.get("ef")
+    e[:] = 2e-3
+    ef[:] = e
+    o.set_scalar("dta", 100.0)
+    o.propagation()
+    mi, ni = o.get("mi")[: o.im], o.get("ni")[: o.im]
+    inner = (mi > 2) & (mi < 9) & (ni > 2) & (ni < 8)
+    assert np.allclose(o.get("ef")[inner], 2e-3, rtol=1e-14)
+
+
+def test_refraction_conserves_each_frequency(O):
+    """src/umwm_advection.F90:102-180 over a sloping shelf: the periodic upwind
+    flux in direction conserves sum_p e for every (o, i); the rotation is the
+    signed max over the grid giving dtr = min(dth / max rot, dts)."""
+    depth = 20.0 + 80.0 * (np.arange(10)[:, None] / 9.0) * np.ones((1, 8))
+    o = regional(O, mm=10, nm=8, om=25, pm=24, depth=depth, wspd0=10.0)
+    o.init_jonswap(abi.make_jonswap(fp=0.12))
+    o.sin_d12()
+    e = o.get("e")
+    o.get("ef")[:] = e
+    o.set_scalar("first", 1)
+    o.set_scalar("dts", 300.0)
+    o.refraction()
+    de = o.get("ef") - e
+    assert np.allclose(de.sum(axis=1), 0, atol=1e-13 * float(e.sum(axis=1).max()))
+    assert float(np.abs(de).max()) > 0
+    rotl = o.get("rotl")
+    dtr_temp = o.scalar("dtr_temp")
+    rmax = float(np.max(np.concatenate([rotl.ravel(), o.get("rotr").ravel()])))
+    # edge averages never exceed the cell-centre maximum
+    assert 2 * math.pi / 24 / dtr_temp >= rmax * (1 - 1e-12)
+
+
+# ----------------------------------------------------------------- index maps
+def _mask_case(O, z, isglobal=False, fill=0):
+    mm, nm = z.shape
+    kw = dict(mm=mm, nm=nm, om=25, pm=24, topofromfile=1, fillestuaries=fill)
+    grid = dict(z=z)
+    if isglobal:
+        dlon = 360.0 / mm
+        lon = (dlon / 2 + dlon * np.arange(mm))[:, None] * np.ones((1, nm))
+        lat = np.ones((mm, 1)) * (-30 + 20.0 * np.arange(nm))[None, :]
+        grid.update(lon=lon, lat=lat)
+        kw.update(isglobal=1, gridfromfile=1)
+    p = abi.make_params(**kw)
+    return O.Oracle(p, abi.make_grid_input(**grid), None, threads=1)
+
+
+def test_maps_regional_hand_enumerated(O):
+    """src/umwm_init.F90:571-694, 875-1014, 1195-1200: ring masked, sea points
+    first with m outer and n inner, land after; aliased neighbours point at
+    the ghost (0) on land, true neighbours keep the land index."""
+    z = np.full((5, 4), -100.0)
+    z[2, 1] = 5.0  # land at (m=3, n=2)
+    o = _mask_case(O, z)
+    assert o.im == 5
+    ii = o.get("ii")  # [n-1, m-1]
+    expect_ii = np.array([[6, 10, 12, 15, 17],
+                          [7, 1, 13, 4, 18],
+                          [8, 2, 3, 5, 19],
+                          [9, 11, 14, 16, 20]])
+    assert np.array_equal(ii, expect_ii)
+    assert list(o.get("mi")[:5]) == [2, 2, 3, 4, 4]
+    assert list(o.get("ni")[:5]) == [2, 3, 3, 2, 3]
+    assert list(o.get("iis")[:5]) == [10, 1, 13, 15, 4]
+    assert list(o.get("iin")[:5]) == [2, 11, 14, 5, 16]
+    assert list(o.get("iiw")[:5]) == [7, 8, 2, 13, 3]
+    assert list(o.get("iie")[:5]) == [13, 3, 5, 18, 19]
+    assert list(o.get("is")[:5]) == [0, 1, 0, 0, 4]
+    assert list(o.get("in")[:5]) == [2, 0, 0, 5, 0]
+    assert list(o.get("iw")[:5]) == [0, 0, 2, 0, 3]
+    assert list(o.get("ie")[:5]) == [0, 3, 5, 0, 0]
+    mask = o.get("mask")
+    assert mask.sum() == 5 and mask[1, 2] == 0
+
+
+def test_maps_global_periodic(O):
+    """Periodic east-west neighbours (src/umwm_init.F90:993-1007)."""
+    o = _mask_case(O, np.full((3, 4), -500.0), isglobal=True)
+    assert o.im == 6
+    assert list(o.get("iiw")[:6]) == [5, 6, 1, 2, 3, 4]
+    assert list(o.get("iie")[:6]) == [3, 4, 5, 6, 1, 2]
+    assert list(o.get("iis")[:6]) == [7, 1, 9, 3, 11, 5]
+    assert list(o.get("iin")[:6]) == [2, 8, 4, 10, 6, 12]
+    assert list(o.get("is")[:6]) == [0, 1, 0, 3, 0, 5]
+    assert list(o.get("in")[:6]) == [2, 0, 4, 0, 6, 0]
+
+
+def test_fill_estuaries(O):
+    """src/umwm_init.F90:607-636: sea cells with at most one sea neighbour are
+    filled repeatedly, so a one-cell channel off an open basin disappears."""
+    z = np.full((9, 8), -100.0)
+    z[:, 4:] = 5.0        # land in the north
+    z[4, 4:7] = -30.0     # a one-cell-wide inlet running north
+    o0 = _mask_case(O, z, fill=0)
+    o1 = _mask_case(O, z, fill=1)
+    assert o0.im - o1.im == 3
+    m1 = o1.get("mask")
+    assert np.all(m1[4:7, 4] == 0)
+
+
+# --------------------------------------------------------------- time step
+def test_cold_first_substep_and_calm_sea(O):
+    """src/umwm_physics.F90:53: the first substep of a cold start has dts = 0
+    and leaves the prognostic bins (o <= oc) unchanged while the diagnostic
+    tail is replaced; with a near-calm sea (e = tiny) and no wind input above
+    the floor the spectrum stays non-negative."""
+    o = regional(O, mm=6, nm=5, om=25, pm=24, dpt=1000.0, wspd0=0.0)
+    e0 = o.get("e").copy()
+    o.substep()
+    assert o.scalar("dts") == 0.0
+    oc = o.get("oc")[: o.im]
+    for i in range(o.im):
+        assert np.array_equal(o.get("e")[i, :, : oc[i]], e0[i, :, : oc[i]])
+    for _ in range(3):
+        o.substep()
+    assert np.all(o.get("e") >= 0)

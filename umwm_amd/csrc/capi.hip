@@ -825,6 +825,8 @@ int umwm_hip_create(const umwm_params* params, const umwm_grid_input* grid,
   if (!params || !out) return fail(UMWM_ERR_ARGUMENT, "null argument");
   if (precision != UMWM_FP32 && precision != UMWM_FP64)
     return fail(UMWM_ERR_ARGUMENT, "precision must be 4 or 8");
+  const std::string perr = check_params(*params);
+  if (!perr.empty()) return fail(UMWM_ERR_ARGUMENT, perr);
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
     return fail(UMWM_ERR_HIP, "no HIP device visible");
