@@ -1,0 +1,83 @@
+// Memory access helpers of the fused kernels.
+//
+// Every per-point stream of the i-contiguous layout is read and written
+// through raw buffer instructions: a row of ldi values (one (o, p) plane of
+// a spectrum, one frequency of a table, one per-point array) becomes a
+// buffer resource held in scalar registers, and each lane addresses it with
+// a 32-bit byte offset (its own point, or a neighbour's) computed once per
+// thread.  The address arithmetic of a row change is then scalar work, the
+// loads carry no 64-bit vector address math, and they are counted by
+// vmcnt alone, so a load issued ahead of its use is waited for exactly.
+// An offset past the row's end reads 0 and drops stores.
+//
+// Tables indexed by a wave-uniform direction or frequency (cth, sth, f ...)
+// are read through the constant address space so they become scalar loads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace umwm {
+
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+
+// Word 3 of a gfx9-family buffer descriptor: 32-bit data format, no
+// swizzle, raw (untyped) addressing.
+constexpr int kRsrcWord3 = 0x00020000;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           kRsrcWord3);
+}
+
+// One row of `n` values of type T starting at `base + row * n` (a row may
+// span several planes of ldi values, addressed by a scalar offset).
+template <typename T>
+__device__ __forceinline__ rsrc_t row_rsrc(const T* base, size_t row, size_t n) {
+  return make_rsrc(base + row * n, (uint32_t)(n * sizeof(T)));
+}
+
+// Loads and stores with a per-lane byte offset `voff` and a wave-uniform
+// byte offset `soff` (held in a scalar register).
+template <typename T>
+__device__ __forceinline__ T bld(rsrc_t r, uint32_t voff, uint32_t soff = 0);
+template <>
+__device__ __forceinline__ float bld<float>(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float,
+                            __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ double bld<double>(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double,
+                            __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ int32_t bld<int32_t>(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+
+__device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, float v, uint32_t soff = 0) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff,
+                                        (int)soff, 0);
+}
+__device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, double v, uint32_t soff = 0) {
+  using u2 = __attribute__((ext_vector_type(2))) uint32_t;
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, int32_t v, uint32_t soff = 0) {
+  __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, r, (int)voff, (int)soff, 0);
+}
+
+// Scalar (constant address space) load of a wave-uniform element.
+template <typename T>
+__device__ __forceinline__ T cld(const T* p, int idx) {
+  return ((const __attribute__((address_space(4))) T*)p)[idx];
+}
+
+// Wave index of the calling thread as a wave-uniform value.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
+}  // namespace umwm

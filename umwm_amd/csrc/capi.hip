@@ -218,6 +218,15 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
 
   const int om = S.om, pm = S.pm, ldi = S.ldi, ldq = S.ldq;
   const size_t ns = (size_t)om * pm * ldi, nt = (size_t)om * ldi;
+  // the fused kernels address one frequency of a spectrum (pm rows), of the
+  // table block (FT_N rows) or of a per-point array with one 32-bit buffer
+  // descriptor each
+  const size_t maxrows = std::max<size_t>({(size_t)pm, (size_t)FT_N, 3});
+  if (maxrows * ldi * std::max(sizeof(T), sizeof(double)) >= (size_t(1) << 32) ||
+      (size_t)ldq * sizeof(double) >= (size_t(1) << 32)) {
+    delete D;
+    return fail(UMWM_ERR_SIZE, "domain too large for 32-bit buffer descriptors");
+  }
   h.e = A.template alloc<T>(ns);
   h.ef = A.template alloc<T>(ns);
   h.es = A.template alloc<T>(ns);
@@ -263,6 +272,25 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   h.snl_arg = sarg; h.bf1r = b1; h.bf2r = b2; h.dwn = dwn; h.cth_curv = ccv; h.sth_curv = scv;
   h.f = f; h.th = th; h.cth = ct; h.sth = st; h.c2th = c2; h.s2th = s2; h.cth2pp = c2pp;
   h.pl = pl; h.pr = pr;
+  // interleaved copy of the tables for the fused kernels, [o][FT_N][ldi]
+  {
+    const std::vector<double>* ft[FT_N] = {&S.cp0, &S.logl2overz, &S.fkovg, &S.k4, &S.cothkd,
+                                           &S.sbf, &S.sdv, &S.snl_arg, &S.k, &S.cg0, &S.k3dk,
+                                           &S.bf1r, &S.bf2r, &S.oneoverk4, &S.kdk};
+    std::vector<double> blk((size_t)FT_N * nt, 0.0);
+    for (int o = 0; o < om; ++o)
+      for (int t = 0; t < FT_N; ++t) {
+        const double* src = ft[t]->data() + (size_t)o * ldi;
+        double* dst = blk.data() + ((size_t)o * FT_N + t) * ldi;
+        std::copy(src, src + ldi, dst);
+      }
+    h.ftab = A.template alloc<T>((size_t)FT_N * nt);
+    if (!h.ftab) {
+      delete D;
+      return fail(UMWM_ERR_NOMEM, "device allocation of the table block failed");
+    }
+    if ((rc = upload<T>(h.ftab, blk, (size_t)FT_N * nt, s))) { delete D; return rc; }
+  }
   // per unrolled point
   T** unr[] = {&h.wspd, &h.wdir, &h.uc, &h.vc, &h.rhoa, &h.rhow, &h.rhorat, &h.fice};
   for (T** p : unr) *p = A.template alloc<T>(ldq);

@@ -85,14 +85,23 @@ struct DomainDev {
   double *ustar, *cd, *taux, *tauy, *taux_form, *tauy_form, *taux_skin;
   double *tauy_skin, *taux_diag, *tauy_diag, *tailatmx, *tailatmy;
   double *us1, *vs1;  // surface Stokes drift used by stress('atm')
-  // fused-path scratch: exclusive frequency prefix of the cos^2 directional
-  // convolution times dth/2, [3][om][ldi] (written by k_rates), and the
-  // per-frequency stress / wave-height partial sums [3][om][ldi]
+  // fused-path copy of the (o, j) tables interleaved by frequency,
+  // [om][FT_N][ldi], so one buffer descriptor per frequency reaches all of
+  // them; the exclusive frequency prefix of the cos^2 directional
+  // convolution times dth/2, [om][3][ldi] (written by k_rates); the
+  // per-frequency stress / wave-height partial sums [om][3][ldi]
+  T* ftab;
   T* pfx;
   double* spart;
   // reduction scratch
   double *part_min, *part_max;
   Scalars* sc;
+};
+
+// Rows of the interleaved table block DomainDev::ftab.
+enum FusedTable {
+  FT_CP0, FT_LOGL2, FT_FKOVG, FT_K4, FT_COTH, FT_SBF, FT_SDV, FT_SARG, FT_K, FT_CG0, FT_K3DK,
+  FT_B1, FT_B2, FT_OK4, FT_KDK, FT_N
 };
 
 // One workgroup of a fused launch: which domain, which 64 sea points.

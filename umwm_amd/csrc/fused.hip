@@ -11,8 +11,8 @@
 //                tail (src/umwm_physics.F90:60-84); writes E_s (ef) and
 //                E* = (E^n + E_s)/2 (es).
 //   k_advect     upwind propagation on E* (src/umwm_advection.F90:18-99),
-//                refraction (:102-180) and e = ef, one thread per
-//                (frequency, sea point); also the per-frequency partial
+//                refraction (:102-180) and e = ef, one wave per
+//                (frequency, 64 sea points); also the per-frequency partial
 //                sums of stress_vector_integral (src/umwm_stress.F90:313-331).
 //   k_stress     per point: frequency sums, tail and the FP64 form/skin
 //                drag chain of stress('atm') (src/umwm_stress.F90:216-264).
@@ -26,105 +26,191 @@
 // uses cos^2(a-b) = (1 + cos2a cos2b + sin2a sin2b)/2: three directional
 // moments per frequency, exchanged between waves through LDS when W > 1,
 // which turns the O(pm^2) convolution into O(pm).  Each sweep prefetches
-// the next frequency's spectrum and tables while computing the current.
+// the next frequency's spectrum (and, in FP32, its tables) while computing
+// the current one.
 //
 // snl(o) = bf1r(o) X(o+1) + bf2r(o) X(o+2) - snl_fac X(o), X = sds e
 // (src/umwm_source_functions.F90:165-171) is folded into the integration
 // as each X becomes known: k_integrate keeps per direction only the
 // partial E_s of the two previous frequencies and their E^n.
 //
-// Per-direction angles use the angle-sum identity with per-point cos/sin
-// (cos(wdir - th) = cos wdir cos th + sin wdir sin th, and the same for the
-// curvature-rotated ray directions of src/umwm_init.F90:1304-1309).
+// Wind input per bin (src/umwm_source_functions.F90:38-80) is evaluated
+// as y = |x| x times one of three per-frequency factors (sheltering for
+// y > 0, sin_diss1 against the wind, sin_diss1 fieldscale2 for swell
+// overrunning it), with x = A cos th + B sin th - cp0 from the angle-sum
+// identity (A, B per frequency) - the reference's expression regrouped,
+// equal to it up to rounding.
+//
+// All per-point streams go through raw buffer loads and stores (bufio.hpp):
+// scalar row descriptors, 32-bit lane offsets computed once per thread.
 #include <hip/hip_runtime.h>
 
+#include "bufio.hpp"
 #include "domain.hpp"
 #include "physics.hpp"
 
 namespace umwm {
 namespace {
 
-// Per-point prologue of sin_d12 (src/umwm_source_functions.F90:21-52):
-// wind floor, cut-off bin, sheltering.  k_rates evaluates it and wave 0
-// stores the results; the later kernels of the substep read them back
-// (load_point_fast) instead of repeating the search and the exp.
+template <typename U>
+__device__ __forceinline__ U pload(const U* a, size_t n, int j) {
+  return bld<U>(make_rsrc(a, (uint32_t)(n * sizeof(U))), (uint32_t)j * (uint32_t)sizeof(U));
+}
+template <typename U>
+__device__ __forceinline__ void pstore(U* a, size_t n, int j, U v) {
+  bst(make_rsrc(a, (uint32_t)(n * sizeof(U))), (uint32_t)j * (uint32_t)sizeof(U), v);
+}
+
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+
+// Per-point inputs of the source terms (src/umwm_source_functions.F90:21-52,
+// :174-177).
 template <typename T>
-__device__ inline void load_point(const DomainDev<T>& d, int j, SinPoint<T>& s, T& csdt,
-                                  double& ice, int& oc, T& cw, T& sw, bool write) {
-  T w = d.wspd[j];
+struct PointState {
+  T wspd, a;   // floored wind speed, 2.5 ustar
+  T pre;       // (1 - fice) 2 pi rhorat
+  T shelt;     // sheltering coefficient
+  T uc, vc;
+  T cw, sw;    // cos, sin of wdir
+  T csdt;      // sdt_fac sqrt(rhorat) ustar
+  T ice;       // Kohout attenuation coefficient (0 below fice_lth)
+  int oc;      // 0-based last prognostic bin
+};
+
+// Fresh evaluation (k_rates): wind floor, cut-off bin, sheltering; wave 0
+// stores them for the later kernels of the substep.
+template <typename T>
+__device__ inline PointState<T> point_fresh(const DomainDev<T>& d, int j, bool write) {
+  const size_t ldi = d.ldi, ldq = d.ldq;
+  PointState<T> s;
+  T w = pload(d.wspd, ldq, j);
   w = w > T(1e-2) ? w : T(1e-2);
   T fc = T(0.53) * T(d.g) / w;
   if (fc > T(d.fprog)) fc = T(d.fprog);
-  oc = cutoff_bin(fc, d.f, d.om);
-  const T shelt = sheltering_coare35(w);
-  const T ust = T(d.ustar[j]);
-  const T rr = d.rhorat[j];
+  int oc = 0;
+  for (int o = d.om - 3; o >= 0; --o) {  // src/umwm_source_functions.F90:30-36
+    oc = o;
+    if (fc > cld(d.f, o)) break;
+  }
+  const T wdir = pload(d.wdir, ldq, j);
+  const T rr = pload(d.rhorat, ldq, j);
+  const T fice = pload(d.fice, ldq, j);
+  const T ust = T(pload(d.ustar, ldi, j));
   s.wspd = w;
   s.a = T(2.5) * ust;
-  s.pre = (T(1) - d.fice[j]) * T(kTwoPiD) * rr;
-  s.shelt = shelt;
-  s.uc = d.uc[j];
-  s.vc = d.vc[j];
+  s.pre = (T(1) - fice) * T(kTwoPiD) * rr;
+  s.shelt = sheltering_coare35(w);
+  s.uc = pload(d.uc, ldq, j);
+  s.vc = pload(d.vc, ldq, j);
+  s.cw = cos(wdir);
+  s.sw = sin(wdir);
+  s.csdt = T(d.sdt_fac) * sqrt(rr) * ust;
+  s.ice = fice > T(d.fice_lth) ? T(sice_coef(pload(d.hs_ice, ldi, j))) : T(0);
   s.oc = oc;
-  csdt = T(d.sdt_fac) * sqrt(rr) * ust;
-  ice = (d.fice[j] > T(d.fice_lth)) ? sice_coef(d.hs_ice[j]) : 0.0;
-  cw = cos(d.wdir[j]);
-  sw = sin(d.wdir[j]);
   if (write) {
-    d.wspd[j] = w;
-    d.fcutoff[j] = fc;
-    d.oc[j] = oc;
-    d.shelt[j] = shelt;
-    d.cwdir[j] = cw;
-    d.swdir[j] = sw;
+    pstore(d.wspd, ldq, j, w);
+    pstore(d.fcutoff, ldi, j, fc);
+    pstore(d.oc, ldi, j, (int32_t)oc);
+    pstore(d.shelt, ldi, j, s.shelt);
+    pstore(d.cwdir, ldi, j, s.cw);
+    pstore(d.swdir, ldi, j, s.sw);
   }
+  return s;
 }
 
+// Later kernels of the substep read back what k_rates stored.
 template <typename T>
-__device__ inline void load_point_fast(const DomainDev<T>& d, int j, SinPoint<T>& s, T& csdt,
-                                       double& ice, int& oc, T& cw, T& sw) {
-  const T ust = T(d.ustar[j]);
-  const T rr = d.rhorat[j];
-  oc = d.oc[j];
-  s.wspd = d.wspd[j];
+__device__ inline PointState<T> point_stored(const DomainDev<T>& d, int j) {
+  const size_t ldi = d.ldi, ldq = d.ldq;
+  PointState<T> s;
+  const T rr = pload(d.rhorat, ldq, j);
+  const T fice = pload(d.fice, ldq, j);
+  const T ust = T(pload(d.ustar, ldi, j));
+  s.wspd = pload(d.wspd, ldq, j);
   s.a = T(2.5) * ust;
-  s.pre = (T(1) - d.fice[j]) * T(kTwoPiD) * rr;
-  s.shelt = d.shelt[j];
-  s.uc = d.uc[j];
-  s.vc = d.vc[j];
-  s.oc = oc;
-  csdt = T(d.sdt_fac) * sqrt(rr) * ust;
-  ice = (d.fice[j] > T(d.fice_lth)) ? sice_coef(d.hs_ice[j]) : 0.0;
-  cw = d.cwdir[j];
-  sw = d.swdir[j];
+  s.pre = (T(1) - fice) * T(kTwoPiD) * rr;
+  s.shelt = pload(d.shelt, ldi, j);
+  s.uc = pload(d.uc, ldq, j);
+  s.vc = pload(d.vc, ldq, j);
+  s.cw = pload(d.cwdir, ldi, j);
+  s.sw = pload(d.swdir, ldi, j);
+  s.csdt = T(d.sdt_fac) * sqrt(rr) * ust;
+  s.ice = fice > T(d.fice_lth) ? T(sice_coef(pload(d.hs_ice, ldi, j))) : T(0);
+  s.oc = pload(d.oc, ldi, j);
+  return s;
 }
 
-// Rotation tendency of src/umwm_advection.F90:335-338 for one direction.
+// Per-frequency factors of the wind input: ssin = y * (y > 0 ? pos :
+// (overrun ? neg2 : neg1)), y = |x| x, x = A cth + B sth - cp0.
 template <typename T>
-__device__ inline T rot_dir(T dcx, T dcy, T sth, T cth, T vce, T vcw, T ucn, T ucs, T odx,
-                            T ody) {
-  return T(0.5) * (((dcx * sth + vce) - vcw) * odx - ((dcy * cth + ucn) - ucs) * ody);
+struct SinFreq {
+  T A, B, cp0, pos, neg1, neg2;
+};
+
+template <typename T>
+__device__ __forceinline__ SinFreq<T> sin_freq(const PointState<T>& s, T logl2, T cp0, T fkovg,
+                                               T sin_diss1, T fs2) {
+  SinFreq<T> r;
+  const T ba = s.wspd + s.a * logl2;
+  r.A = ba * s.cw - s.uc;
+  r.B = ba * s.sw - s.vc;
+  r.cp0 = cp0;
+  const T g = s.pre * fkovg;
+  r.pos = g * s.shelt;
+  r.neg1 = g * sin_diss1;
+  r.neg2 = r.neg1 * fs2;
+  return r;
 }
 
-// Per-frequency, per-point inputs of the source terms.
+template <typename T>
+__device__ __forceinline__ T sin_bin(const SinFreq<T>& q, const PointState<T>& s, T ct, T st,
+                                     bool tail) {
+  const T x = q.A * ct + q.B * st - q.cp0;
+  const T y = fabs(x) * x;
+  const T cwt = s.cw * ct + s.sw * st;  // cos(wdir - th)
+  T r = y * (y > T(0) ? q.pos : (cwt > T(0) ? q.neg2 : q.neg1));
+  if (tail && r < T(0)) r = T(0);  // src/umwm_source_functions.F90:73-80
+  return r;
+}
+
+// Rotation tendency of src/umwm_advection.F90:335-338 for one direction:
+// 0.5 ((dcx sth + dvc) odx - (dcy cth + duc) ody).
+template <typename T>
+__device__ __forceinline__ T rot_dir(T dcx, T dcy, T sth, T cth, T dvc, T duc, T odx, T ody) {
+  return T(0.5) * ((dcx * sth + dvc) * odx - (dcy * cth + duc) * ody);
+}
+
+// Per-frequency, per-point tables of the source terms.
 template <typename T>
 struct FreqTab {
   T cp0, logl2, fkovg, k4, coth, sbf, sdv, sarg, k, cg0, k3dk;
 };
 
+// Descriptor of frequency o's slice of a [om][n_rows][ldi] block (a
+// spectrum plane with n_rows = pm, the table block with FT_N, the prefix
+// block with 3); rows are then reached with the scalar offset row * ldi.
 template <typename T>
-__device__ inline void load_tab(const DomainDev<T>& d, size_t t, FreqTab<T>& r) {
-  r.cp0 = d.cp0[t];
-  r.logl2 = d.logl2overz[t];
-  r.fkovg = d.fkovg[t];
-  r.k4 = d.k4[t];
-  r.coth = d.cothkd[t];
-  r.sbf = d.sbf[t];
-  r.sdv = d.sdv[t];
-  r.sarg = d.snl_arg[t];
-  r.k = d.k[t];
-  r.cg0 = d.cg0[t];
-  r.k3dk = d.k3dk[t];
+__device__ __forceinline__ rsrc_t freq_rsrc(const T* base, int o, int n_rows, size_t ldi) {
+  return row_rsrc(base, (size_t)o, (size_t)n_rows * ldi);
+}
+
+template <typename T>
+__device__ __forceinline__ void load_tab(const DomainDev<T>& d, int o, uint32_t vj,
+                                         FreqTab<T>& r) {
+  const uint32_t rb = (uint32_t)d.ldi * (uint32_t)sizeof(T);
+  const rsrc_t tb = freq_rsrc(d.ftab, o, FT_N, d.ldi);
+  r.cp0 = bld<T>(tb, vj, FT_CP0 * rb);
+  r.logl2 = bld<T>(tb, vj, FT_LOGL2 * rb);
+  r.fkovg = bld<T>(tb, vj, FT_FKOVG * rb);
+  r.k4 = bld<T>(tb, vj, FT_K4 * rb);
+  r.coth = bld<T>(tb, vj, FT_COTH * rb);
+  r.sbf = bld<T>(tb, vj, FT_SBF * rb);
+  r.sdv = bld<T>(tb, vj, FT_SDV * rb);
+  r.sarg = bld<T>(tb, vj, FT_SARG * rb);
+  r.k = bld<T>(tb, vj, FT_K * rb);
+  r.cg0 = bld<T>(tb, vj, FT_CG0 * rb);
+  r.k3dk = bld<T>(tb, vj, FT_K3DK * rb);
 }
 
 // ---------------------------------------------------------------------
@@ -138,38 +224,37 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
   const BlockTask tk = tasks[blockIdx.x];
   const DomainDev<T>& d = doms[tk.dom];
   Scalars* sc = d.sc;
-  const int W = blockDim.x >> 6, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int W = blockDim.x >> 6, lane = threadIdx.x & 63, w = wave_id();
   const int j = tk.i0 + lane;
   const bool active = sc->sumt < d.dtg;
   const bool valid = active && j < d.im;
   const int pm = d.pm, om = d.om;
-  const size_t ldi = d.ldi, sl = (size_t)pm * ldi;
+  const size_t ldi = d.ldi, ldq = d.ldq;
   const int D = (pm + W - 1) / W, p0 = w * D;
   const int nd = max(0, min(D, pm - p0));  // directions of this wave (uniform)
+  const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
+  const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);  // bytes of one row
   double pts = INFINITY, rotmax = -INFINITY;
 
   if (active) {
-    SinPoint<T> s;
-    T csdt;
-    double ice;
-    int oc;
-    T cw, sw;
-    load_point(d, j, s, csdt, ice, oc, cw, sw, valid && w == 0);
-    if (rotation && valid && w == 0) d.oc_rot[j] = oc;
-    int ie = 0, iw = 0, in = 0, is = 0;
-    T vce = 0, vcw = 0, ucn = 0, ucs = 0, odx = 0, ody = 0;
+    const PointState<T> s = point_fresh(d, j, valid && w == 0);
+    const int oc = s.oc;
+    if (rotation && valid && w == 0) pstore(d.oc_rot, ldi, j, (int32_t)oc);
+    uint32_t vie = 0, viw = 0, vin = 0, vis = 0;
+    T dvc = 0, duc = 0, odx = 0, ody = 0;
     if (rotation) {
-      ie = d.ie[j]; iw = d.iw[j]; in = d.in_[j]; is = d.is_[j];
-      vce = d.vc[d.iie[j]]; vcw = d.vc[d.iiw[j]]; ucn = d.uc[d.iin[j]]; ucs = d.uc[d.iis[j]];
-      odx = d.oneovdx[j];
-      ody = d.oneovdy[j];
+      vie = (uint32_t)pload(d.ie, ldi, j) * (uint32_t)sizeof(T);
+      viw = (uint32_t)pload(d.iw, ldi, j) * (uint32_t)sizeof(T);
+      vin = (uint32_t)pload(d.in_, ldi, j) * (uint32_t)sizeof(T);
+      vis = (uint32_t)pload(d.is_, ldi, j) * (uint32_t)sizeof(T);
+      dvc = pload(d.vc, ldq, pload(d.iie, ldi, j)) - pload(d.vc, ldq, pload(d.iiw, ldi, j));
+      duc = pload(d.uc, ldq, pload(d.iin, ldi, j)) - pload(d.uc, ldq, pload(d.iis, ldi, j));
+      odx = pload(d.oneovdx, ldi, j);
+      ody = pload(d.oneovdy, ldi, j);
     }
-    const T sin_fac = T(d.sin_fac), fs1 = T(d.fieldscale1), fs2 = T(d.fieldscale2);
+    const T sin_diss1 = T(d.sin_fac * d.fieldscale1), fs2 = T(d.fieldscale2);
     const T tpsf = T(d.twopisds_fac), spow = T(d.sds_power), mssf = T(d.mss_fac);
     const double hdth = 0.5 * d.dth;
-    T* __restrict__ PF = d.pfx + j;
-    const T* __restrict__ EN = d.e + j;
-    const size_t pstr = (size_t)om * ldi;
     double A0 = 0, AC = 0, AS = 0;
     T rmax = 0;
     // FP32 prefetches the next frequency's tables as well as its spectrum;
@@ -177,39 +262,42 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     constexpr bool kPrefetchTables = sizeof(T) == 4;
     T ec[DMAX];
     FreqTab<T> tc;
+    {
+      const rsrc_t ep = freq_rsrc(d.e, 0, pm, ldi);
 #pragma unroll
-    for (int q = 0; q < DMAX; ++q) ec[q] = (q < nd) ? EN[(size_t)(p0 + q) * ldi] : T(0);
-    if constexpr (kPrefetchTables) load_tab(d, j, tc);
+      for (int q = 0; q < DMAX; ++q) ec[q] = (q < nd) ? bld<T>(ep, vj, (p0 + q) * rb) : T(0);
+    }
+    if constexpr (kPrefetchTables) load_tab(d, 0, vj, tc);
     for (int o = 0; o < om; ++o) {
       // issue the next frequency's loads before computing this one
       const int on = o + 1 < om ? o + 1 : o;
       T en[DMAX];
       FreqTab<T> tn;
+      const rsrc_t epn = freq_rsrc(d.e, on, pm, ldi);
 #pragma unroll
-      for (int q = 0; q < DMAX; ++q)
-        en[q] = (q < nd) ? EN[(size_t)on * sl + (size_t)(p0 + q) * ldi] : T(0);
+      for (int q = 0; q < DMAX; ++q) en[q] = (q < nd) ? bld<T>(epn, vj, (p0 + q) * rb) : T(0);
       if constexpr (kPrefetchTables)
-        load_tab(d, (size_t)on * ldi + j, tn);
+        load_tab(d, on, vj, tn);
       else
-        load_tab(d, (size_t)o * ldi + j, tc);
+        load_tab(d, o, vj, tc);
       T dcx = 0, dcy = 0;
       if (rotation) {
-        const T* cp = d.cp0 + (size_t)o * ldi;
-        dcx = cp[ie] - cp[iw];
-        dcy = cp[in] - cp[is];
+        const rsrc_t tb = freq_rsrc(d.ftab, o, FT_N, ldi);
+        dcx = bld<T>(tb, vie, FT_CP0 * rb) - bld<T>(tb, viw, FT_CP0 * rb);
+        dcy = bld<T>(tb, vin, FT_CP0 * rb) - bld<T>(tb, vis, FT_CP0 * rb);
       }
-      const T sdt = csdt * tc.k;
-      const T sice = ice != 0.0 ? T(2 * (double)tc.cg0 * ice) : T(0);
-      const T fo = d.f[o];
-      const T ba = s.wspd + s.a * tc.logl2;
-      const T Aw = ba * cw - s.uc, Bw = ba * sw - s.vc;
+      const T sdt = s.csdt * tc.k;
+      const T sice = T(2) * tc.cg0 * s.ice;
+      const T crest = tc.sbf + sdt + tc.sdv - sice;
+      const SinFreq<T> sf = sin_freq(s, tc.logl2, tc.cp0, tc.fkovg, sin_diss1, fs2);
       const T pa0 = T(hdth * A0), pac = T(hdth * AC), pas = T(hdth * AS);
       if (valid && w == 0) {
-        PF[(size_t)o * ldi] = pa0;
-        PF[pstr + (size_t)o * ldi] = pac;
-        PF[2 * pstr + (size_t)o * ldi] = pas;
+        const rsrc_t pf = freq_rsrc(d.pfx, o, 3, ldi);
+        bst(pf, vj, pa0, 0);
+        bst(pf, vj, pac, rb);
+        bst(pf, vj, pas, 2 * rb);
       }
-      const T cf = tpsf * fo;
+      const T cf = tpsf * cld(d.f, o) * tc.coth;
       const bool prog = o <= oc;
       T m0 = 0, mc = 0, ms = 0;
 #pragma unroll
@@ -217,19 +305,16 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
         if (q < nd) {
           const int p = p0 + q;
           const T e = ec[q];
-          const T ct = d.cth[p], st = d.sth[p], c2 = d.c2th[p], s2 = d.s2th[p];
-          const T cwt = cw * ct + sw * st;
-          const T ss = ssin_ab(s, Aw, Bw, cwt, tc.cp0, ct, st, tc.fkovg, sin_fac, fs1, fs2, o);
-          const T mss = pa0 + pac * c2 + pas * s2;
-          T dm = T(1) + mssf * mss;
+          const T ct = cld(d.cth, p), st = cld(d.sth, p);
+          const T c2 = cld(d.c2th, p), s2 = cld(d.s2th, p);
+          const T ss = sin_bin(sf, s, ct, st, !prog);
+          T dm = T(1) + mssf * (pa0 + pac * c2 + pas * s2);
           dm = dm * dm;
-          const T sds = cf * dm * pw(e * tc.k4, spow) * tc.coth;
+          const T sds = cf * dm * pw(e * tc.k4, spow);
           if (prog) {
-            const T r = ss - sds * tc.sarg - tc.sbf - sdt - tc.sdv + sice;
-            const T a = fabs(r);
-            rmax = a > rmax ? a : rmax;
-            if (rotation)
-              rotmax = fmax(rotmax, (double)rot_dir(dcx, dcy, st, ct, vce, vcw, ucn, ucs, odx, ody));
+            const T r = ss - sds * tc.sarg - crest;
+            rmax = fmax(rmax, fabs(r));
+            if (rotation) rotmax = fmax(rotmax, (double)rot_dir(dcx, dcy, st, ct, dvc, duc, odx, ody));
           } else if (rotation) {
             rotmax = fmax(rotmax, 0.0);
           }
@@ -268,7 +353,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       double mx = 0;
       for (int q = 0; q < W; ++q) mx = fmax(mx, smem[q * 64 + lane]);
       pts = d.explim / mx;
-      if (valid) d.pts[j] = pts;
+      if (valid) pstore(d.pts, ldi, j, pts);
       if (!valid) pts = INFINITY;
     }
     if (!valid) rotmax = -INFINITY;
@@ -341,6 +426,27 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
 // ---------------------------------------------------------------------
 // k_integrate
 // ---------------------------------------------------------------------
+template <typename T>
+struct IntTab {
+  FreqTab<T> t;
+  T b1, b2, ok4, pa0, pac, pas;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_itab(const DomainDev<T>& d, int o, uint32_t vj,
+                                          IntTab<T>& r) {
+  const uint32_t rb = (uint32_t)d.ldi * (uint32_t)sizeof(T);
+  load_tab(d, o, vj, r.t);
+  const rsrc_t tb = freq_rsrc(d.ftab, o, FT_N, d.ldi);
+  r.b1 = bld<T>(tb, vj, FT_B1 * rb);
+  r.b2 = bld<T>(tb, vj, FT_B2 * rb);
+  r.ok4 = bld<T>(tb, vj, FT_OK4 * rb);
+  const rsrc_t pf = freq_rsrc(d.pfx, o, 3, d.ldi);
+  r.pa0 = bld<T>(pf, vj, 0);
+  r.pac = bld<T>(pf, vj, rb);
+  r.pas = bld<T>(pf, vj, 2 * rb);
+}
+
 template <typename T, int DMAX>
 __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restrict__ doms,
                                                    const BlockTask* __restrict__ tasks) {
@@ -348,119 +454,91 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   const DomainDev<T>& d = doms[tk.dom];
   const Scalars* sc = d.sc;
   if (!sc->active) return;
-  const int W = blockDim.x >> 6, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int W = blockDim.x >> 6, lane = threadIdx.x & 63, w = wave_id();
   const int j = tk.i0 + lane;
   const bool valid = j < d.im;
   const int pm = d.pm, om = d.om;
-  const size_t ldi = d.ldi, sl = (size_t)pm * ldi;
+  const size_t ldi = d.ldi;
   const int D = (pm + W - 1) / W, p0 = w * D;
   const int nd = max(0, min(D, pm - p0));
   const T dts = T(sc->dts);
+  const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
+  const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);
 
-  SinPoint<T> s;
-  T csdt;
-  double ice;
-  int oc;
-  T cw, sw;
-  load_point_fast(d, j, s, csdt, ice, oc, cw, sw);
-  const T sin_fac = T(d.sin_fac), fs1 = T(d.fieldscale1), fs2 = T(d.fieldscale2);
+  const PointState<T> s = point_stored(d, j);
+  const int oc = s.oc;
+  const T sin_diss1 = T(d.sin_fac * d.fieldscale1), fs2 = T(d.fieldscale2);
   const T tpsf = T(d.twopisds_fac), spow = T(d.sds_power), ipow = T(d.inv_sds_power);
   const T mssf = T(d.mss_fac), dsnl = dts * T(d.snl_fac);
-  const T* __restrict__ PF = d.pfx + j;
-  const T* __restrict__ EN = d.e + j;
-  const size_t pstr = (size_t)om * ldi;
   // per direction: partial E_s and E^n of the two previous frequencies
   T G1[DMAX], G2[DMAX], E1[DMAX], E2[DMAX];
 #pragma unroll
   for (int q = 0; q < DMAX; ++q) G1[q] = G2[q] = E1[q] = E2[q] = 0;
   T b1m1 = 0, b2m2 = 0, b2m1 = 0;  // bf1r(o-1), bf2r(o-2), bf2r(o-1)
-  T* __restrict__ EF = d.ef + j;
-  T* __restrict__ ES = d.es + j;
-  // FP32 prefetches the next frequency's tables as well as its spectrum;
-  // FP64 prefetches only the spectrum to stay within the register file
   constexpr bool kPrefetchTables = sizeof(T) == 4;
   T ec[DMAX];
+  {
+    const rsrc_t ep = freq_rsrc(d.e, 0, pm, ldi);
 #pragma unroll
-  for (int q = 0; q < DMAX; ++q) ec[q] = (q < nd) ? EN[(size_t)(p0 + q) * ldi] : T(0);
-  FreqTab<T> tc;
-  T bc1 = 0, bc2 = 0, okc = 0, pa0 = 0, pac = 0, pas = 0;
-  if constexpr (kPrefetchTables) {
-    load_tab(d, j, tc);
-    bc1 = d.bf1r[j];
-    bc2 = d.bf2r[j];
-    okc = d.oneoverk4[j];
-    pa0 = PF[0];
-    pac = PF[pstr];
-    pas = PF[2 * pstr];
+    for (int q = 0; q < DMAX; ++q) ec[q] = (q < nd) ? bld<T>(ep, vj, (p0 + q) * rb) : T(0);
   }
+  IntTab<T> tc;
+  if constexpr (kPrefetchTables) load_itab(d, 0, vj, tc);
   for (int o = 0; o < om; ++o) {
     const int on = o + 1 < om ? o + 1 : o;
-    const size_t tn_ = (size_t)on * ldi + j;
     T en[DMAX];
+    const rsrc_t epn = freq_rsrc(d.e, on, pm, ldi);
 #pragma unroll
-    for (int q = 0; q < DMAX; ++q)
-      en[q] = (q < nd) ? EN[(size_t)on * sl + (size_t)(p0 + q) * ldi] : T(0);
-    FreqTab<T> tn;
-    T bn1 = 0, bn2 = 0, okn = 0, qa0 = 0, qac = 0, qas = 0;
-    if constexpr (kPrefetchTables) {
-      load_tab(d, tn_, tn);
-      bn1 = d.bf1r[tn_];
-      bn2 = d.bf2r[tn_];
-      okn = d.oneoverk4[tn_];
-      qa0 = PF[(size_t)on * ldi];
-      qac = PF[pstr + (size_t)on * ldi];
-      qas = PF[2 * pstr + (size_t)on * ldi];
-    } else {
-      const size_t tc_ = (size_t)o * ldi + j;
-      load_tab(d, tc_, tc);
-      bc1 = d.bf1r[tc_];
-      bc2 = d.bf2r[tc_];
-      okc = d.oneoverk4[tc_];
-      pa0 = PF[(size_t)o * ldi];
-      pac = PF[pstr + (size_t)o * ldi];
-      pas = PF[2 * pstr + (size_t)o * ldi];
-    }
+    for (int q = 0; q < DMAX; ++q) en[q] = (q < nd) ? bld<T>(epn, vj, (p0 + q) * rb) : T(0);
+    IntTab<T> tn;
+    if constexpr (kPrefetchTables)
+      load_itab(d, on, vj, tn);
+    else
+      load_itab(d, o, vj, tc);
 
-    const T sdt = csdt * tc.k;
-    const T sice = ice != 0.0 ? T(2 * (double)tc.cg0 * ice) : T(0);
-    const T fo = d.f[o];
-    const T ba = s.wspd + s.a * tc.logl2;
-    const T Aw = ba * cw - s.uc, Bw = ba * sw - s.vc;
-    const T cf = tpsf * fo;
+    const T sdt = s.csdt * tc.t.k;
+    const T sice = T(2) * tc.t.cg0 * s.ice;
+    const T crest = tc.t.sbf + sdt + tc.t.sdv - sice;
+    const T ctail = sdt + tc.t.sdv - sice;
+    const SinFreq<T> sf = sin_freq(s, tc.t.logl2, tc.t.cp0, tc.t.fkovg, sin_diss1, fs2);
+    const T cf = tpsf * cld(d.f, o);
+    const T cfc = cf * tc.t.coth;
     const bool prog = o <= oc;
     const bool in1 = (o >= 1) && (o - 1 <= oc);  // bin o-1 is prognostic
     const bool fin = (o >= 2) && (o - 2 <= oc);  // bin o-2 completes now
+    const T db1 = dts * b1m1, db2 = dts * b2m2;
+    const int o2 = o >= 2 ? o - 2 : 0;
+    const rsrc_t rf0 = freq_rsrc(d.ef, o, pm, ldi), rs0 = freq_rsrc(d.es, o, pm, ldi);
+    const rsrc_t rf2 = freq_rsrc(d.ef, o2, pm, ldi), rs2 = freq_rsrc(d.es, o2, pm, ldi);
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) {
       if (q < nd) {
         const int p = p0 + q;
         const T e = ec[q];
-        const T ct = d.cth[p], st = d.sth[p], c2 = d.c2th[p], s2 = d.s2th[p];
-        const T cwt = cw * ct + sw * st;
-        const T ss = ssin_ab(s, Aw, Bw, cwt, tc.cp0, ct, st, tc.fkovg, sin_fac, fs1, fs2, o);
-        const T mss = pa0 + pac * c2 + pas * s2;
-        T dm = T(1) + mssf * mss;
+        const T ct = cld(d.cth, p), st = cld(d.sth, p);
+        const T c2 = cld(d.c2th, p), s2 = cld(d.s2th, p);
+        const T ss = sin_bin(sf, s, ct, st, !prog);
+        T dm = T(1) + mssf * (tc.pa0 + tc.pac * c2 + tc.pas * s2);
         dm = dm * dm;
-        const T sdsp = cf * dm * pw(e * tc.k4, spow);  // sds before cothkd
+        const T sdsp = cf * dm * pw(e * tc.t.k4, spow);  // sds before cothkd
         const T X = sdsp * e;
         T G = 0;
         if (prog) {
-          G = e * ex(dts * (ss - sdsp * tc.coth - tc.sbf - sdt - tc.sdv + sice)) - dsnl * X;
+          G = e * ex(dts * (ss - sdsp * tc.t.coth - crest)) - dsnl * X;
         } else if (valid) {
-          const T num = ss - sdt - tc.sdv + sice;
+          // diagnostic tail, src/umwm_physics.F90:70-80
+          const T num = ss - ctail;
           T es_ = 0;
-          if (num >= T(0)) es_ = okc * pw(num / (cf * dm * tc.coth), ipow);
-          const size_t b = (size_t)o * sl + (size_t)p * ldi;
-          EF[b] = es_;
-          ES[b] = T(0.5) * (e + es_);
+          if (num >= T(0)) es_ = tc.ok4 * pw(fdiv(num, cfc * dm), ipow);
+          bst(rf0, vj, es_, p * rb);
+          bst(rs0, vj, T(0.5) * (e + es_), p * rb);
         }
         T g1 = G1[q];
-        if (in1) g1 = g1 + dts * b1m1 * X;  // bf1r(o-1) X(o) into bin o-1
+        if (in1) g1 = g1 + db1 * X;  // bf1r(o-1) X(o) into bin o-1
         if (fin && valid) {
-          const T es_ = G2[q] + dts * b2m2 * X;  // bf2r(o-2) X(o) completes bin o-2
-          const size_t b = (size_t)(o - 2) * sl + (size_t)p * ldi;
-          EF[b] = es_;
-          ES[b] = T(0.5) * (E2[q] + es_);
+          const T es_ = G2[q] + db2 * X;  // bf2r(o-2) X(o) completes bin o-2
+          bst(rf2, vj, es_, p * rb);
+          bst(rs2, vj, T(0.5) * (E2[q] + es_), p * rb);
         }
         G2[q] = g1;
         G1[q] = G;
@@ -469,29 +547,19 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
       }
     }
     b2m2 = b2m1;
-    b2m1 = bc2;
-    b1m1 = bc1;
+    b2m1 = tc.b2;
+    b1m1 = tc.b1;
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) ec[q] = en[q];
-    if constexpr (kPrefetchTables) {
-      tc = tn;
-      bc1 = bn1;
-      bc2 = bn2;
-      okc = okn;
-      pa0 = qa0;
-      pac = qac;
-      pas = qas;
-    }
+    if constexpr (kPrefetchTables) tc = tn;
   }
 }
 
 // ---------------------------------------------------------------------
-// k_advect: propagation + refraction + e = ef, one thread per
-// (frequency o, sea point); workgroups of FW waves take FW consecutive
-// frequencies of 64 points.  Directions are processed in blocks of NB
-// (pm is a multiple of 4) with the next block's loads in flight while the
-// current block is computed, so each wave keeps 2*NB directions' worth of
-// gathers outstanding.
+// k_advect: propagation + refraction + e = ef, one wave per (frequency o,
+// 64 sea points); workgroups of FW waves take FW consecutive frequencies.
+// Directions are processed in blocks of NB (pm is a multiple of 4) with
+// the next block's loads in flight while the current block is computed.
 // ---------------------------------------------------------------------
 template <typename T>
 struct AdvLoads {
@@ -503,66 +571,62 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
                                                 const BlockTask* __restrict__ tasks) {
   const BlockTask tk = tasks[blockIdx.x];
   const DomainDev<T>& d = doms[tk.dom];
-  const int o = blockIdx.y * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int o = (int)blockIdx.y * (int)(blockDim.x >> 6) + wave_id();
   const Scalars* sc = d.sc;
   if (o >= d.om || !sc->active) return;
   const int j = tk.i0 + (threadIdx.x & 63);
   const bool valid = j < d.im;
   const int pm = d.pm;
-  const size_t ldi = d.ldi, sl = (size_t)pm * ldi;
+  const size_t ldi = d.ldi, ldq = d.ldq;
+  const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
+  const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);
   const T kdta = T(0.25) * T(sc->dta), dtr = T(sc->dtr);
   const bool anycur = sc->anycur != 0;
 
-  SinPoint<T> s;
-  T csdt;
-  double ice;
-  int oc;
-  T cw, sw;
-  load_point_fast(d, j, s, csdt, ice, oc, cw, sw);
-  (void)csdt;
-  (void)ice;
-  const bool prog = o <= oc, rot = o <= d.oc_rot[j];
-  const size_t t = (size_t)o * ldi + j;
-  // e, ef and es are distinct arrays; restrict lets loads move past stores
-  const T* __restrict__ ESo = d.es + (size_t)o * sl;
-  const T* __restrict__ EFo = d.ef + (size_t)o * sl + j;
-  T* __restrict__ Eo = d.e + (size_t)o * sl + j;
-  const T sin_fac = T(d.sin_fac), fs1 = T(d.fieldscale1), fs2 = T(d.fieldscale2);
-  const T* cp = d.cp0 + (size_t)o * ldi;
-  const T ba = s.wspd + s.a * d.logl2overz[t];
-  const T Aw = ba * cw - s.uc, Bw = ba * sw - s.vc;
-  const T fkovg = d.fkovg[t], cp0 = cp[j];
+  const PointState<T> s = point_stored(d, j);
+  const bool prog = o <= s.oc, rot = o <= pload(d.oc_rot, ldi, j);
+  const T sin_diss1 = T(d.sin_fac * d.fieldscale1), fs2 = T(d.fieldscale2);
+  const rsrc_t tb = freq_rsrc(d.ftab, o, FT_N, ldi);
+  const T cp0 = bld<T>(tb, vj, FT_CP0 * rb);
+  const SinFreq<T> sf = sin_freq(s, bld<T>(tb, vj, FT_LOGL2 * rb), cp0,
+                                 bld<T>(tb, vj, FT_FKOVG * rb), sin_diss1, fs2);
+  const rsrc_t pe = freq_rsrc(d.e, o, pm, ldi);    // E^{n+1} out
+  const rsrc_t pf = freq_rsrc(d.ef, o, pm, ldi);   // E_s
+  const rsrc_t ps = freq_rsrc(d.es, o, pm, ldi);   // E*
   double tx = 0, ty = 0, hs = 0;
-  auto stress_acc = [&](int p, T v) {
-    const T ct = d.cth[p], st = d.sth[p];
-    const T ss = ssin_ab(s, Aw, Bw, cw * ct + sw * st, cp0, ct, st, fkovg, sin_fac, fs1, fs2, o);
-    const double es_ = (double)v * (double)ss;
+  auto stress_acc = [&](T ct, T st, T v) {
+    const double es_ = (double)v * (double)sin_bin(sf, s, ct, st, !prog);
     tx += es_ * (double)ct;
     ty += es_ * (double)st;
     hs += (double)v;
   };
   if (!prog) {
     // above the cut-off only the diagnostic tail: e = ef
-    for (int p0 = 0; p0 < pm; p0 += NB) {
+    for (int pb = 0; pb < pm; pb += NB) {
       T v[NB];
 #pragma unroll
-      for (int q = 0; q < NB; ++q) v[q] = EFo[(size_t)(p0 + q) * ldi];
+      for (int q = 0; q < NB; ++q) v[q] = bld<T>(pf, vj, (pb + q) * rb);
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
-        if (valid) Eo[(size_t)(p0 + q) * ldi] = v[q];
-        stress_acc(p0 + q, v[q]);
+        if (valid) bst(pe, vj, v[q], (pb + q) * rb);
+        stress_acc(cld(d.cth, pb + q), cld(d.sth, pb + q), v[q]);
       }
     }
   } else {
-    const int ie = d.ie[j], iw = d.iw[j], in = d.in_[j], is = d.is_[j];
-    const T dye = d.dye[j], dyw = d.dyw[j], dxn = d.dxn[j], dxs = d.dxs[j];
-    const T oneovar = d.oneovar[j], odx = d.oneovdx[j], ody = d.oneovdy[j];
-    const T vce = d.vc[d.iie[j]], vcw = d.vc[d.iiw[j]], ucn = d.uc[d.iin[j]], ucs = d.uc[d.iis[j]];
-    const bool iced = d.fice[j] > T(d.fice_uth);
+    const int ie = pload(d.ie, ldi, j), iw = pload(d.iw, ldi, j);
+    const int in = pload(d.in_, ldi, j), is = pload(d.is_, ldi, j);
+    const uint32_t vie = (uint32_t)ie * (uint32_t)sizeof(T), viw = (uint32_t)iw * (uint32_t)sizeof(T);
+    const uint32_t vin = (uint32_t)in * (uint32_t)sizeof(T), vis = (uint32_t)is * (uint32_t)sizeof(T);
+    const T dye = pload(d.dye, ldi, j), dyw = pload(d.dyw, ldi, j);
+    const T dxn = pload(d.dxn, ldi, j), dxs = pload(d.dxs, ldi, j);
+    const T oneovar = pload(d.oneovar, ldi, j);
+    const bool iced = pload(d.fice, ldq, j) > T(d.fice_uth);
+    const int iie = pload(d.iie, ldi, j), iiw = pload(d.iiw, ldi, j);
+    const int iin = pload(d.iin, ldi, j), iis = pload(d.iis, ldi, j);
     T feup = 0, fedn = 0, fwup = 0, fwdn = 0, fnup = 0, fndn = 0, fsup = 0, fsdn = 0;
     if (anycur) {
-      const T ue = s.uc + d.uc[d.iie[j]], uw = s.uc + d.uc[d.iiw[j]];
-      const T vn = s.vc + d.vc[d.iin[j]], vs = s.vc + d.vc[d.iis[j]];
+      const T ue = s.uc + pload(d.uc, ldq, iie), uw = s.uc + pload(d.uc, ldq, iiw);
+      const T vn = s.vc + pload(d.vc, ldq, iin), vs = s.vc + pload(d.vc, ldq, iis);
       feup = (ue + fabs(ue)) * dye;
       fedn = (ue - fabs(ue)) * dye;
       fwup = (uw + fabs(uw)) * dyw;
@@ -572,41 +636,50 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
       fsup = (vs + fabs(vs)) * dxs;
       fsdn = (vs - fabs(vs)) * dxs;
     }
-    const T* cg = d.cg0 + (size_t)o * ldi;
-    const T cgi = cg[j], cge_ = cg[ie], cgw_ = cg[iw], cgs_ = cg[is], cgn_ = cg[in];
-    const T dcx = cp[ie] - cp[iw], dcy = cp[in] - cp[is];
-    const T ccu = d.ccurv[j], scu = d.scurv[j];
+    // rotation inputs (zero above the first substep's cut-off)
+    T dcx = 0, dcy = 0, dvc = 0, duc = 0, odx = 0, ody = 0;
+    if (rot) {
+      dcx = bld<T>(tb, vie, FT_CP0 * rb) - bld<T>(tb, viw, FT_CP0 * rb);
+      dcy = bld<T>(tb, vin, FT_CP0 * rb) - bld<T>(tb, vis, FT_CP0 * rb);
+      dvc = pload(d.vc, ldq, iie) - pload(d.vc, ldq, iiw);
+      duc = pload(d.uc, ldq, iin) - pload(d.uc, ldq, iis);
+      odx = pload(d.oneovdx, ldi, j);
+      ody = pload(d.oneovdy, ldi, j);
+    }
+    const uint32_t scg = FT_CG0 * rb;
+    const T cgi = bld<T>(tb, vj, scg);
+    const T cge_ = bld<T>(tb, vie, scg), cgw_ = bld<T>(tb, viw, scg);
+    const T cgs_ = bld<T>(tb, vis, scg), cgn_ = bld<T>(tb, vin, scg);
+    const T ccu = pload(d.ccurv, ldi, j), scu = pload(d.scurv, ldi, j);
     const T odth = T(d.oneovdth);
-    auto rotp = [&](int p) {
-      return rot ? rot_dir(dcx, dcy, d.sth[p], d.cth[p], vce, vcw, ucn, ucs, odx, ody) : T(0);
-    };
-    auto load_block = [&](int p0, AdvLoads<T>* L) {
+    auto rotp = [&](int p) { return rot_dir(dcx, dcy, cld(d.sth, p), cld(d.cth, p), dvc, duc, odx, ody); };
+    auto load_block = [&](int pb, AdvLoads<T>* L) {
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
-        const T* Ep = ESo + (size_t)(p0 + q) * ldi;
-        L[q].eo = Ep[j];
-        L[q].ee = Ep[ie];
-        L[q].ew = Ep[iw];
-        L[q].en = Ep[in];
-        L[q].es = Ep[is];
-        L[q].ef = EFo[(size_t)(p0 + q) * ldi];
+        const uint32_t so = (pb + q) * rb;
+        L[q].eo = bld<T>(ps, vj, so);
+        L[q].ee = bld<T>(ps, vie, so);
+        L[q].ew = bld<T>(ps, viw, so);
+        L[q].en = bld<T>(ps, vin, so);
+        L[q].es = bld<T>(ps, vis, so);
+        L[q].ef = bld<T>(pf, vj, so);
       }
     };
     AdvLoads<T> A[NB], B[NB];
-    const T efirst = ESo[j];  // E*(0) for the periodic neighbour of pm-1
-    T eprev = ESo[(size_t)(pm - 1) * ldi + j];
+    const T efirst = bld<T>(ps, vj, 0);  // E*(0), periodic neighbour of pm-1
+    T eprev = bld<T>(ps, vj, (pm - 1) * rb);
     load_block(0, A);
     T rprev = rotp(pm - 1), rcur = rotp(0);
-    for (int p0 = 0; p0 < pm; p0 += NB) {
-      const bool more = p0 + NB < pm;
-      if (more) load_block(p0 + NB, B);
+    for (int pb = 0; pb < pm; pb += NB) {
+      const bool more = pb + NB < pm;
+      if (more) load_block(pb + NB, B);
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
-        const int p = p0 + q;
+        const int p = pb + q;
         const T enext = q + 1 < NB ? A[(q + 1) < NB ? q + 1 : 0].eo : (more ? B[0].eo : efirst);
         const int pn = p + 1 < pm ? p + 1 : 0;
         const T rnext = rotp(pn);
-        const T ct = d.cth[p], st = d.sth[p];
+        const T ct = cld(d.cth, p), st = cld(d.sth, p);
         const T cc = ct * ccu - st * scu;  // cos(th + curv)
         const T sn = st * ccu + ct * scu;  // sin(th + curv)
         const T cge = cgi * cc + cge_ * cc;
@@ -622,7 +695,6 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
           flux = flux + (feup * e0 + fedn * ee - fwup * ew - fwdn * e0 + fnup * e0 + fndn * en -
                          fsup * es - fsdn * e0);
         T v = A[q].ef - kdta * flux * oneovar;
-        if (iced) v = 0;
         const T rl = T(0.5) * (rcur + rnext), rr = T(0.5) * (rcur + rprev);
         const T fr = T(0.5) *
                      ((rl + fabs(rl)) * e0 + (rl - fabs(rl)) * enext - (rr + fabs(rr)) * eprev -
@@ -630,8 +702,8 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
                      odth;
         v = v - dtr * fr;
         if (iced) v = 0;
-        if (valid) Eo[(size_t)p * ldi] = v;
-        stress_acc(p, v);
+        if (valid) bst(pe, vj, v, p * rb);
+        stress_acc(ct, st, v);
         eprev = e0;
         rprev = rcur;
         rcur = rnext;
@@ -641,12 +713,13 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
     }
   }
   if (valid) {
-    const size_t st3 = (size_t)d.om * ldi;
     const int jc = (j == 0) ? d.im : j - 1;  // cp0(o, i-1), src/umwm_stress.F90:219,316
-    const double icp = 1.0 / (double)cp[jc];
-    d.spart[t] = tx * icp;
-    d.spart[st3 + t] = ty * icp;
-    d.spart[2 * st3 + t] = hs * (double)d.kdk[t];
+    const double icp = 1.0 / (double)bld<T>(tb, (uint32_t)jc * (uint32_t)sizeof(T), FT_CP0 * rb);
+    const uint32_t vj8 = (uint32_t)j * 8u, rb8 = (uint32_t)ldi * 8u;
+    const rsrc_t sp = freq_rsrc(d.spart, o, 3, ldi);
+    bst(sp, vj8, tx * icp, 0);
+    bst(sp, vj8, ty * icp, rb8);
+    bst(sp, vj8, hs * (double)bld<T>(tb, vj, FT_KDK * rb), 2 * rb8);
   }
 }
 
@@ -661,18 +734,21 @@ __global__ __launch_bounds__(64) void k_stress(const DomainDev<T>* __restrict__ 
   if (!d.sc->active) return;
   const int j = tk.i0 + (int)threadIdx.x;
   if (j >= d.im) return;
-  const size_t ldi = d.ldi, st3 = (size_t)d.om * ldi;
+  const size_t ldi = d.ldi, ldq = d.ldq;
   const int om = d.om;
-  T w = d.wspd[j];
+  const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T), vj8 = (uint32_t)j * 8u;
+  const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T), rb8 = (uint32_t)ldi * 8u;
+  T w = pload(d.wspd, ldq, j);
   w = w > T(1e-2) ? w : T(1e-2);
-  const int oc = d.oc[j];
+  const int oc = pload(d.oc, ldi, j);
   StressIn si;
   si.sx = si.sy = si.dx = si.dy = 0;
   double hs = 0;
   for (int o = 0; o < om; ++o) {
-    const size_t t = (size_t)o * ldi + j;
-    const double kdk = d.kdk[t];
-    const double tx = d.spart[t], ty = d.spart[st3 + t];
+    const double kdk = bld<T>(freq_rsrc(d.ftab, o, FT_N, ldi), vj, FT_KDK * rb);
+    const rsrc_t sp = freq_rsrc(d.spart, o, 3, ldi);
+    const double tx = bld<double>(sp, vj8, 0);
+    const double ty = bld<double>(sp, vj8, rb8);
     si.sx += tx * kdk;
     si.sy += ty * kdk;
     if (o >= oc) {
@@ -683,19 +759,19 @@ __global__ __launch_bounds__(64) void k_stress(const DomainDev<T>* __restrict__ 
       si.tomx = tx;
       si.tomy = ty;
     }
-    hs += d.spart[2 * st3 + t];
+    hs += bld<double>(sp, vj8, 2 * rb8);
   }
   si.wspd = w;
-  si.wdir = d.wdir[j];
-  si.uc = d.uc[j];
-  si.vc = d.vc[j];
-  si.us1 = d.us1[j];
-  si.vs1 = d.vs1[j];
-  si.rhoa = d.rhoa[j];
-  si.rhow = d.rhow[j];
-  si.kom = d.k[(size_t)(om - 1) * ldi + j];
+  si.wdir = pload(d.wdir, ldq, j);
+  si.uc = pload(d.uc, ldq, j);
+  si.vc = pload(d.vc, ldq, j);
+  si.us1 = pload(d.us1, ldi, j);
+  si.vs1 = pload(d.vs1, ldi, j);
+  si.rhoa = pload(d.rhoa, ldq, j);
+  si.rhow = pload(d.rhow, ldq, j);
+  si.kom = bld<T>(freq_rsrc(d.ftab, om - 1, FT_N, ldi), vj, FT_K * rb);
   store_stress(d, j, stress_atm_point(si, d.dthg, d.z, d.nu_air, d.kappa));
-  d.hs_ice[j] = 4 * sqrt(hs * d.dth);
+  pstore(d.hs_ice, ldi, j, 4 * sqrt(hs * d.dth));
 }
 
 template <typename T, int D1, int D2>
