@@ -69,6 +69,54 @@ __device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, int32_t v, uint32_t
   __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, r, (int)voff, (int)soff, 0);
 }
 
+// 16-byte accesses.  The builtins' unsigned vectors are always bit-cast as
+// a whole to a typed vector: extracting single elements of the builtin's
+// own return value does not reliably keep the wide load.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = f32x4;
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  using type = f64x2;
+  static constexpr int n = 2;
+};
+
+template <typename T>
+__device__ __forceinline__ typename Vec16<T>::type bld16(rsrc_t r, uint32_t voff,
+                                                         uint32_t soff = 0) {
+  return __builtin_bit_cast(typename Vec16<T>::type,
+                            __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+
+template <typename V>
+__device__ __forceinline__ void bst16(rsrc_t r, uint32_t voff, V v, uint32_t soff = 0) {
+  static_assert(sizeof(V) == 16, "16-byte store");
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)voff, (int)soff,
+                                         0);
+}
+
+// N consecutive values of T starting at byte offset voff (N * sizeof(T) a
+// multiple of 16).
+template <typename T, int N>
+__device__ __forceinline__ void bldn(rsrc_t r, uint32_t voff, T (&out)[N]) {
+  constexpr int k = Vec16<T>::n;
+  static_assert(N % k == 0, "whole 16-byte chunks");
+#pragma unroll
+  for (int c = 0; c < N / k; ++c) {
+    const auto v = bld16<T>(r, voff + 16u * (uint32_t)c);
+#pragma unroll
+    for (int i = 0; i < k; ++i) out[c * k + i] = v[i];
+  }
+}
+
 // Scalar (constant address space) load of a wave-uniform element.
 template <typename T>
 __device__ __forceinline__ T cld(const T* p, int idx) {

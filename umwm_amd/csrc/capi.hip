@@ -221,7 +221,7 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   // the fused kernels address one frequency of a spectrum (pm rows), of the
   // table block (FT_N rows) or of a per-point array with one 32-bit buffer
   // descriptor each
-  const size_t maxrows = std::max<size_t>({(size_t)pm, (size_t)FT_N, 3});
+  const size_t maxrows = std::max<size_t>({(size_t)pm, (size_t)FT_N, 4});
   if (maxrows * ldi * std::max(sizeof(T), sizeof(double)) >= (size_t(1) << 32) ||
       (size_t)ldq * sizeof(double) >= (size_t(1) << 32)) {
     delete D;
@@ -272,18 +272,14 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   h.snl_arg = sarg; h.bf1r = b1; h.bf2r = b2; h.dwn = dwn; h.cth_curv = ccv; h.sth_curv = scv;
   h.f = f; h.th = th; h.cth = ct; h.sth = st; h.c2th = c2; h.s2th = s2; h.cth2pp = c2pp;
   h.pl = pl; h.pr = pr;
-  // interleaved copy of the tables for the fused kernels, [o][FT_N][ldi]
+  // per-(frequency, point) table records for the fused kernels, [o][ldi][FT_N]
   {
-    const std::vector<double>* ft[FT_N] = {&S.cp0, &S.logl2overz, &S.fkovg, &S.k4, &S.cothkd,
-                                           &S.sbf, &S.sdv, &S.snl_arg, &S.k, &S.cg0, &S.k3dk,
-                                           &S.bf1r, &S.bf2r, &S.oneoverk4, &S.kdk};
+    const std::vector<double>* ft[FT_PAD] = {&S.cp0, &S.logl2overz, &S.fkovg, &S.k4, &S.cothkd,
+                                             &S.sbf, &S.sdv, &S.snl_arg, &S.k, &S.cg0, &S.k3dk,
+                                             &S.kdk, &S.bf1r, &S.bf2r, &S.oneoverk4};
     std::vector<double> blk((size_t)FT_N * nt, 0.0);
-    for (int o = 0; o < om; ++o)
-      for (int t = 0; t < FT_N; ++t) {
-        const double* src = ft[t]->data() + (size_t)o * ldi;
-        double* dst = blk.data() + ((size_t)o * FT_N + t) * ldi;
-        std::copy(src, src + ldi, dst);
-      }
+    for (size_t q = 0; q < nt; ++q)
+      for (int t = 0; t < FT_PAD; ++t) blk[q * FT_N + t] = (*ft[t])[q];
     h.ftab = A.template alloc<T>((size_t)FT_N * nt);
     if (!h.ftab) {
       delete D;
@@ -377,14 +373,14 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   h.block0 = 0;
   h.part_min = A.template alloc<double>(std::max(h.nblocks, 1));
   h.part_max = A.template alloc<double>(std::max(h.nblocks, 1));
-  h.pfx = A.template alloc<T>(3 * nt);
-  h.spart = A.template alloc<double>(3 * nt);
+  h.pfx = A.template alloc<T>(4 * nt);
+  h.spart = A.template alloc<double>(4 * nt);
   h.sc = A.template alloc<Scalars>(1);
   if (!h.sc || !h.pfx || !h.spart) {
     delete D;
     return fail(UMWM_ERR_NOMEM, "device allocation failed");
   }
-  if ((rc = dfill<T>(h.pfx, 3 * nt, T(0), s)) || (rc = dfill<double>(h.spart, 3 * nt, 0.0, s))) {
+  if ((rc = dfill<T>(h.pfx, 4 * nt, T(0), s)) || (rc = dfill<double>(h.spart, 4 * nt, 0.0, s))) {
     delete D;
     return rc;
   }

@@ -85,11 +85,12 @@ struct DomainDev {
   double *ustar, *cd, *taux, *tauy, *taux_form, *tauy_form, *taux_skin;
   double *tauy_skin, *taux_diag, *tauy_diag, *tailatmx, *tailatmy;
   double *us1, *vs1;  // surface Stokes drift used by stress('atm')
-  // fused-path copy of the (o, j) tables interleaved by frequency,
-  // [om][FT_N][ldi], so one buffer descriptor per frequency reaches all of
-  // them; the exclusive frequency prefix of the cos^2 directional
-  // convolution times dth/2, [om][3][ldi] (written by k_rates); the
-  // per-frequency stress / wave-height partial sums [om][3][ldi]
+  // fused-path copy of the (o, j) tables, [om][ldi][FT_N]: the FT_N values
+  // of one (frequency, point) are contiguous, so a lane fetches them with a
+  // few 16-byte loads and a wave reads one contiguous span; the exclusive
+  // frequency prefix of the cos^2 directional convolution times dth/2,
+  // [om][ldi][4] (3 used, written by k_rates); the per-frequency stress /
+  // wave-height partial sums, [om][ldi][4] (3 used)
   T* ftab;
   T* pfx;
   double* spart;
@@ -98,11 +99,13 @@ struct DomainDev {
   Scalars* sc;
 };
 
-// Rows of the interleaved table block DomainDev::ftab.
+// Slots of one (frequency, point) record of DomainDev::ftab.  The first
+// twelve are what the rates kernel reads.
 enum FusedTable {
   FT_CP0, FT_LOGL2, FT_FKOVG, FT_K4, FT_COTH, FT_SBF, FT_SDV, FT_SARG, FT_K, FT_CG0, FT_K3DK,
-  FT_B1, FT_B2, FT_OK4, FT_KDK, FT_N
+  FT_KDK, FT_B1, FT_B2, FT_OK4, FT_PAD, FT_N
 };
+constexpr int kRatesSlots = 12;
 
 // One workgroup of a fused launch: which domain, which 64 sea points.
 struct BlockTask {

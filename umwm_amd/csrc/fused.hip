@@ -42,7 +42,10 @@
 // equal to it up to rounding.
 //
 // All per-point streams go through raw buffer loads and stores (bufio.hpp):
-// scalar row descriptors, 32-bit lane offsets computed once per thread.
+// one descriptor per frequency in scalar registers, 32-bit lane offsets
+// computed once per thread, directions reached with a scalar offset.  The
+// (frequency, point) tables are one 16-value record per lane (a few 16-byte
+// loads); the prefix moments and stress partial sums are 4-value records.
 #include <hip/hip_runtime.h>
 
 #include "bufio.hpp"
@@ -169,8 +172,12 @@ __device__ __forceinline__ T sin_bin(const SinFreq<T>& q, const PointState<T>& s
   const T x = q.A * ct + q.B * st - q.cp0;
   const T y = fabs(x) * x;
   const T cwt = s.cw * ct + s.sw * st;  // cos(wdir - th)
-  T r = y * (y > T(0) ? q.pos : (cwt > T(0) ? q.neg2 : q.neg1));
-  if (tail && r < T(0)) r = T(0);  // src/umwm_source_functions.F90:73-80
+  // select between values (a select between members would become an
+  // address select and spill the struct to scratch)
+  const T pos = q.pos, neg1 = q.neg1, neg2 = q.neg2;
+  const T neg = cwt > T(0) ? neg2 : neg1;
+  T r = y * (y > T(0) ? pos : neg);
+  if (tail) r = r < T(0) ? T(0) : r;  // src/umwm_source_functions.F90:73-80
   return r;
 }
 
@@ -181,36 +188,32 @@ __device__ __forceinline__ T rot_dir(T dcx, T dcy, T sth, T cth, T dvc, T duc, T
   return T(0.5) * ((dcx * sth + dvc) * odx - (dcy * cth + duc) * ody);
 }
 
-// Per-frequency, per-point tables of the source terms.
+// Per-frequency, per-point tables of the source terms: one record of
+// DomainDev::ftab.
 template <typename T>
 struct FreqTab {
-  T cp0, logl2, fkovg, k4, coth, sbf, sdv, sarg, k, cg0, k3dk;
+  T v[FT_N];
+  __device__ __forceinline__ T operator[](int i) const { return v[i]; }
 };
 
-// Descriptor of frequency o's slice of a [om][n_rows][ldi] block (a
-// spectrum plane with n_rows = pm, the table block with FT_N, the prefix
-// block with 3); rows are then reached with the scalar offset row * ldi.
+// Descriptor of frequency o's slice of a [om][rows][ldi] spectrum (rows =
+// pm; a direction is then the scalar offset p * ldi * sizeof(T)) or of a
+// [om][ldi][rec] record block (rec values per point).
 template <typename T>
-__device__ __forceinline__ rsrc_t freq_rsrc(const T* base, int o, int n_rows, size_t ldi) {
-  return row_rsrc(base, (size_t)o, (size_t)n_rows * ldi);
+__device__ __forceinline__ rsrc_t freq_rsrc(const T* base, int o, int rows, size_t ldi) {
+  return row_rsrc(base, (size_t)o, (size_t)rows * ldi);
 }
 
-template <typename T>
-__device__ __forceinline__ void load_tab(const DomainDev<T>& d, int o, uint32_t vj,
+// First N slots of the record of (o, j); vr = j * FT_N * sizeof(T).
+template <int N, typename T>
+__device__ __forceinline__ void load_tab(const DomainDev<T>& d, int o, uint32_t vr,
                                          FreqTab<T>& r) {
-  const uint32_t rb = (uint32_t)d.ldi * (uint32_t)sizeof(T);
-  const rsrc_t tb = freq_rsrc(d.ftab, o, FT_N, d.ldi);
-  r.cp0 = bld<T>(tb, vj, FT_CP0 * rb);
-  r.logl2 = bld<T>(tb, vj, FT_LOGL2 * rb);
-  r.fkovg = bld<T>(tb, vj, FT_FKOVG * rb);
-  r.k4 = bld<T>(tb, vj, FT_K4 * rb);
-  r.coth = bld<T>(tb, vj, FT_COTH * rb);
-  r.sbf = bld<T>(tb, vj, FT_SBF * rb);
-  r.sdv = bld<T>(tb, vj, FT_SDV * rb);
-  r.sarg = bld<T>(tb, vj, FT_SARG * rb);
-  r.k = bld<T>(tb, vj, FT_K * rb);
-  r.cg0 = bld<T>(tb, vj, FT_CG0 * rb);
-  r.k3dk = bld<T>(tb, vj, FT_K3DK * rb);
+  T tmp[N];
+  bldn<T, N>(freq_rsrc(d.ftab, o, FT_N, d.ldi), vr, tmp);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.v[i] = tmp[i];
+#pragma unroll
+  for (int i = N; i < FT_N; ++i) r.v[i] = T(0);
 }
 
 // ---------------------------------------------------------------------
@@ -233,7 +236,8 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
   const int D = (pm + W - 1) / W, p0 = w * D;
   const int nd = max(0, min(D, pm - p0));  // directions of this wave (uniform)
   const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
-  const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);  // bytes of one row
+  const uint32_t vr = (uint32_t)j * (uint32_t)(FT_N * sizeof(T));
+  const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);  // bytes of one direction row
   double pts = INFINITY, rotmax = -INFINITY;
 
   if (active) {
@@ -257,9 +261,6 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     const double hdth = 0.5 * d.dth;
     double A0 = 0, AC = 0, AS = 0;
     T rmax = 0;
-    // FP32 prefetches the next frequency's tables as well as its spectrum;
-    // FP64 prefetches only the spectrum to stay within the register file
-    constexpr bool kPrefetchTables = sizeof(T) == 4;
     T ec[DMAX];
     FreqTab<T> tc;
     {
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) ec[q] = (q < nd) ? bld<T>(ep, vj, (p0 + q) * rb) : T(0);
     }
-    if constexpr (kPrefetchTables) load_tab(d, 0, vj, tc);
+    load_tab<kRatesSlots>(d, 0, vr, tc);
     for (int o = 0; o < om; ++o) {
       // issue the next frequency's loads before computing this one
       const int on = o + 1 < om ? o + 1 : o;
@@ -276,30 +277,27 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       const rsrc_t epn = freq_rsrc(d.e, on, pm, ldi);
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) en[q] = (q < nd) ? bld<T>(epn, vj, (p0 + q) * rb) : T(0);
-      if constexpr (kPrefetchTables)
-        load_tab(d, on, vj, tn);
-      else
-        load_tab(d, o, vj, tc);
-      T dcx = 0, dcy = 0;
-      if (rotation) {
-        const rsrc_t tb = freq_rsrc(d.ftab, o, FT_N, ldi);
-        dcx = bld<T>(tb, vie, FT_CP0 * rb) - bld<T>(tb, viw, FT_CP0 * rb);
-        dcy = bld<T>(tb, vin, FT_CP0 * rb) - bld<T>(tb, vis, FT_CP0 * rb);
-      }
-      const T sdt = s.csdt * tc.k;
-      const T sice = T(2) * tc.cg0 * s.ice;
-      const T crest = tc.sbf + sdt + tc.sdv - sice;
-      const SinFreq<T> sf = sin_freq(s, tc.logl2, tc.cp0, tc.fkovg, sin_diss1, fs2);
+      load_tab<kRatesSlots>(d, on, vr, tn);
+
+      const T sdt = s.csdt * tc[FT_K];
+      const T sice = T(2) * tc[FT_CG0] * s.ice;
+      const T crest = tc[FT_SBF] + sdt + tc[FT_SDV] - sice;
+      const SinFreq<T> sf = sin_freq(s, tc[FT_LOGL2], tc[FT_CP0], tc[FT_FKOVG], sin_diss1, fs2);
       const T pa0 = T(hdth * A0), pac = T(hdth * AC), pas = T(hdth * AS);
       if (valid && w == 0) {
-        const rsrc_t pf = freq_rsrc(d.pfx, o, 3, ldi);
-        bst(pf, vj, pa0, 0);
-        bst(pf, vj, pac, rb);
-        bst(pf, vj, pas, 2 * rb);
+        typename Vec16<T>::type v;
+        if constexpr (sizeof(T) == 4) {
+          v = typename Vec16<T>::type{pa0, pac, pas, T(0)};
+          bst16(freq_rsrc(d.pfx, o, 4, ldi), (uint32_t)j * 16u, v);
+        } else {
+          const rsrc_t pf = freq_rsrc(d.pfx, o, 4, ldi);
+          bst16(pf, (uint32_t)j * 32u, typename Vec16<T>::type{pa0, pac});
+          bst16(pf, (uint32_t)j * 32u + 16u, typename Vec16<T>::type{pas, T(0)});
+        }
       }
-      const T cf = tpsf * cld(d.f, o) * tc.coth;
-      const bool prog = o <= oc;
-      T m0 = 0, mc = 0, ms = 0;
+      const T cf = tpsf * cld(d.f, o) * tc[FT_COTH];
+      const T k4 = tc[FT_K4], sarg = tc[FT_SARG];
+      T m0 = 0, mc = 0, ms = 0, rq = 0;
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) {
         if (q < nd) {
@@ -307,21 +305,33 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
           const T e = ec[q];
           const T ct = cld(d.cth, p), st = cld(d.sth, p);
           const T c2 = cld(d.c2th, p), s2 = cld(d.s2th, p);
-          const T ss = sin_bin(sf, s, ct, st, !prog);
+          const T ss = sin_bin(sf, s, ct, st, false);
           T dm = T(1) + mssf * (pa0 + pac * c2 + pas * s2);
           dm = dm * dm;
-          const T sds = cf * dm * pw(e * tc.k4, spow);
-          if (prog) {
-            const T r = ss - sds * tc.sarg - crest;
-            rmax = fmax(rmax, fabs(r));
-            if (rotation) rotmax = fmax(rotmax, (double)rot_dir(dcx, dcy, st, ct, dvc, duc, odx, ody));
-          } else if (rotation) {
-            rotmax = fmax(rotmax, 0.0);
-          }
+          const T sds = cf * dm * pw(e * k4, spow);
+          const T r = ss - sds * sarg - crest;
+          rq = fmax(rq, fabs(r));
           m0 += e;
           mc += e * c2;
           ms += e * s2;
         }
+      }
+      if (o <= oc) rmax = fmax(rmax, rq);  // rates of prognostic bins only
+      if (rotation) {
+        // rotation tendency of the first substep (src/umwm_advection.F90:329-349)
+        const rsrc_t cpr = row_rsrc(d.cp0, o, ldi);
+        const T dcx = bld<T>(cpr, vie) - bld<T>(cpr, viw);
+        const T dcy = bld<T>(cpr, vin) - bld<T>(cpr, vis);
+        double rm = 0;  // bins above oc hold 0 and join the signed max
+        if (o <= oc) {
+          rm = -INFINITY;
+          for (int q = 0; q < nd; ++q) {
+            const int p = p0 + q;
+            rm = fmax(rm, (double)rot_dir(dcx, dcy, cld(d.sth, p), cld(d.cth, p), dvc, duc, odx,
+                                          ody));
+          }
+        }
+        if (nd > 0) rotmax = fmax(rotmax, rm);
       }
       double s0 = m0, s2c = mc, s2s = ms;
       if (W > 1) {
@@ -337,13 +347,13 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
           s2s += b[(2 * W + q) * 64 + lane];
         }
       }
-      const double k3 = tc.k3dk;
+      const double k3 = tc[FT_K3DK];
       A0 += k3 * s0;
       AC += k3 * s2c;
       AS += k3 * s2s;
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) ec[q] = en[q];
-      if constexpr (kPrefetchTables) tc = tn;
+      tc = tn;
     }
     // max |rate| of the point over all waves
     __syncthreads();
@@ -427,24 +437,26 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
 // k_integrate
 // ---------------------------------------------------------------------
 template <typename T>
-struct IntTab {
-  FreqTab<T> t;
-  T b1, b2, ok4, pa0, pac, pas;
+struct Prefix {
+  T a0, ac, as;
 };
 
 template <typename T>
-__device__ __forceinline__ void load_itab(const DomainDev<T>& d, int o, uint32_t vj,
-                                          IntTab<T>& r) {
-  const uint32_t rb = (uint32_t)d.ldi * (uint32_t)sizeof(T);
-  load_tab(d, o, vj, r.t);
-  const rsrc_t tb = freq_rsrc(d.ftab, o, FT_N, d.ldi);
-  r.b1 = bld<T>(tb, vj, FT_B1 * rb);
-  r.b2 = bld<T>(tb, vj, FT_B2 * rb);
-  r.ok4 = bld<T>(tb, vj, FT_OK4 * rb);
-  const rsrc_t pf = freq_rsrc(d.pfx, o, 3, d.ldi);
-  r.pa0 = bld<T>(pf, vj, 0);
-  r.pac = bld<T>(pf, vj, rb);
-  r.pas = bld<T>(pf, vj, 2 * rb);
+__device__ __forceinline__ Prefix<T> load_prefix(const DomainDev<T>& d, int o, int j) {
+  const rsrc_t pf = freq_rsrc(d.pfx, o, 4, d.ldi);
+  Prefix<T> r;
+  if constexpr (sizeof(T) == 4) {
+    const auto v = bld16<T>(pf, (uint32_t)j * 16u);
+    r.a0 = v[0];
+    r.ac = v[1];
+    r.as = v[2];
+  } else {
+    const auto v = bld16<T>(pf, (uint32_t)j * 32u);
+    r.a0 = v[0];
+    r.ac = v[1];
+    r.as = bld<T>(pf, (uint32_t)j * 32u + 16u);
+  }
+  return r;
 }
 
 template <typename T, int DMAX>
@@ -463,7 +475,9 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   const int nd = max(0, min(D, pm - p0));
   const T dts = T(sc->dts);
   const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
+  const uint32_t vr = (uint32_t)j * (uint32_t)(FT_N * sizeof(T));
   const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);
+  (void)lane;
 
   const PointState<T> s = point_stored(d, j);
   const int oc = s.oc;
@@ -475,83 +489,89 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
 #pragma unroll
   for (int q = 0; q < DMAX; ++q) G1[q] = G2[q] = E1[q] = E2[q] = 0;
   T b1m1 = 0, b2m2 = 0, b2m1 = 0;  // bf1r(o-1), bf2r(o-2), bf2r(o-1)
-  constexpr bool kPrefetchTables = sizeof(T) == 4;
   T ec[DMAX];
   {
     const rsrc_t ep = freq_rsrc(d.e, 0, pm, ldi);
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) ec[q] = (q < nd) ? bld<T>(ep, vj, (p0 + q) * rb) : T(0);
   }
-  IntTab<T> tc;
-  if constexpr (kPrefetchTables) load_itab(d, 0, vj, tc);
+  FreqTab<T> tc;
+  load_tab<FT_N>(d, 0, vr, tc);
+  Prefix<T> pc = load_prefix(d, 0, j);
   for (int o = 0; o < om; ++o) {
     const int on = o + 1 < om ? o + 1 : o;
     T en[DMAX];
     const rsrc_t epn = freq_rsrc(d.e, on, pm, ldi);
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) en[q] = (q < nd) ? bld<T>(epn, vj, (p0 + q) * rb) : T(0);
-    IntTab<T> tn;
-    if constexpr (kPrefetchTables)
-      load_itab(d, on, vj, tn);
-    else
-      load_itab(d, o, vj, tc);
+    FreqTab<T> tn;
+    load_tab<FT_N>(d, on, vr, tn);
+    const Prefix<T> pn = load_prefix(d, on, j);
 
-    const T sdt = s.csdt * tc.t.k;
-    const T sice = T(2) * tc.t.cg0 * s.ice;
-    const T crest = tc.t.sbf + sdt + tc.t.sdv - sice;
-    const T ctail = sdt + tc.t.sdv - sice;
-    const SinFreq<T> sf = sin_freq(s, tc.t.logl2, tc.t.cp0, tc.t.fkovg, sin_diss1, fs2);
+    const T sdt = s.csdt * tc[FT_K];
+    const T sice = T(2) * tc[FT_CG0] * s.ice;
+    const T crest = tc[FT_SBF] + sdt + tc[FT_SDV] - sice;
+    const T ctail = sdt + tc[FT_SDV] - sice;
+    const SinFreq<T> sf = sin_freq(s, tc[FT_LOGL2], tc[FT_CP0], tc[FT_FKOVG], sin_diss1, fs2);
     const T cf = tpsf * cld(d.f, o);
-    const T cfc = cf * tc.t.coth;
+    const T coth = tc[FT_COTH], k4 = tc[FT_K4], ok4 = tc[FT_OK4];
+    const T cfc = cf * coth;
     const bool prog = o <= oc;
     const bool in1 = (o >= 1) && (o - 1 <= oc);  // bin o-1 is prognostic
-    const bool fin = (o >= 2) && (o - 2 <= oc);  // bin o-2 completes now
+    const bool fin = valid && (o >= 2) && (o - 2 <= oc);  // bin o-2 completes now
     const T db1 = dts * b1m1, db2 = dts * b2m2;
     const int o2 = o >= 2 ? o - 2 : 0;
     const rsrc_t rf0 = freq_rsrc(d.ef, o, pm, ldi), rs0 = freq_rsrc(d.es, o, pm, ldi);
     const rsrc_t rf2 = freq_rsrc(d.ef, o2, pm, ldi), rs2 = freq_rsrc(d.es, o2, pm, ldi);
-#pragma unroll
-    for (int q = 0; q < DMAX; ++q) {
-      if (q < nd) {
-        const int p = p0 + q;
-        const T e = ec[q];
-        const T ct = cld(d.cth, p), st = cld(d.sth, p);
-        const T c2 = cld(d.c2th, p), s2 = cld(d.s2th, p);
-        const T ss = sin_bin(sf, s, ct, st, !prog);
-        T dm = T(1) + mssf * (tc.pa0 + tc.pac * c2 + tc.pas * s2);
-        dm = dm * dm;
-        const T sdsp = cf * dm * pw(e * tc.t.k4, spow);  // sds before cothkd
-        const T X = sdsp * e;
-        T G = 0;
-        if (prog) {
-          G = e * ex(dts * (ss - sdsp * tc.t.coth - crest)) - dsnl * X;
-        } else if (valid) {
-          // diagnostic tail, src/umwm_physics.F90:70-80
-          const T num = ss - ctail;
-          T es_ = 0;
-          if (num >= T(0)) es_ = tc.ok4 * pw(fdiv(num, cfc * dm), ipow);
-          bst(rf0, vj, es_, p * rb);
-          bst(rs0, vj, T(0.5) * (e + es_), p * rb);
-        }
-        T g1 = G1[q];
-        if (in1) g1 = g1 + db1 * X;  // bf1r(o-1) X(o) into bin o-1
-        if (fin && valid) {
-          const T es_ = G2[q] + db2 * X;  // bf2r(o-2) X(o) completes bin o-2
-          bst(rf2, vj, es_, p * rb);
-          bst(rs2, vj, T(0.5) * (E2[q] + es_), p * rb);
-        }
-        G2[q] = g1;
-        G1[q] = G;
-        E2[q] = E1[q];
-        E1[q] = e;
+    // one bin: X = sds e (before cothkd) feeds snl of bins o-1 and o-2
+    auto bin = [&](int q, bool progbin) {
+      const int p = p0 + q;
+      const T e = ec[q];
+      const T ct = cld(d.cth, p), st = cld(d.sth, p);
+      const T c2 = cld(d.c2th, p), s2 = cld(d.s2th, p);
+      const T ss = sin_bin(sf, s, ct, st, !progbin);
+      T dm = T(1) + mssf * (pc.a0 + pc.ac * c2 + pc.as * s2);
+      dm = dm * dm;
+      const T sdsp = cf * dm * pw(e * k4, spow);
+      const T X = sdsp * e;
+      T G = 0;
+      if (progbin) {
+        G = e * ex(dts * (ss - sdsp * coth - crest)) - dsnl * X;
+      } else if (valid) {
+        // diagnostic tail, src/umwm_physics.F90:70-80
+        const T num = ss - ctail;
+        const T es_ = num >= T(0) ? ok4 * pw(fdiv(num, cfc * dm), ipow) : T(0);
+        bst(rf0, vj, es_, p * rb);
+        bst(rs0, vj, T(0.5) * (e + es_), p * rb);
       }
+      T g1 = G1[q];
+      if (in1) g1 = g1 + db1 * X;  // bf1r(o-1) X(o) into bin o-1
+      if (fin) {
+        const T es_ = G2[q] + db2 * X;  // bf2r(o-2) X(o) completes bin o-2
+        bst(rf2, vj, es_, p * rb);
+        bst(rs2, vj, T(0.5) * (E2[q] + es_), p * rb);
+      }
+      G2[q] = g1;
+      G1[q] = G;
+      E2[q] = E1[q];
+      E1[q] = e;
+    };
+    if (prog) {
+#pragma unroll
+      for (int q = 0; q < DMAX; ++q)
+        if (q < nd) bin(q, true);
+    } else {
+#pragma unroll
+      for (int q = 0; q < DMAX; ++q)
+        if (q < nd) bin(q, false);
     }
     b2m2 = b2m1;
-    b2m1 = tc.b2;
-    b1m1 = tc.b1;
+    b2m1 = tc[FT_B2];
+    b1m1 = tc[FT_B1];
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) ec[q] = en[q];
-    if constexpr (kPrefetchTables) tc = tn;
+    tc = tn;
+    pc = pn;
   }
 }
 
@@ -580,16 +600,19 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
   const size_t ldi = d.ldi, ldq = d.ldq;
   const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
   const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);
+  constexpr uint32_t E = sizeof(T);
   const T kdta = T(0.25) * T(sc->dta), dtr = T(sc->dtr);
   const bool anycur = sc->anycur != 0;
 
   const PointState<T> s = point_stored(d, j);
   const bool prog = o <= s.oc, rot = o <= pload(d.oc_rot, ldi, j);
   const T sin_diss1 = T(d.sin_fac * d.fieldscale1), fs2 = T(d.fieldscale2);
-  const rsrc_t tb = freq_rsrc(d.ftab, o, FT_N, ldi);
-  const T cp0 = bld<T>(tb, vj, FT_CP0 * rb);
-  const SinFreq<T> sf = sin_freq(s, bld<T>(tb, vj, FT_LOGL2 * rb), cp0,
-                                 bld<T>(tb, vj, FT_FKOVG * rb), sin_diss1, fs2);
+  // neighbour gathers read the frequency-major tables (one plane per
+  // table), where 64 neighbours share a few cache lines
+  const rsrc_t cpr = row_rsrc(d.cp0, o, ldi), cgr = row_rsrc(d.cg0, o, ldi);
+  const T cp0 = bld<T>(cpr, vj);
+  const SinFreq<T> sf = sin_freq(s, bld<T>(row_rsrc(d.logl2overz, o, ldi), vj), cp0,
+                                 bld<T>(row_rsrc(d.fkovg, o, ldi), vj), sin_diss1, fs2);
   const rsrc_t pe = freq_rsrc(d.e, o, pm, ldi);    // E^{n+1} out
   const rsrc_t pf = freq_rsrc(d.ef, o, pm, ldi);   // E_s
   const rsrc_t ps = freq_rsrc(d.es, o, pm, ldi);   // E*
@@ -615,16 +638,16 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
   } else {
     const int ie = pload(d.ie, ldi, j), iw = pload(d.iw, ldi, j);
     const int in = pload(d.in_, ldi, j), is = pload(d.is_, ldi, j);
-    const uint32_t vie = (uint32_t)ie * (uint32_t)sizeof(T), viw = (uint32_t)iw * (uint32_t)sizeof(T);
-    const uint32_t vin = (uint32_t)in * (uint32_t)sizeof(T), vis = (uint32_t)is * (uint32_t)sizeof(T);
+    const uint32_t vie = (uint32_t)ie * E, viw = (uint32_t)iw * E;
+    const uint32_t vin = (uint32_t)in * E, vis = (uint32_t)is * E;
     const T dye = pload(d.dye, ldi, j), dyw = pload(d.dyw, ldi, j);
     const T dxn = pload(d.dxn, ldi, j), dxs = pload(d.dxs, ldi, j);
     const T oneovar = pload(d.oneovar, ldi, j);
     const bool iced = pload(d.fice, ldq, j) > T(d.fice_uth);
-    const int iie = pload(d.iie, ldi, j), iiw = pload(d.iiw, ldi, j);
-    const int iin = pload(d.iin, ldi, j), iis = pload(d.iis, ldi, j);
     T feup = 0, fedn = 0, fwup = 0, fwdn = 0, fnup = 0, fndn = 0, fsup = 0, fsdn = 0;
     if (anycur) {
+      const int iie = pload(d.iie, ldi, j), iiw = pload(d.iiw, ldi, j);
+      const int iin = pload(d.iin, ldi, j), iis = pload(d.iis, ldi, j);
       const T ue = s.uc + pload(d.uc, ldq, iie), uw = s.uc + pload(d.uc, ldq, iiw);
       const T vn = s.vc + pload(d.vc, ldq, iin), vs = s.vc + pload(d.vc, ldq, iis);
       feup = (ue + fabs(ue)) * dye;
@@ -639,17 +662,18 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
     // rotation inputs (zero above the first substep's cut-off)
     T dcx = 0, dcy = 0, dvc = 0, duc = 0, odx = 0, ody = 0;
     if (rot) {
-      dcx = bld<T>(tb, vie, FT_CP0 * rb) - bld<T>(tb, viw, FT_CP0 * rb);
-      dcy = bld<T>(tb, vin, FT_CP0 * rb) - bld<T>(tb, vis, FT_CP0 * rb);
-      dvc = pload(d.vc, ldq, iie) - pload(d.vc, ldq, iiw);
-      duc = pload(d.uc, ldq, iin) - pload(d.uc, ldq, iis);
+      dcx = bld<T>(cpr, vie) - bld<T>(cpr, viw);
+      dcy = bld<T>(cpr, vin) - bld<T>(cpr, vis);
+      dvc = pload(d.vc, ldq, pload(d.iie, ldi, j)) - pload(d.vc, ldq, pload(d.iiw, ldi, j));
+      duc = pload(d.uc, ldq, pload(d.iin, ldi, j)) - pload(d.uc, ldq, pload(d.iis, ldi, j));
       odx = pload(d.oneovdx, ldi, j);
       ody = pload(d.oneovdy, ldi, j);
     }
-    const uint32_t scg = FT_CG0 * rb;
-    const T cgi = bld<T>(tb, vj, scg);
-    const T cge_ = bld<T>(tb, vie, scg), cgw_ = bld<T>(tb, viw, scg);
-    const T cgs_ = bld<T>(tb, vis, scg), cgn_ = bld<T>(tb, vin, scg);
+    const T cgi = bld<T>(cgr, vj);
+    const T cge_ = bld<T>(cgr, vie), cgw_ = bld<T>(cgr, viw);
+    const T cgs_ = bld<T>(cgr, vis), cgn_ = bld<T>(cgr, vin);
+    // doubled edge group speeds cg(i) + cg(neighbour) (src/umwm_advection.F90:30-34)
+    const T sge = cgi + cge_, sgw = cgi + cgw_, sgs = cgi + cgs_, sgn = cgi + cgn_;
     const T ccu = pload(d.ccurv, ldi, j), scu = pload(d.scurv, ldi, j);
     const T odth = T(d.oneovdth);
     auto rotp = [&](int p) { return rot_dir(dcx, dcy, cld(d.sth, p), cld(d.cth, p), dvc, duc, odx, ody); };
@@ -682,10 +706,7 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
         const T ct = cld(d.cth, p), st = cld(d.sth, p);
         const T cc = ct * ccu - st * scu;  // cos(th + curv)
         const T sn = st * ccu + ct * scu;  // sin(th + curv)
-        const T cge = cgi * cc + cge_ * cc;
-        const T cgw = cgi * cc + cgw_ * cc;
-        const T cgs = cgi * sn + cgs_ * sn;
-        const T cgn = cgi * sn + cgn_ * sn;
+        const T cge = sge * cc, cgw = sgw * cc, cgs = sgs * sn, cgn = sgn * sn;
         const T e0 = A[q].eo, ee = A[q].ee, ew = A[q].ew, en = A[q].en, es = A[q].es;
         T flux = (cge + fabs(cge)) * dye * e0 + (cge - fabs(cge)) * dye * ee -
                  (cgw + fabs(cgw)) * dyw * ew - (cgw - fabs(cgw)) * dyw * e0 +
@@ -714,12 +735,11 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
   }
   if (valid) {
     const int jc = (j == 0) ? d.im : j - 1;  // cp0(o, i-1), src/umwm_stress.F90:219,316
-    const double icp = 1.0 / (double)bld<T>(tb, (uint32_t)jc * (uint32_t)sizeof(T), FT_CP0 * rb);
-    const uint32_t vj8 = (uint32_t)j * 8u, rb8 = (uint32_t)ldi * 8u;
-    const rsrc_t sp = freq_rsrc(d.spart, o, 3, ldi);
-    bst(sp, vj8, tx * icp, 0);
-    bst(sp, vj8, ty * icp, rb8);
-    bst(sp, vj8, hs * (double)bld<T>(tb, vj, FT_KDK * rb), 2 * rb8);
+    const double icp = 1.0 / (double)bld<T>(cpr, (uint32_t)jc * E);
+    const double kdk = bld<T>(row_rsrc(d.kdk, o, ldi), vj);
+    const rsrc_t sp = freq_rsrc(d.spart, o, 4, ldi);
+    bst16(sp, (uint32_t)j * 32u, f64x2{tx * icp, ty * icp});
+    bst(sp, (uint32_t)j * 32u + 16u, hs * kdk);
   }
 }
 
@@ -736,8 +756,7 @@ __global__ __launch_bounds__(64) void k_stress(const DomainDev<T>* __restrict__ 
   if (j >= d.im) return;
   const size_t ldi = d.ldi, ldq = d.ldq;
   const int om = d.om;
-  const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T), vj8 = (uint32_t)j * 8u;
-  const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T), rb8 = (uint32_t)ldi * 8u;
+  constexpr uint32_t E = sizeof(T);
   T w = pload(d.wspd, ldq, j);
   w = w > T(1e-2) ? w : T(1e-2);
   const int oc = pload(d.oc, ldi, j);
@@ -745,10 +764,10 @@ __global__ __launch_bounds__(64) void k_stress(const DomainDev<T>* __restrict__ 
   si.sx = si.sy = si.dx = si.dy = 0;
   double hs = 0;
   for (int o = 0; o < om; ++o) {
-    const double kdk = bld<T>(freq_rsrc(d.ftab, o, FT_N, ldi), vj, FT_KDK * rb);
-    const rsrc_t sp = freq_rsrc(d.spart, o, 3, ldi);
-    const double tx = bld<double>(sp, vj8, 0);
-    const double ty = bld<double>(sp, vj8, rb8);
+    const double kdk = bld<T>(row_rsrc(d.kdk, o, ldi), (uint32_t)j * E);
+    const rsrc_t sp = freq_rsrc(d.spart, o, 4, ldi);
+    const f64x2 txy = bld16<double>(sp, (uint32_t)j * 32u);
+    const double tx = txy[0], ty = txy[1];
     si.sx += tx * kdk;
     si.sy += ty * kdk;
     if (o >= oc) {
@@ -759,7 +778,7 @@ __global__ __launch_bounds__(64) void k_stress(const DomainDev<T>* __restrict__ 
       si.tomx = tx;
       si.tomy = ty;
     }
-    hs += bld<double>(sp, vj8, 2 * rb8);
+    hs += bld<double>(sp, (uint32_t)j * 32u + 16u);
   }
   si.wspd = w;
   si.wdir = pload(d.wdir, ldq, j);
@@ -769,7 +788,7 @@ __global__ __launch_bounds__(64) void k_stress(const DomainDev<T>* __restrict__ 
   si.vs1 = pload(d.vs1, ldi, j);
   si.rhoa = pload(d.rhoa, ldq, j);
   si.rhow = pload(d.rhow, ldq, j);
-  si.kom = bld<T>(freq_rsrc(d.ftab, om - 1, FT_N, ldi), vj, FT_K * rb);
+  si.kom = bld<T>(row_rsrc(d.k, om - 1, ldi), (uint32_t)j * E);
   store_stress(d, j, stress_atm_point(si, d.dthg, d.z, d.nu_air, d.kappa));
   pstore(d.hs_ice, ldi, j, 4 * sqrt(hs * d.dth));
 }

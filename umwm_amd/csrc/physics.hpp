@@ -12,17 +12,16 @@ namespace umwm {
 
 constexpr double kTwoPiD = 2 * 3.141592653589793;  // src/umwm_module.F90:87-89
 
-// x**y for x >= 0 (Fortran real power).  FP64 evaluates exp(y log x) with
-// the device library's FP64 exp and log (relative error ~|y log x| ulp,
-// below 1e-14 for the spectra here); FP32 uses the hardware log2/exp2
-// pair, inside the FP32 path's error budget (Hs within 0.5 % of FP64).
-__device__ inline double pw(double x, double y) {
-  if (!(x > 0.0)) return x == 0.0 ? 0.0 : __builtin_nan("");
-  return exp(y * log(x));
-}
+// x**y (Fortran real power) for the spectral densities, y > 0.  FP64
+// evaluates exp(y log x) with the device library's FP64 exp and log
+// (relative error ~|y log x| ulp, below 1e-14 for the spectra here); FP32
+// uses the hardware log2/exp2 pair, inside the FP32 path's error budget
+// (Hs within 0.5 % of FP64).  A non-positive base gives 0: round-off can
+// leave a bin at -1e-30 where refraction or propagation empties it exactly
+// (Courant number 1), and the reference's x**y would turn that into NaN.
+__device__ inline double pw(double x, double y) { return x > 0.0 ? exp(y * log(x)) : 0.0; }
 __device__ inline float pw(float x, float y) {
-  if (!(x > 0.f)) return x == 0.f ? 0.f : __builtin_nanf("");
-  return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+  return x > 0.f ? __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)) : 0.f;
 }
 
 __device__ inline double ex(double x) { return exp(x); }
