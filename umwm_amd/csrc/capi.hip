@@ -505,6 +505,7 @@ int fused_substep(DomainImpl<T>& D) {
   rc = launch_fused<T>(D.d, D.tasks, D.ntasks, D.h.om, D.first ? 1 : 0, rotation, D.tile,
                        D.stream, D.timing ? D.ev : nullptr);
   if (rc) return rc;
+  D.ef_mirrors_e = true;
   D.rot_valid = true;
   if (D.first) {
     if ((rc = launch_diag(D))) return rc;
@@ -561,7 +562,7 @@ bool find_field(DomainImpl<T>& D, const std::string& n, FieldRef& r) {
   auto set = [&](FieldKind k, void* p) { r.kind = k; r.ptr = p; return true; };
   // spectra
   if (n == "e") return set(F_SPEC, h.e);
-  if (n == "ef") return set(F_SPEC, h.ef);
+  if (n == "ef") return set(F_SPEC, D.ef_mirrors_e ? h.e : h.ef);
   if (n == "es") return set(F_SPEC, h.es);
   if (n == "ssin" || n == "sds" || n == "snl" || n == "dummy" || n == "rotl" || n == "rotr" ||
       n == "sdt" || n == "sice") {
@@ -717,6 +718,7 @@ int get_field_impl(DomainImpl<T>& D, const std::string& name, double* buf, int64
 
 template <typename T>
 int set_field_impl(DomainImpl<T>& D, const std::string& name, const double* buf, int64_t n) {
+  if (name == "ef") D.ef_mirrors_e = false;
   FieldRef r;
   if (!find_field(D, name, r)) return fail(UMWM_ERR_UNKNOWN_FIELD, "unknown field: " + name);
   const int64_t want = field_size(D, r.kind);
@@ -970,6 +972,7 @@ OP_ENTRY(umwm_hip_update, launch_op_update)
 int umwm_hip_source(umwm_domain* dom) {
   return dispatch(dom, [&](auto& D) {
     int rc = ensure_ops(D);
+    D.ef_mirrors_e = false;
     if (!rc) rc = launch_op_source(D);
     if (!rc) rc = sync_and_check(D);
     return rc;
@@ -1219,6 +1222,7 @@ int batch_launch_substep(BatchImpl<T>& B, int first, int rotation) {
   int rc = launch_fused<T>(B.d_doms, B.tasks, B.ntasks, B.omax, first, rotation, B.tile,
                            B.stream, nullptr);
   if (rc) return rc;
+  for (auto* D : B.doms) D->ef_mirrors_e = true;
   const int n = (int)B.doms.size();
   hipLaunchKernelGGL(k_gather_scalars, dim3((n + 63) / 64), dim3(64), 0, B.stream,
                      (Scalars* const*)B.d_scs, n, B.d_view);

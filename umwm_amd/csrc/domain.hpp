@@ -56,6 +56,9 @@ struct DomainBase {
   bool first = true;      // src/umwm_module.F90:47
   bool firstdtg = true;
   bool timing = false;
+  // the fused substep does not store E_s: after it, ef equals e (as after
+  // the reference's e = ef) and reads of "ef" are served from e
+  bool ef_mirrors_e = false;
   double ms[3] = {0, 0, 0};
   int64_t launches[3] = {0, 0, 0};
 };

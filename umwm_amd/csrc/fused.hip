@@ -8,12 +8,15 @@
 //                writes the exclusive frequency prefix of the directional
 //                cos^2 convolution (3 values per frequency and point).
 //   k_integrate  sin + sds + snl + exponential integration + diagnostic
-//                tail (src/umwm_physics.F90:60-84); writes E_s (ef) and
+//                tail (src/umwm_physics.F90:60-84); writes only
 //                E* = (E^n + E_s)/2 (es).
 //   k_advect     upwind propagation on E* (src/umwm_advection.F90:18-99),
 //                refraction (:102-180) and e = ef, one wave per
-//                (frequency, 64 sea points); also the per-frequency partial
-//                sums of stress_vector_integral (src/umwm_stress.F90:313-331).
+//                (frequency, 64 sea points), with E_s recovered in place as
+//                2 E* - E^n from the e it overwrites (one spectrum stream
+//                fewer than storing E_s; the round-off is |E^n + E_s| ulp);
+//                also the per-frequency partial sums of
+//                stress_vector_integral (src/umwm_stress.F90:313-331).
 //   k_stress     per point: frequency sums, tail and the FP64 form/skin
 //                drag chain of stress('atm') (src/umwm_stress.F90:216-264).
 //
@@ -521,8 +524,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     const bool fin = valid && (o >= 2) && (o - 2 <= oc);  // bin o-2 completes now
     const T db1 = dts * b1m1, db2 = dts * b2m2;
     const int o2 = o >= 2 ? o - 2 : 0;
-    const rsrc_t rf0 = freq_rsrc(d.ef, o, pm, ldi), rs0 = freq_rsrc(d.es, o, pm, ldi);
-    const rsrc_t rf2 = freq_rsrc(d.ef, o2, pm, ldi), rs2 = freq_rsrc(d.es, o2, pm, ldi);
+    const rsrc_t rs0 = freq_rsrc(d.es, o, pm, ldi), rs2 = freq_rsrc(d.es, o2, pm, ldi);
     // one bin: X = sds e (before cothkd) feeds snl of bins o-1 and o-2
     auto bin = [&](int q, bool progbin) {
       const int p = p0 + q;
@@ -541,14 +543,12 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
         // diagnostic tail, src/umwm_physics.F90:70-80
         const T num = ss - ctail;
         const T es_ = num >= T(0) ? ok4 * pw(fdiv(num, cfc * dm), ipow) : T(0);
-        bst(rf0, vj, es_, p * rb);
         bst(rs0, vj, T(0.5) * (e + es_), p * rb);
       }
       T g1 = G1[q];
       if (in1) g1 = g1 + db1 * X;  // bf1r(o-1) X(o) into bin o-1
       if (fin) {
         const T es_ = G2[q] + db2 * X;  // bf2r(o-2) X(o) completes bin o-2
-        bst(rf2, vj, es_, p * rb);
         bst(rs2, vj, T(0.5) * (E2[q] + es_), p * rb);
       }
       G2[q] = g1;
@@ -583,7 +583,8 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
 // ---------------------------------------------------------------------
 template <typename T>
 struct AdvLoads {
-  T eo, ee, ew, en, es, ef;
+  T eo, ee, ew, en, es;  // E* at the point and its four neighbours
+  T en0;                 // E^n at the point
 };
 
 template <typename T, int NB>
@@ -613,8 +614,7 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
   const T cp0 = bld<T>(cpr, vj);
   const SinFreq<T> sf = sin_freq(s, bld<T>(row_rsrc(d.logl2overz, o, ldi), vj), cp0,
                                  bld<T>(row_rsrc(d.fkovg, o, ldi), vj), sin_diss1, fs2);
-  const rsrc_t pe = freq_rsrc(d.e, o, pm, ldi);    // E^{n+1} out
-  const rsrc_t pf = freq_rsrc(d.ef, o, pm, ldi);   // E_s
+  const rsrc_t pe = freq_rsrc(d.e, o, pm, ldi);    // E^n in, E^{n+1} out
   const rsrc_t ps = freq_rsrc(d.es, o, pm, ldi);   // E*
   double tx = 0, ty = 0, hs = 0;
   auto stress_acc = [&](T ct, T st, T v) {
@@ -624,11 +624,12 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
     hs += (double)v;
   };
   if (!prog) {
-    // above the cut-off only the diagnostic tail: e = ef
+    // above the cut-off only the diagnostic tail: e = ef = E_s
     for (int pb = 0; pb < pm; pb += NB) {
       T v[NB];
 #pragma unroll
-      for (int q = 0; q < NB; ++q) v[q] = bld<T>(pf, vj, (pb + q) * rb);
+      for (int q = 0; q < NB; ++q)
+        v[q] = T(2) * bld<T>(ps, vj, (pb + q) * rb) - bld<T>(pe, vj, (pb + q) * rb);
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
         if (valid) bst(pe, vj, v[q], (pb + q) * rb);
@@ -686,7 +687,7 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
         L[q].ew = bld<T>(ps, viw, so);
         L[q].en = bld<T>(ps, vin, so);
         L[q].es = bld<T>(ps, vis, so);
-        L[q].ef = bld<T>(pf, vj, so);
+        L[q].en0 = bld<T>(pe, vj, so);
       }
     };
     AdvLoads<T> A[NB], B[NB];
@@ -715,7 +716,7 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
         if (anycur)
           flux = flux + (feup * e0 + fedn * ee - fwup * ew - fwdn * e0 + fnup * e0 + fndn * en -
                          fsup * es - fsdn * e0);
-        T v = A[q].ef - kdta * flux * oneovar;
+        T v = (T(2) * e0 - A[q].en0) - kdta * flux * oneovar;  // E_s - propagation
         const T rl = T(0.5) * (rcur + rnext), rr = T(0.5) * (rcur + rprev);
         const T fr = T(0.5) *
                      ((rl + fabs(rl)) * e0 + (rl - fabs(rl)) * enext - (rr + fabs(rr)) * eprev -
