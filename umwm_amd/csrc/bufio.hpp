@@ -123,6 +123,22 @@ __device__ __forceinline__ T cld(const T* p, int idx) {
   return ((const __attribute__((address_space(4))) T*)p)[idx];
 }
 
+// Four consecutive wave-uniform values (one scalar load).
+template <typename T>
+__device__ __forceinline__ void cld4(const T* p, int idx, T& a, T& b, T& c, T& d) {
+  typedef T v4 __attribute__((ext_vector_type(4)));
+  const v4 v = ((const __attribute__((address_space(4))) v4*)p)[idx];
+  a = v[0];
+  b = v[1];
+  c = v[2];
+  d = v[3];
+}
+
+// A lane offset past every descriptor's range: the hardware range check
+// drops a store made with it (descriptors cover less than 2^31 bytes), so
+// stores can be predicated per lane without a divergent branch.
+constexpr uint32_t kDropOffset = 0x80000000u;
+
 // Wave index of the calling thread as a wave-uniform value.
 __device__ __forceinline__ int wave_id() {
   return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));

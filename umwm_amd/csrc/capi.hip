@@ -219,11 +219,11 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   const int om = S.om, pm = S.pm, ldi = S.ldi, ldq = S.ldq;
   const size_t ns = (size_t)om * pm * ldi, nt = (size_t)om * ldi;
   // the fused kernels address one frequency of a spectrum (pm rows), of the
-  // table block (FT_N rows) or of a per-point array with one 32-bit buffer
-  // descriptor each
+  // table block (FT_N rows) or of a per-point array with one buffer
+  // descriptor each, and drop predicated stores with an offset of 2^31
   const size_t maxrows = std::max<size_t>({(size_t)pm, (size_t)FT_N, 4});
-  if (maxrows * ldi * std::max(sizeof(T), sizeof(double)) >= (size_t(1) << 32) ||
-      (size_t)ldq * sizeof(double) >= (size_t(1) << 32)) {
+  if (maxrows * ldi * std::max(sizeof(T), sizeof(double)) >= (size_t(1) << 31) ||
+      (size_t)ldq * sizeof(double) >= (size_t(1) << 31)) {
     delete D;
     return fail(UMWM_ERR_SIZE, "domain too large for 32-bit buffer descriptors");
   }
@@ -271,6 +271,19 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   h.invcp0 = icp; h.fkovg = fkg; h.logl2overz = ll2; h.cothkd = cth; h.sbf = sbf; h.sdv = sdv;
   h.snl_arg = sarg; h.bf1r = b1; h.bf2r = b2; h.dwn = dwn; h.cth_curv = ccv; h.sth_curv = scv;
   h.f = f; h.th = th; h.cth = ct; h.sth = st; h.c2th = c2; h.s2th = s2; h.cth2pp = c2pp;
+  {
+    std::vector<double> d4((size_t)4 * pm);
+    for (int p = 0; p < pm; ++p) {
+      d4[4 * p] = S.cth[p];
+      d4[4 * p + 1] = S.sth[p];
+      d4[4 * p + 2] = S.c2th[p];
+      d4[4 * p + 3] = S.s2th[p];
+    }
+    T* dd = A.template alloc<T>((size_t)4 * pm);
+    if (!dd) { delete D; return fail(UMWM_ERR_NOMEM, "device allocation failed"); }
+    if ((rc = upload<T>(dd, d4, (size_t)4 * pm, s))) { delete D; return rc; }
+    h.dir4 = dd;
+  }
   h.pl = pl; h.pr = pr;
   // per-(frequency, point) table records for the fused kernels, [o][ldi][FT_N]
   {

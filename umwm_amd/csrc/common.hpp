@@ -70,6 +70,7 @@ struct DomainDev {
   const T *cth_curv, *sth_curv;
   // spectral constants
   const T *f, *th, *cth, *sth, *c2th, *s2th, *cth2pp;
+  const T* dir4;  // per direction {cth, sth, cos 2th, sin 2th}, one 4-value record
   const int32_t *pl, *pr;
   // per unrolled point (ldq)
   T *wspd, *wdir, *uc, *vc, *rhoa, *rhow, *rhorat, *fice;

@@ -51,6 +51,8 @@
 // loads); the prefix moments and stress partial sums are 4-value records.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "bufio.hpp"
 #include "domain.hpp"
 #include "physics.hpp"
@@ -219,6 +221,41 @@ __device__ __forceinline__ void load_tab(const DomainDev<T>& d, int o, uint32_t 
   for (int i = N; i < FT_N; ++i) r.v[i] = T(0);
 }
 
+// Transcendentals of the sweeps.  FP32 works in base 2 on the hardware
+// log2/exp2 (exponents pre-scaled by log2 e where a natural exponential is
+// meant); FP64 uses the device library's natural log and exp.
+template <typename T>
+struct Tr;
+template <>
+struct Tr<float> {
+  static constexpr float kExpScale = 1.4426950408889634f;  // log2(e)
+  __device__ static __forceinline__ float lg(float x) { return __builtin_amdgcn_logf(x); }
+  __device__ static __forceinline__ float ex(float x) { return __builtin_amdgcn_exp2f(x); }
+  __device__ static __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+};
+template <>
+struct Tr<double> {
+  static constexpr double kExpScale = 1.0;
+  __device__ static __forceinline__ double lg(double x) { return log(x); }
+  __device__ static __forceinline__ double ex(double x) { return exp(x); }
+  __device__ static __forceinline__ double rcp(double x) { return 1.0 / x; }
+};
+
+// Per-frequency wind-input factors in the form the sweeps use: ssin =
+// y * (y > 0 ? pos : neg1 * m), y = |x| x, x = A cth + B sth - cp0, with
+// m = fieldscale2 where the wind overruns the direction (cos(wdir - th) >
+// 0), else 1 (a per-lane, per-direction constant).
+template <typename T>
+__device__ __forceinline__ T sin_y(const SinFreq<T>& q, T ct, T st) {
+  const T x = fma(q.A, ct, fma(q.B, st, -q.cp0));
+  return fabs(x) * x;
+}
+template <typename T>
+__device__ __forceinline__ T sin_m(const SinFreq<T>& q, T y, T m) {
+  const T pos = q.pos, neg = q.neg1 * m;
+  return y * (y > T(0) ? pos : neg);
+}
+
 // ---------------------------------------------------------------------
 // k_rates
 // ---------------------------------------------------------------------
@@ -262,6 +299,14 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     const T sin_diss1 = T(d.sin_fac * d.fieldscale1), fs2 = T(d.fieldscale2);
     const T tpsf = T(d.twopisds_fac), spow = T(d.sds_power), mssf = T(d.mss_fac);
     const double hdth = 0.5 * d.dth;
+    // overrun factor per direction of this wave (frequency independent)
+    T mq[DMAX];
+#pragma unroll
+    for (int q = 0; q < DMAX; ++q) {
+      T ct = 0, st = 0, c2 = 0, s2 = 0;
+      if (q < nd) cld4(d.dir4, p0 + q, ct, st, c2, s2);
+      mq[q] = s.cw * ct + s.sw * st > T(0) ? fs2 : T(1);
+    }
     double A0 = 0, AC = 0, AS = 0;
     T rmax = 0;
     T ec[DMAX];
@@ -271,7 +316,10 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) ec[q] = (q < nd) ? bld<T>(ep, vj, (p0 + q) * rb) : T(0);
     }
-    load_tab<kRatesSlots>(d, 0, vr, tc);
+    // FP32 prefetches the next frequency's table record with its spectrum;
+    // FP64 (library exp/log, many live registers) loads it when needed
+    constexpr bool kPrefetchTab = sizeof(T) == 4;
+    if constexpr (kPrefetchTab) load_tab<kRatesSlots>(d, 0, vr, tc);
     for (int o = 0; o < om; ++o) {
       // issue the next frequency's loads before computing this one
       const int on = o + 1 < om ? o + 1 : o;
@@ -280,7 +328,10 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       const rsrc_t epn = freq_rsrc(d.e, on, pm, ldi);
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) en[q] = (q < nd) ? bld<T>(epn, vj, (p0 + q) * rb) : T(0);
-      load_tab<kRatesSlots>(d, on, vr, tn);
+      if constexpr (kPrefetchTab)
+        load_tab<kRatesSlots>(d, on, vr, tn);
+      else
+        load_tab<kRatesSlots>(d, o, vr, tc);
 
       const T sdt = s.csdt * tc[FT_K];
       const T sice = T(2) * tc[FT_CG0] * s.ice;
@@ -288,35 +339,36 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       const SinFreq<T> sf = sin_freq(s, tc[FT_LOGL2], tc[FT_CP0], tc[FT_FKOVG], sin_diss1, fs2);
       const T pa0 = T(hdth * A0), pac = T(hdth * AC), pas = T(hdth * AS);
       if (valid && w == 0) {
-        typename Vec16<T>::type v;
         if constexpr (sizeof(T) == 4) {
-          v = typename Vec16<T>::type{pa0, pac, pas, T(0)};
-          bst16(freq_rsrc(d.pfx, o, 4, ldi), (uint32_t)j * 16u, v);
+          bst16(freq_rsrc(d.pfx, o, 4, ldi), (uint32_t)j * 16u,
+                typename Vec16<T>::type{pa0, pac, pas, T(0)});
         } else {
           const rsrc_t pf = freq_rsrc(d.pfx, o, 4, ldi);
           bst16(pf, (uint32_t)j * 32u, typename Vec16<T>::type{pa0, pac});
           bst16(pf, (uint32_t)j * 32u + 16u, typename Vec16<T>::type{pas, T(0)});
         }
       }
-      const T cf = tpsf * cld(d.f, o) * tc[FT_COTH];
-      const T k4 = tc[FT_K4], sarg = tc[FT_SARG];
+      // sds (after cothkd) = cf coth dm^2 (e k4)^p = dm^2 ex(p lg e + Lk)
+      const T cfo = tpsf * cld(d.f, o) * tc[FT_COTH];
+      const T Lk = spow * Tr<T>::lg(tc[FT_K4]) + Tr<T>::lg(cfo);
+      const T cs = tc[FT_SARG];
+      const T q0 = T(1) + mssf * pa0, qc = mssf * pac, qs = mssf * pas;
       T m0 = 0, mc = 0, ms = 0, rq = 0;
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) {
         if (q < nd) {
-          const int p = p0 + q;
+          T ct, st, c2, s2;
+          cld4(d.dir4, p0 + q, ct, st, c2, s2);
           const T e = ec[q];
-          const T ct = cld(d.cth, p), st = cld(d.sth, p);
-          const T c2 = cld(d.c2th, p), s2 = cld(d.s2th, p);
-          const T ss = sin_bin(sf, s, ct, st, false);
-          T dm = T(1) + mssf * (pa0 + pac * c2 + pas * s2);
+          const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
+          T dm = fma(qc, c2, fma(qs, s2, q0));
           dm = dm * dm;
-          const T sds = cf * dm * pw(e * k4, spow);
-          const T r = ss - sds * sarg - crest;
+          const T sds = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fmax(e, T(0))), Lk));
+          const T r = ss - fma(sds, cs, crest);
           rq = fmax(rq, fabs(r));
           m0 += e;
-          mc += e * c2;
-          ms += e * s2;
+          mc = fma(e, c2, mc);
+          ms = fma(e, s2, ms);
         }
       }
       if (o <= oc) rmax = fmax(rmax, rq);  // rates of prognostic bins only
@@ -356,7 +408,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       AS += k3 * s2s;
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) ec[q] = en[q];
-      tc = tn;
+      if constexpr (kPrefetchTab) tc = tn;
     }
     // max |rate| of the point over all waves
     __syncthreads();
@@ -469,24 +521,31 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   const DomainDev<T>& d = doms[tk.dom];
   const Scalars* sc = d.sc;
   if (!sc->active) return;
-  const int W = blockDim.x >> 6, lane = threadIdx.x & 63, w = wave_id();
-  const int j = tk.i0 + lane;
+  const int W = blockDim.x >> 6, w = wave_id();
+  const int j = tk.i0 + (threadIdx.x & 63);
   const bool valid = j < d.im;
   const int pm = d.pm, om = d.om;
   const size_t ldi = d.ldi;
   const int D = (pm + W - 1) / W, p0 = w * D;
   const int nd = max(0, min(D, pm - p0));
   const T dts = T(sc->dts);
+  const T dtx = dts * Tr<T>::kExpScale;  // dts in the exponential's base
   const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
   const uint32_t vr = (uint32_t)j * (uint32_t)(FT_N * sizeof(T));
   const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);
-  (void)lane;
 
   const PointState<T> s = point_stored(d, j);
   const int oc = s.oc;
   const T sin_diss1 = T(d.sin_fac * d.fieldscale1), fs2 = T(d.fieldscale2);
   const T tpsf = T(d.twopisds_fac), spow = T(d.sds_power), ipow = T(d.inv_sds_power);
   const T mssf = T(d.mss_fac), dsnl = dts * T(d.snl_fac);
+  T mq[DMAX];
+#pragma unroll
+  for (int q = 0; q < DMAX; ++q) {
+    T ct = 0, st = 0, c2 = 0, s2 = 0;
+    if (q < nd) cld4(d.dir4, p0 + q, ct, st, c2, s2);
+    mq[q] = s.cw * ct + s.sw * st > T(0) ? fs2 : T(1);
+  }
   // per direction: partial E_s and E^n of the two previous frequencies
   T G1[DMAX], G2[DMAX], E1[DMAX], E2[DMAX];
 #pragma unroll
@@ -499,8 +558,12 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     for (int q = 0; q < DMAX; ++q) ec[q] = (q < nd) ? bld<T>(ep, vj, (p0 + q) * rb) : T(0);
   }
   FreqTab<T> tc;
-  load_tab<FT_N>(d, 0, vr, tc);
-  Prefix<T> pc = load_prefix(d, 0, j);
+  Prefix<T> pc{};
+  constexpr bool kPrefetchTab = sizeof(T) == 4;  // as in k_rates
+  if constexpr (kPrefetchTab) {
+    load_tab<FT_N>(d, 0, vr, tc);
+    pc = load_prefix(d, 0, j);
+  }
   for (int o = 0; o < om; ++o) {
     const int on = o + 1 < om ? o + 1 : o;
     T en[DMAX];
@@ -508,8 +571,14 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) en[q] = (q < nd) ? bld<T>(epn, vj, (p0 + q) * rb) : T(0);
     FreqTab<T> tn;
-    load_tab<FT_N>(d, on, vr, tn);
-    const Prefix<T> pn = load_prefix(d, on, j);
+    Prefix<T> pn{};
+    if constexpr (kPrefetchTab) {
+      load_tab<FT_N>(d, on, vr, tn);
+      pn = load_prefix(d, on, j);
+    } else {
+      load_tab<FT_N>(d, o, vr, tc);
+      pc = load_prefix(d, o, j);
+    }
 
     const T sdt = s.csdt * tc[FT_K];
     const T sice = T(2) * tc[FT_CG0] * s.ice;
@@ -517,61 +586,68 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     const T ctail = sdt + tc[FT_SDV] - sice;
     const SinFreq<T> sf = sin_freq(s, tc[FT_LOGL2], tc[FT_CP0], tc[FT_FKOVG], sin_diss1, fs2);
     const T cf = tpsf * cld(d.f, o);
-    const T coth = tc[FT_COTH], k4 = tc[FT_K4], ok4 = tc[FT_OK4];
-    const T cfc = cf * coth;
+    const T coth = tc[FT_COTH], ok4 = tc[FT_OK4];
+    const T Lk = spow * Tr<T>::lg(tc[FT_K4]) + Tr<T>::lg(cf);  // sds before cothkd
+    const T c0 = -dtx * crest, dtc = dtx * coth;
+    const T icfc = Tr<T>::rcp(cf * coth);
+    const T q0 = T(1) + mssf * pc.a0, qc = mssf * pc.ac, qs = mssf * pc.as;
     const bool prog = o <= oc;
     const bool in1 = (o >= 1) && (o - 1 <= oc);  // bin o-1 is prognostic
     const bool fin = valid && (o >= 2) && (o - 2 <= oc);  // bin o-2 completes now
-    const T db1 = dts * b1m1, db2 = dts * b2m2;
-    const int o2 = o >= 2 ? o - 2 : 0;
-    const rsrc_t rs0 = freq_rsrc(d.es, o, pm, ldi), rs2 = freq_rsrc(d.es, o2, pm, ldi);
-    // one bin: X = sds e (before cothkd) feeds snl of bins o-1 and o-2
-    auto bin = [&](int q, bool progbin) {
-      const int p = p0 + q;
-      const T e = ec[q];
-      const T ct = cld(d.cth, p), st = cld(d.sth, p);
-      const T c2 = cld(d.c2th, p), s2 = cld(d.s2th, p);
-      const T ss = sin_bin(sf, s, ct, st, !progbin);
-      T dm = T(1) + mssf * (pc.a0 + pc.ac * c2 + pc.as * s2);
-      dm = dm * dm;
-      const T sdsp = cf * dm * pw(e * k4, spow);
-      const T X = sdsp * e;
-      T G = 0;
-      if (progbin) {
-        G = e * ex(dts * (ss - sdsp * coth - crest)) - dsnl * X;
-      } else if (valid) {
-        // diagnostic tail, src/umwm_physics.F90:70-80
-        const T num = ss - ctail;
-        const T es_ = num >= T(0) ? ok4 * pw(fdiv(num, cfc * dm), ipow) : T(0);
-        bst(rs0, vj, T(0.5) * (e + es_), p * rb);
-      }
-      T g1 = G1[q];
-      if (in1) g1 = g1 + db1 * X;  // bf1r(o-1) X(o) into bin o-1
-      if (fin) {
-        const T es_ = G2[q] + db2 * X;  // bf2r(o-2) X(o) completes bin o-2
-        bst(rs2, vj, T(0.5) * (E2[q] + es_), p * rb);
-      }
-      G2[q] = g1;
-      G1[q] = G;
-      E2[q] = E1[q];
-      E1[q] = e;
-    };
-    if (prog) {
+    const T db1 = in1 ? dts * b1m1 : T(0), db2 = dts * b2m2;
+    const uint32_t vtail = (valid && !prog) ? vj : kDropOffset;
+    const uint32_t vfin = fin ? vj : kDropOffset;
+    const bool anyfin = __any(fin), anytail = __any(valid && !prog);
+    const rsrc_t rs0 = freq_rsrc(d.es, o, pm, ldi);
+    const rsrc_t rs2 = freq_rsrc(d.es, o >= 2 ? o - 2 : 0, pm, ldi);
 #pragma unroll
-      for (int q = 0; q < DMAX; ++q)
-        if (q < nd) bin(q, true);
-    } else {
-#pragma unroll
-      for (int q = 0; q < DMAX; ++q)
-        if (q < nd) bin(q, false);
-    }
+      for (int q = 0; q < DMAX; ++q) {
+        if (q < nd) {
+          const int p = p0 + q;
+          T ct, st, c2, s2;
+          cld4(d.dir4, p, ct, st, c2, s2);
+          const T e = ec[q];
+          const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
+          T dm = fma(qc, c2, fma(qs, s2, q0));
+          dm = dm * dm;
+          const T sdsp = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fmax(e, T(0))), Lk));
+          const T X = sdsp * e;
+          T G = 0;
+          if (prog) {
+            // E_s = e exp(dts (ssin - sds - sbf - sdt - sdv + sice)) + dts snl
+            const T a = fma(-dtc, sdsp, fma(dtx, ss, c0));
+            G = fma(-dsnl, X, e * Tr<T>::ex(a));
+          }
+          if (anytail) {
+            // diagnostic tail, src/umwm_physics.F90:70-80
+            T es_ = 0;
+            if (!prog) {
+              const T num = fmax(ss, T(0)) - ctail;
+              const T t = fmax(num, T(0)) * (icfc * Tr<T>::rcp(dm));
+              es_ = ok4 * Tr<T>::ex(ipow * Tr<T>::lg(t));
+            }
+            bst(rs0, vtail, T(0.5) * (e + es_), p * rb);
+          }
+          const T g1 = fma(db1, X, G1[q]);  // bf1r(o-1) X(o) into bin o-1
+          if (anyfin) {
+            const T es_ = fma(db2, X, G2[q]);  // bf2r(o-2) X(o) completes bin o-2
+            bst(rs2, vfin, T(0.5) * (E2[q] + es_), p * rb);
+          }
+          G2[q] = g1;
+          G1[q] = G;
+          E2[q] = E1[q];
+          E1[q] = e;
+        }
+      }
     b2m2 = b2m1;
     b2m1 = tc[FT_B2];
     b1m1 = tc[FT_B1];
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) ec[q] = en[q];
-    tc = tn;
-    pc = pn;
+    if constexpr (kPrefetchTab) {
+      tc = tn;
+      pc = pn;
+    }
   }
 }
 
@@ -580,6 +656,13 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
 // 64 sea points); workgroups of FW waves take FW consecutive frequencies.
 // Directions are processed in blocks of NB (pm is a multiple of 4) with
 // the next block's loads in flight while the current block is computed.
+//
+// The upwind flux of src/umwm_advection.F90:28-44 is evaluated by sign:
+// both x-edge speeds cg(i)+cg(nbr) times cos(th + curv) share the sign of
+// cos(th + curv), so the x flux is c (Ae E*(upwind east) - Aw E*(upwind
+// west)), Ae = 2 (cg(i) + cg(ie)) dye, and likewise in y; refraction's
+// upwind flux (:166-172) is r (E* upwind) per edge.  Regrouped, equal to
+// the reference up to rounding.
 // ---------------------------------------------------------------------
 template <typename T>
 struct AdvLoads {
@@ -600,6 +683,7 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
   const int pm = d.pm;
   const size_t ldi = d.ldi, ldq = d.ldq;
   const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
+  const uint32_t vst = valid ? vj : kDropOffset;
   const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);
   constexpr uint32_t E = sizeof(T);
   const T kdta = T(0.25) * T(sc->dta), dtr = T(sc->dtr);
@@ -616,23 +700,28 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
                                  bld<T>(row_rsrc(d.fkovg, o, ldi), vj), sin_diss1, fs2);
   const rsrc_t pe = freq_rsrc(d.e, o, pm, ldi);    // E^n in, E^{n+1} out
   const rsrc_t ps = freq_rsrc(d.es, o, pm, ldi);   // E*
-  double tx = 0, ty = 0, hs = 0;
+  // stress_vector_integral partial sums over the directions of this
+  // frequency (FP32 within the frequency in the FP32 path, FP64 across)
+  T tx = 0, ty = 0, hs = 0;
   auto stress_acc = [&](T ct, T st, T v) {
-    const double es_ = (double)v * (double)sin_bin(sf, s, ct, st, !prog);
-    tx += es_ * (double)ct;
-    ty += es_ * (double)st;
-    hs += (double)v;
+    const T m = s.cw * ct + s.sw * st > T(0) ? fs2 : T(1);
+    T ss = sin_m(sf, sin_y(sf, ct, st), m);
+    if (!prog) ss = fmax(ss, T(0));
+    const T wv = v * ss;
+    tx = fma(wv, ct, tx);
+    ty = fma(wv, st, ty);
+    hs += v;
   };
-  if (!prog) {
+  if (__all(!prog || !valid)) {
     // above the cut-off only the diagnostic tail: e = ef = E_s
     for (int pb = 0; pb < pm; pb += NB) {
       T v[NB];
 #pragma unroll
       for (int q = 0; q < NB; ++q)
-        v[q] = T(2) * bld<T>(ps, vj, (pb + q) * rb) - bld<T>(pe, vj, (pb + q) * rb);
+        v[q] = fma(T(2), bld<T>(ps, vj, (pb + q) * rb), -bld<T>(pe, vj, (pb + q) * rb));
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
-        if (valid) bst(pe, vj, v[q], (pb + q) * rb);
+        bst(pe, vst, v[q], (pb + q) * rb);
         stress_acc(cld(d.cth, pb + q), cld(d.sth, pb + q), v[q]);
       }
     }
@@ -643,8 +732,9 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
     const uint32_t vin = (uint32_t)in * E, vis = (uint32_t)is * E;
     const T dye = pload(d.dye, ldi, j), dyw = pload(d.dyw, ldi, j);
     const T dxn = pload(d.dxn, ldi, j), dxs = pload(d.dxs, ldi, j);
-    const T oneovar = pload(d.oneovar, ldi, j);
-    const bool iced = pload(d.fice, ldq, j) > T(d.fice_uth);
+    const T kk = kdta * pload(d.oneovar, ldi, j);
+    // ice above fice_uth zeroes the update (src/umwm_advection.F90:93-96)
+    const T keep = pload(d.fice, ldq, j) > T(d.fice_uth) ? T(0) : T(1);
     T feup = 0, fedn = 0, fwup = 0, fwdn = 0, fnup = 0, fndn = 0, fsup = 0, fsdn = 0;
     if (anycur) {
       const int iie = pload(d.iie, ldi, j), iiw = pload(d.iiw, ldi, j);
@@ -660,24 +750,25 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
       fsup = (vs + fabs(vs)) * dxs;
       fsdn = (vs - fabs(vs)) * dxs;
     }
-    // rotation inputs (zero above the first substep's cut-off)
-    T dcx = 0, dcy = 0, dvc = 0, duc = 0, odx = 0, ody = 0;
+    // rotation: r(p)/2 = rx sth + ry cth + r0 (zero above the first
+    // substep's cut-off), src/umwm_advection.F90:119-156
+    T rx = 0, ry = 0, r0 = 0;
     if (rot) {
-      dcx = bld<T>(cpr, vie) - bld<T>(cpr, viw);
-      dcy = bld<T>(cpr, vin) - bld<T>(cpr, vis);
-      dvc = pload(d.vc, ldq, pload(d.iie, ldi, j)) - pload(d.vc, ldq, pload(d.iiw, ldi, j));
-      duc = pload(d.uc, ldq, pload(d.iin, ldi, j)) - pload(d.uc, ldq, pload(d.iis, ldi, j));
-      odx = pload(d.oneovdx, ldi, j);
-      ody = pload(d.oneovdy, ldi, j);
+      const T dcx = bld<T>(cpr, vie) - bld<T>(cpr, viw);
+      const T dcy = bld<T>(cpr, vin) - bld<T>(cpr, vis);
+      const T dvc = pload(d.vc, ldq, pload(d.iie, ldi, j)) - pload(d.vc, ldq, pload(d.iiw, ldi, j));
+      const T duc = pload(d.uc, ldq, pload(d.iin, ldi, j)) - pload(d.uc, ldq, pload(d.iis, ldi, j));
+      const T odx = pload(d.oneovdx, ldi, j), ody = pload(d.oneovdy, ldi, j);
+      rx = T(0.25) * dcx * odx;
+      ry = T(-0.25) * dcy * ody;
+      r0 = T(0.25) * (dvc * odx - duc * ody);
     }
     const T cgi = bld<T>(cgr, vj);
-    const T cge_ = bld<T>(cgr, vie), cgw_ = bld<T>(cgr, viw);
-    const T cgs_ = bld<T>(cgr, vis), cgn_ = bld<T>(cgr, vin);
-    // doubled edge group speeds cg(i) + cg(neighbour) (src/umwm_advection.F90:30-34)
-    const T sge = cgi + cge_, sgw = cgi + cgw_, sgs = cgi + cgs_, sgn = cgi + cgn_;
+    const T Ae = T(2) * (cgi + bld<T>(cgr, vie)) * dye, Aw = T(2) * (cgi + bld<T>(cgr, viw)) * dyw;
+    const T As = T(2) * (cgi + bld<T>(cgr, vis)) * dxs, An = T(2) * (cgi + bld<T>(cgr, vin)) * dxn;
     const T ccu = pload(d.ccurv, ldi, j), scu = pload(d.scurv, ldi, j);
-    const T odth = T(d.oneovdth);
-    auto rotp = [&](int p) { return rot_dir(dcx, dcy, cld(d.sth, p), cld(d.cth, p), dvc, duc, odx, ody); };
+    const T kr = dtr * T(d.oneovdth);
+    auto rotp = [&](int p) { return fma(rx, cld(d.sth, p), fma(ry, cld(d.cth, p), r0)); };
     auto load_block = [&](int pb, AdvLoads<T>* L) {
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
@@ -705,26 +796,21 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
         const int pn = p + 1 < pm ? p + 1 : 0;
         const T rnext = rotp(pn);
         const T ct = cld(d.cth, p), st = cld(d.sth, p);
-        const T cc = ct * ccu - st * scu;  // cos(th + curv)
-        const T sn = st * ccu + ct * scu;  // sin(th + curv)
-        const T cge = sge * cc, cgw = sgw * cc, cgs = sgs * sn, cgn = sgn * sn;
+        const T cc = fma(ct, ccu, -st * scu);  // cos(th + curv)
+        const T sn = fma(st, ccu, ct * scu);   // sin(th + curv)
         const T e0 = A[q].eo, ee = A[q].ee, ew = A[q].ew, en = A[q].en, es = A[q].es;
-        T flux = (cge + fabs(cge)) * dye * e0 + (cge - fabs(cge)) * dye * ee -
-                 (cgw + fabs(cgw)) * dyw * ew - (cgw - fabs(cgw)) * dyw * e0 +
-                 (cgn + fabs(cgn)) * dxn * e0 + (cgn - fabs(cgn)) * dxn * en -
-                 (cgs + fabs(cgs)) * dxs * es - (cgs - fabs(cgs)) * dxs * e0;
+        const bool px = cc > T(0), py = sn > T(0);
+        T flux = cc * fma(Ae, px ? e0 : ee, -Aw * (px ? ew : e0)) +
+                 sn * fma(An, py ? e0 : en, -As * (py ? es : e0));
         if (anycur)
           flux = flux + (feup * e0 + fedn * ee - fwup * ew - fwdn * e0 + fnup * e0 + fndn * en -
                          fsup * es - fsdn * e0);
-        T v = (T(2) * e0 - A[q].en0) - kdta * flux * oneovar;  // E_s - propagation
-        const T rl = T(0.5) * (rcur + rnext), rr = T(0.5) * (rcur + rprev);
-        const T fr = T(0.5) *
-                     ((rl + fabs(rl)) * e0 + (rl - fabs(rl)) * enext - (rr + fabs(rr)) * eprev -
-                      (rr - fabs(rr)) * e0) *
-                     odth;
-        v = v - dtr * fr;
-        if (iced) v = 0;
-        if (valid) bst(pe, vj, v, p * rb);
+        const T Es = fma(T(2), e0, -A[q].en0);  // E_s = 2 E* - E^n
+        const T rl = rcur + rnext, rr = rcur + rprev;  // edge rotation rates
+        const T fr = fma(rl, rl > T(0) ? e0 : enext, -rr * (rr > T(0) ? eprev : e0));
+        T v = fma(-kr, fr, fma(-kk, flux, Es));
+        v = prog ? v * keep : Es;
+        bst(pe, vst, v, p * rb);
         stress_acc(ct, st, v);
         eprev = e0;
         rprev = rcur;
@@ -739,8 +825,8 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
     const double icp = 1.0 / (double)bld<T>(cpr, (uint32_t)jc * E);
     const double kdk = bld<T>(row_rsrc(d.kdk, o, ldi), vj);
     const rsrc_t sp = freq_rsrc(d.spart, o, 4, ldi);
-    bst16(sp, (uint32_t)j * 32u, f64x2{tx * icp, ty * icp});
-    bst(sp, (uint32_t)j * 32u + 16u, hs * kdk);
+    bst16(sp, (uint32_t)j * 32u, f64x2{(double)tx * icp, (double)ty * icp});
+    bst(sp, (uint32_t)j * 32u + 16u, (double)hs * kdk);
   }
 }
 
