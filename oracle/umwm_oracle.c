@@ -13,7 +13,30 @@
  * modules use NetCDF-Fortran and datetime-fortran, neither of which exists
  * in this image.
  *
- * Pinning status of each part is recorded in tests/test_oracle_pins.py.
+ * Pinning (tests/test_oracle_pins.py; the reference ships no tests, golden
+ * vectors or fixtures for this path, so every pin is a closed form, an
+ * invariant, a textbook reduction or a hand-enumerated case):
+ *   pinned   sheltering_coare35, stress_tail, friction_velocity_skin,
+ *            drag_coefficient_skin, distance_haversine (closed forms);
+ *            f, th, cth2pp, bf1/bf2, snl_arg, dom, cfllim, dtamin (closed
+ *            forms); dispersion k, cp0, cg0 (root solve of the dispersion
+ *            relation, deep-water limits); masks/remap index maps
+ *            (hand-enumerated regional and periodic masks, estuary fill);
+ *            init_jonswap + diag Hs/Tm (closed form, Hs = 0.607 m);
+ *            sds_d12 (isotropic prefix closed form); snl_d12 (energy
+ *            telescoping); propagation (conservation, exact first moment,
+ *            steady uniform field); refraction (per-frequency conservation,
+ *            dtr bound); stokes_drift us/vs (textbook Stokes drift of one
+ *            bin, deep and finite depth); source dts choice, E* = (e+E_s)/2
+ *            and the diagnostic tail (breaking/input equilibrium);
+ *            cold first substep.
+ *   partial  sin_d12 (signs, monotonicity, tail clamp; the magnitude of
+ *            each factor only through its closed-form pieces).
+ *   parity unpinned: the exponential integration of prognostic bins in
+ *            source; refraction's rotation magnitude; the stress('atm')
+ *            frequency/direction integral (incl. the cp0(o,i-1) quirk) and
+ *            its tail; the Stokes e-folding depth; diag outputs other than
+ *            Hs and Tm; forcinginterpolate.
  */
 #include "umwm_oracle.h"
 

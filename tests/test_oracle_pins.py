@@ -464,3 +464,88 @@ def test_cold_first_substep_and_calm_sea(O):
     for _ in range(3):
         o.substep()
     assert np.all(o.get("e") >= 0)
+
+
+# ------------------------------------------------------------------ Stokes
+@pytest.mark.parametrize("dpt", [4000.0, 20.0])
+def test_stokes_drift_single_bin_textbook(O, dpt):
+    """src/umwm_stokes.F90:49-100 for a spectrum with one nonzero bin: the
+    drift is the textbook Stokes drift of a monochromatic wave,
+    u_s(z) = w k a^2 cosh(2k(z + d)) / (2 sinh^2(kd)) (deep water: w k a^2
+    e^{2kz}), with a^2 = 2 x variance = 2 e kdk dth, along the bin's
+    direction; levels below the bottom carry no drift."""
+    depths = [0.5, 1.0, 2.0, 5.0, 10.0, 15.0, 30.0]
+    o = regional(O, mm=6, nm=5, om=25, pm=24, dpt=dpt, depths=depths)
+    o.stokes_drift(True)
+    e = o.get("e")
+    e[:] = 0
+    q, p = 8, 3
+    e[:, p, q] = 0.05
+    o.stokes_drift(False)
+    us, vs = o.get("us"), o.get("vs")
+    f = o.get("f")[q]
+    k = o.get("k")[0, q]
+    kdk = o.get("kdk")[0, q]
+    th = o.get("th")[p]
+    dth = 2 * math.pi / 24
+    a2 = 2 * 0.05 * kdk * dth
+    w = 2 * math.pi * f
+    for l, z in enumerate(depths):
+        if z > dpt:
+            expect = 0.0
+        elif k * dpt > 50 or 2 * k * (dpt - z) > 50:
+            expect = w * k * a2 * math.exp(-2 * k * z)
+        else:
+            expect = w * k * a2 * math.cosh(2 * k * (dpt - z)) / (2 * math.sinh(k * dpt) ** 2)
+        assert np.allclose(us[l], expect * math.cos(th), rtol=1e-12, atol=1e-300)
+        assert np.allclose(vs[l], expect * math.sin(th), rtol=1e-12, atol=1e-300)
+
+
+# --------------------------------------------------------------- source()
+def _after_source(O):
+    o = regional(O, mm=7, nm=6, om=37, pm=36, dpt=60.0, wspd0=14.0, wdir0=0.4)
+    o.init_jonswap(abi.make_jonswap(fp=0.15))
+    o.substep()  # cold substep (dts = 0)
+    o.forcinginterpolate()
+    o.sin_d12()
+    o.sds_d12()
+    o.snl_d12()
+    o.s_ice()
+    e0 = o.get("e").copy()
+    sumt0 = o.scalar("sumt")
+    o.source()
+    return o, e0, sumt0
+
+
+def test_source_time_step_and_half_step(O):
+    """src/umwm_physics.F90:41-84: dts = min(min(physics_time_step), dtamin,
+    dtg - sumt) after the cold substep, sumt advances by dts, and the spectrum
+    left in e is E* = (E^n + E_s)/2 with E_s in ef."""
+    o, e0, sumt0 = _after_source(O)
+    pts = o.get("physics_time_step")[: o.im]
+    dts = o.scalar("dts")
+    assert dts == min(float(pts.min()), o.scalar("dtamin"), 3600.0 - sumt0)
+    assert o.scalar("sumt") == pytest.approx(sumt0 + dts, rel=1e-15)
+    assert np.allclose(o.get("e"), 0.5 * (e0 + o.get("ef")), rtol=1e-15, atol=0)
+
+
+def test_source_diagnostic_tail_is_an_equilibrium(O):
+    """src/umwm_physics.F90:70-80: above the cut-off, E_s is the spectrum at
+    which breaking balances input: 2 pi sds_fac f dummy cothkd (E_s k^4)^p =
+    ssin - sdt - sdv + sice wherever that net input is positive, else 0."""
+    o, e0, _ = _after_source(O)
+    oc = o.get("oc")[: o.im]
+    f = o.get("f")
+    k4, coth = o.get("k4"), o.get("cothkd")
+    sdt, sdv, sice = o.get("sdt"), o.get("sdv"), o.get("sice")
+    ssin, dummy, ef = o.get("ssin"), o.get("dummy"), o.get("ef")
+    checked = 0
+    for i in range(o.im):
+        for q in range(int(oc[i]), o.om):  # 0-based q > oc - 1 are tail bins
+            num = ssin[i, :, q] - sdt[i, q] - sdv[i, q] + sice[i, q]
+            lhs = 2 * math.pi * 42.0 * f[q] * dummy[i, :, q] * coth[i, q] * (ef[i, :, q] * k4[i, q]) ** 2.4
+            pos = num >= 0
+            assert np.allclose(lhs[pos], num[pos], rtol=1e-10)
+            assert np.all(ef[i, ~pos, q] == 0)
+            checked += int(pos.sum())
+    assert checked > 100
