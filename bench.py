@@ -12,6 +12,7 @@ Prints one JSON line.  --gpus N with N > 1 exits: the build targets one GPU.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -26,8 +27,52 @@ METRIC = "spectral-bin updates/s (source+advect+refract), chosen vs plain at Hs 
 HBM_PEAK_GBS = 8000.0
 
 
+KERNELS = ("k_rates", "k_integrate", "k_advect", "k_stress")
+# algorithmic HBM words per bin-update of each fused kernel (one word = the
+# spectrum's precision): k_rates reads E^n; k_integrate reads E^n, writes E*;
+# k_advect reads E* and E^n, writes E^{n+1}; k_stress works on O(om) per point
+ALG_WORDS = {"k_rates": 1, "k_integrate": 2, "k_advect": 3, "k_stress": 0}
+PROFILE = os.path.join(ROOT, "profiles", "k_advect_c2_fp32.md")
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def profile_traffic():
+    """HBM bytes per k_advect launch on this workload, measured with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a 1 GiB copy) and
+    recorded in the committed profile summary; None when absent."""
+    try:
+        txt = open(PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def roofline(r, word, traffic):
+    """Roofline of the dominant kernel (the longest of the four): its
+    algorithmic bytes per launch over its mean HIP-event duration."""
+    kms = dict(zip(KERNELS, r["kms"]))
+    dom = max(kms, key=kms.get)
+    alg = ALG_WORDS[dom] * word * r["bins"]
+    ach = alg / (kms[dom] * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": traffic if dom == "k_advect" and word == 4 else None,
+            "alg_bytes_per_launch": alg, "ms": round(kms[dom], 4)}
+
+
+def roofline_substep(r, word):
+    """The whole substep against B_alg = 3 words per bin-update, the lower
+    bound for any implementation of the path (one read of E^n for the rates
+    and the global dts, one read to integrate and advect, one write)."""
+    t = sum(r["kms"]) * 1e-3
+    ach = 3 * word * r["bins"] / t / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "b_alg_bytes_per_bin": 3 * word,
+            "ms": round(t * 1e3, 4)}
 
 
 def make_domain(wl, precision):
@@ -165,7 +210,8 @@ def main():
                                       tune=not (args.no_tune or args.tile))
         tile = tile or (tuple(args.tile) if args.tile else None)
         rate = args.steps * bins / wall
-        log(f"bench: {tag} {rate / 1e9:.2f} G bin-updates/s, kernels ms {kms}")
+        log(f"bench: {tag} {rate / 1e9:.2f} G bin-updates/s, kernels ms "
+            + ", ".join(f"{k} {v:.3f}" for k, v in zip(KERNELS, kms)))
         if not args.no_finish:
             log(f"bench: {tag} finishing {args.hours} h")
             t0 = time.perf_counter()
@@ -179,10 +225,9 @@ def main():
     if args.only:
         r = res[args.only]
         word = 4 if args.only == "fp32" else 8
-        t = sum(r["kms"]) * 1e-3
         print(json.dumps({"path": args.only, "value": r["rate"], "unit": "bin-updates/s",
-                          "kernel_ms": r["kms"], "tile": r["tile"],
-                          "achieved_GBs": 3 * word * r["bins"] / t / 1e9}), flush=True)
+                          "kernel_ms": dict(zip(KERNELS, r["kms"])), "tile": r["tile"],
+                          "roofline": roofline(r, word, None)}), flush=True)
         return
 
     r32, r64 = res["fp32"], res["fp64"]
@@ -202,15 +247,6 @@ def main():
             / max(np.sqrt(np.mean(np.hypot(r64["us"], r64["vs"]) ** 2)), 1e-30)),
     }
 
-    # roofline of the fused substep: B_alg = 3 words per bin-update
-    def roof(r, word):
-        t = sum(r["kms"]) * 1e-3
-        ach = 3 * word * r["bins"] / t / 1e9
-        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "fused substep (k_rates + k_integrate + k_advect/k_stress)",
-                "kernel_ms": [round(x, 4) for x in r["kms"]], "tile": r["tile"]}
-
     out = {
         "metric": METRIC,
         "value": r32["rate"],
@@ -224,15 +260,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "c2: 720x360 global 0.5 deg, om=37 pm=36, GRF land/depth/winds, "
-                               f"JONSWAP start, {args.hours} h" if not args.small else "c2 small",
+        "config": {"workload": ("c2: 720x360 global 0.5 deg lat-lon, om=37 pm=36, GRF land 30%, "
+                                "log-normal depth, GRF winds 0-25 m/s, JONSWAP start, "
+                                f"{args.hours} h") if not args.small else "c2 quarter size",
                    "sea_points": r32["im"], "bins_per_step": r32["bins"],
                    "chosen": "fp32 fused", "plain": "fp64 fused"},
-        "plain": {"value": r64["rate"], "dtype": "f64", "ms_per_step": 1e3 * r64["wall"] / args.steps},
+        "kernel_ms": dict(zip(KERNELS, [round(x, 4) for x in r32["kms"]])),
+        "tile": r32["tile"],
+        "plain": {"value": r64["rate"], "dtype": "f64",
+                  "ms_per_step": 1e3 * r64["wall"] / args.steps,
+                  "kernel_ms": dict(zip(KERNELS, [round(x, 4) for x in r64["kms"]])),
+                  "tile": r64["tile"], "roofline": roofline(r64, 8, None)},
         "chosen_vs_plain": r32["rate"] / r64["rate"],
         "errors_vs_fp64": errs,
-        "roofline": roof(r32, 4),
-        "roofline_plain": roof(r64, 8),
+        "roofline": roofline(r32, 4, profile_traffic()),
+        "roofline_substep": roofline_substep(r32, 4),
     }
     if not args.no_cpu:
         log("bench: cpu baseline")

@@ -188,7 +188,7 @@ int umwm_hip_end_first(umwm_domain* dom);
 /*
  * One substep: forcinginterpolate, sin_d12, sds_d12, snl_d12, s_ice,
  * source, propagation, refraction, e = ef, stress('atm'), and on the first
- * substep diag and first = .false. (src/umwm_top.F90:91-141), in three
+ * substep diag and first = .false. (src/umwm_top.F90:91-141), in four
  * fused kernels.  Returns the new sumt and the dts taken.
  */
 int umwm_hip_substep(umwm_domain* dom, double* sumt, double* dts);
@@ -232,10 +232,10 @@ int64_t umwm_hip_field_size(const umwm_domain* dom, const char* name);
 /* ---------------------------------------------------------------------- */
 /* Enables per-kernel event timing of the fused path (off by default). */
 int umwm_hip_set_timing(umwm_domain* dom, int enable);
-/* Accumulated milliseconds and launch counts of the three fused kernels
- * since the last reset: ms[0..2] = rates, integrate, advect+stress;
- * counts[0..2] likewise.  Resets the accumulators when reset != 0. */
-int umwm_hip_get_timing(umwm_domain* dom, double ms[3], int64_t counts[3],
+/* Accumulated milliseconds and launch counts of the four fused kernels
+ * since the last reset: ms[0..3] = rates, integrate, advect, stress;
+ * counts[0..3] likewise.  Resets the accumulators when reset != 0. */
+int umwm_hip_get_timing(umwm_domain* dom, double ms[4], int64_t counts[4],
                         int reset);
 /* Waits for all work queued on the domain's stream. */
 int umwm_hip_synchronize(umwm_domain* dom);

@@ -29,7 +29,7 @@ def candidates(pm, ds):
 def autotune(dom, steps=3, verbose=None):
     """Chooses (D1, W1, D2, W2, NB) for `dom` by timing live substeps;
     returns the chosen tuple and the per-candidate mean kernel times (ms:
-    rates, integrate, advect + stress)."""
+    rates, integrate, advect, stress)."""
     pm = dom.pm
     c1 = candidates(pm, RATES_D)
     c2 = candidates(pm, INTEGRATE_D)

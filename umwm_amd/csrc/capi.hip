@@ -525,8 +525,8 @@ int fused_substep(DomainImpl<T>& D) {
     D.first = false;
   }
   if (D.timing) {
-    UMWM_HIP_TRY(hipEventSynchronize(D.ev[3]));
-    for (int q = 0; q < 3; ++q) {
+    UMWM_HIP_TRY(hipEventSynchronize(D.ev[4]));
+    for (int q = 0; q < 4; ++q) {
       float ms = 0;
       UMWM_HIP_TRY(hipEventElapsedTime(&ms, D.ev[q], D.ev[q + 1]));
       D.ms[q] += ms;
@@ -1118,9 +1118,9 @@ int umwm_hip_set_timing(umwm_domain* dom, int enable) {
   });
 }
 
-int umwm_hip_get_timing(umwm_domain* dom, double ms[3], int64_t counts[3], int reset) {
+int umwm_hip_get_timing(umwm_domain* dom, double ms[4], int64_t counts[4], int reset) {
   return dispatch(dom, [&](auto& D) {
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < 4; ++q) {
       if (ms) ms[q] = D.ms[q];
       if (counts) counts[q] = D.launches[q];
       if (reset) {

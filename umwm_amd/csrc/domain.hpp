@@ -59,8 +59,8 @@ struct DomainBase {
   // the fused substep does not store E_s: after it, ef equals e (as after
   // the reference's e = ef) and reads of "ef" are served from e
   bool ef_mirrors_e = false;
-  double ms[3] = {0, 0, 0};
-  int64_t launches[3] = {0, 0, 0};
+  double ms[4] = {0, 0, 0, 0};          // rates, integrate, advect, stress
+  int64_t launches[4] = {0, 0, 0, 0};
 };
 
 template <typename T>
@@ -77,7 +77,7 @@ struct DomainImpl : DomainBase {
   Scalars* sc_host = nullptr;  // pinned mirror
   bool ops_ready = false;      // operator-path arrays allocated
   bool rot_valid = false;      // rotation computed at least once
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   // forcing device buffers (level n and n+1)
   T *uwb = nullptr, *uwf = nullptr, *vwb = nullptr, *vwf = nullptr;
   T *ucb = nullptr, *ucf = nullptr, *vcb = nullptr, *vcf = nullptr;

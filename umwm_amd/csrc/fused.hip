@@ -899,8 +899,9 @@ int launch_all(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int
     hipLaunchKernelGGL((k_advect<T, 4>), ga, dim3(64 * fw), 0, s, doms, tasks);
   else
     hipLaunchKernelGGL((k_advect<T, 2>), ga, dim3(64 * fw), 0, s, doms, tasks);
-  hipLaunchKernelGGL((k_stress<T>), grid, dim3(64), 0, s, doms, tasks);
   if (ev) (void)hipEventRecord(ev[3], s);
+  hipLaunchKernelGGL((k_stress<T>), grid, dim3(64), 0, s, doms, tasks);
+  if (ev) (void)hipEventRecord(ev[4], s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(UMWM_ERR_HIP, std::string("fused launch: ") + hipGetErrorString(e));
   return 0;

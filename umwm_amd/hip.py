@@ -254,8 +254,8 @@ class Domain:
         check(lib().umwm_hip_set_timing(self.h, int(on)))
 
     def timing(self, reset=False):
-        ms = (C.c_double * 3)()
-        cnt = (C.c_int64 * 3)()
+        ms = (C.c_double * 4)()
+        cnt = (C.c_int64 * 4)()
         check(lib().umwm_hip_get_timing(self.h, ms, cnt, int(reset)))
         return list(ms), list(cnt)
 
