@@ -214,6 +214,7 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   h.sds_power = P.sds_power; h.inv_sds_power = S.inv_sds_power; h.mss_fac = P.mss_fac;
   h.snl_fac = P.snl_fac; h.sdt_fac = P.sdt_fac; h.z = P.z; h.nu_air = P.nu_air;
   h.kappa = P.kappa; h.fice_lth = P.fice_lth; h.fice_uth = P.fice_uth;
+  h.nu_water = P.nu_water;
   h.tiny = sizeof(T) == 8 ? DBL_MIN : (double)FLT_MIN;
 
   const int om = S.om, pm = S.pm, ldi = S.ldi, ldq = S.ldq;
@@ -287,9 +288,8 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   h.pl = pl; h.pr = pr;
   // per-(frequency, point) table records for the fused kernels, [o][ldi][FT_N]
   {
-    const std::vector<double>* ft[FT_PAD] = {&S.cp0, &S.logl2overz, &S.fkovg, &S.k4, &S.cothkd,
-                                             &S.sbf, &S.sdv, &S.snl_arg, &S.k, &S.cg0, &S.k3dk,
-                                             &S.kdk, &S.bf1r, &S.bf2r, &S.oneoverk4};
+    const std::vector<double>* ft[FT_PAD] = {&S.cp0, &S.logl2overz, &S.cothkd, &S.sbf, &S.k,
+                                             &S.cg0, &S.bf1r, &S.bf2r, &S.kdk};
     std::vector<double> blk((size_t)FT_N * nt, 0.0);
     for (size_t q = 0; q < nt; ++q)
       for (int t = 0; t < FT_PAD; ++t) blk[q * FT_N + t] = (*ft[t])[q];
@@ -387,13 +387,13 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   h.part_min = A.template alloc<double>(std::max(h.nblocks, 1));
   h.part_max = A.template alloc<double>(std::max(h.nblocks, 1));
   h.pfx = A.template alloc<T>(4 * nt);
-  h.spart = A.template alloc<double>(4 * nt);
+  h.spart = A.template alloc<T>(4 * nt);
   h.sc = A.template alloc<Scalars>(1);
   if (!h.sc || !h.pfx || !h.spart) {
     delete D;
     return fail(UMWM_ERR_NOMEM, "device allocation failed");
   }
-  if ((rc = dfill<T>(h.pfx, 4 * nt, T(0), s)) || (rc = dfill<double>(h.spart, 4 * nt, 0.0, s))) {
+  if ((rc = dfill<T>(h.pfx, 4 * nt, T(0), s)) || (rc = dfill<T>(h.spart, 4 * nt, T(0), s))) {
     delete D;
     return rc;
   }

@@ -57,7 +57,7 @@ struct DomainDev {
   // namelist and derived constants (double; kernels cast to T)
   double g, explim, fprog, dtg, dtamin, dth, dthg, oneovdth, twopisds_fac;
   double sin_fac, fieldscale1, fieldscale2, sds_power, inv_sds_power;
-  double mss_fac, snl_fac, sdt_fac, z, nu_air, kappa, fice_lth, fice_uth;
+  double mss_fac, snl_fac, sdt_fac, z, nu_air, kappa, fice_lth, fice_uth, nu_water;
   double tiny;  // tiny(e) of the working precision
   // spectra
   T *e, *ef, *es;
@@ -91,22 +91,25 @@ struct DomainDev {
   // few 16-byte loads and a wave reads one contiguous span; the exclusive
   // frequency prefix of the cos^2 directional convolution times dth/2,
   // [om][ldi][4] (3 used, written by k_rates); the per-frequency stress /
-  // wave-height partial sums, [om][ldi][4] (3 used)
+  // wave-height partial sums, [om][ldi][4] (3 used; summed over frequency
+  // in FP64 by k_stress)
   T* ftab;
   T* pfx;
-  double* spart;
+  T* spart;
   // reduction scratch
   double *part_min, *part_max;
   Scalars* sc;
 };
 
-// Slots of one (frequency, point) record of DomainDev::ftab.  The first
-// twelve are what the rates kernel reads.
+// Slots of one (frequency, point) record of DomainDev::ftab: the tables the
+// sweep kernels cannot rebuild from k cheaply (k^4, 1/k^4, sdv, f k/g,
+// snl_arg and k^3 dk are formed from k, kdk, bf1r and bf2r on the fly).
+// The integrate kernel reads the first eight, the rates kernel all nine.
 enum FusedTable {
-  FT_CP0, FT_LOGL2, FT_FKOVG, FT_K4, FT_COTH, FT_SBF, FT_SDV, FT_SARG, FT_K, FT_CG0, FT_K3DK,
-  FT_KDK, FT_B1, FT_B2, FT_OK4, FT_PAD, FT_N
+  FT_CP0, FT_LOGL2, FT_COTH, FT_SBF, FT_K, FT_CG0, FT_B1, FT_B2, FT_KDK, FT_PAD, FT_N = 12
 };
 constexpr int kRatesSlots = 12;
+constexpr int kIntegrateSlots = 8;
 
 // One workgroup of a fused launch: which domain, which 64 sea points.
 struct BlockTask {

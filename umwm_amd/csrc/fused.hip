@@ -298,6 +298,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     }
     const T sin_diss1 = T(d.sin_fac * d.fieldscale1), fs2 = T(d.fieldscale2);
     const T tpsf = T(d.twopisds_fac), spow = T(d.sds_power), mssf = T(d.mss_fac);
+    const T nu4 = T(4 * d.nu_water), ig = T(1 / d.g);
     const double hdth = 0.5 * d.dth;
     // overrun factor per direction of this wave (frequency independent)
     T mq[DMAX];
@@ -333,10 +334,11 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       else
         load_tab<kRatesSlots>(d, o, vr, tc);
 
-      const T sdt = s.csdt * tc[FT_K];
+      const T k = tc[FT_K], k2 = k * k, fo = cld(d.f, o);
+      const T sdt = s.csdt * k;
       const T sice = T(2) * tc[FT_CG0] * s.ice;
-      const T crest = tc[FT_SBF] + sdt + tc[FT_SDV] - sice;
-      const SinFreq<T> sf = sin_freq(s, tc[FT_LOGL2], tc[FT_CP0], tc[FT_FKOVG], sin_diss1, fs2);
+      const T crest = tc[FT_SBF] + sdt + nu4 * k2 - sice;  // sbf + sdt + sdv - sice
+      const SinFreq<T> sf = sin_freq(s, tc[FT_LOGL2], tc[FT_CP0], fo * k * ig, sin_diss1, fs2);
       const T pa0 = T(hdth * A0), pac = T(hdth * AC), pas = T(hdth * AS);
       if (valid && w == 0) {
         if constexpr (sizeof(T) == 4) {
@@ -349,9 +351,9 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
         }
       }
       // sds (after cothkd) = cf coth dm^2 (e k4)^p = dm^2 ex(p lg e + Lk)
-      const T cfo = tpsf * cld(d.f, o) * tc[FT_COTH];
-      const T Lk = spow * Tr<T>::lg(tc[FT_K4]) + Tr<T>::lg(cfo);
-      const T cs = tc[FT_SARG];
+      const T cfo = tpsf * fo * tc[FT_COTH];
+      const T Lk = T(4) * spow * Tr<T>::lg(k) + Tr<T>::lg(cfo);  // lg(k^4) = 4 lg k
+      const T cs = T(1) - (tc[FT_B1] + tc[FT_B2]);                // snl_arg
       const T q0 = T(1) + mssf * pa0, qc = mssf * pac, qs = mssf * pas;
       T m0 = 0, mc = 0, ms = 0, rq = 0;
 #pragma unroll
@@ -402,7 +404,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
           s2s += b[(2 * W + q) * 64 + lane];
         }
       }
-      const double k3 = tc[FT_K3DK];
+      const double k3 = k2 * tc[FT_KDK];  // k^3 dk
       A0 += k3 * s0;
       AC += k3 * s2c;
       AS += k3 * s2s;
@@ -539,6 +541,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   const T sin_diss1 = T(d.sin_fac * d.fieldscale1), fs2 = T(d.fieldscale2);
   const T tpsf = T(d.twopisds_fac), spow = T(d.sds_power), ipow = T(d.inv_sds_power);
   const T mssf = T(d.mss_fac), dsnl = dts * T(d.snl_fac);
+  const T nu4 = T(4 * d.nu_water), ig = T(1 / d.g);
   T mq[DMAX];
 #pragma unroll
   for (int q = 0; q < DMAX; ++q) {
@@ -561,7 +564,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   Prefix<T> pc{};
   constexpr bool kPrefetchTab = sizeof(T) == 4;  // as in k_rates
   if constexpr (kPrefetchTab) {
-    load_tab<FT_N>(d, 0, vr, tc);
+    load_tab<kIntegrateSlots>(d, 0, vr, tc);
     pc = load_prefix(d, 0, j);
   }
   for (int o = 0; o < om; ++o) {
@@ -573,21 +576,23 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     FreqTab<T> tn;
     Prefix<T> pn{};
     if constexpr (kPrefetchTab) {
-      load_tab<FT_N>(d, on, vr, tn);
+      load_tab<kIntegrateSlots>(d, on, vr, tn);
       pn = load_prefix(d, on, j);
     } else {
-      load_tab<FT_N>(d, o, vr, tc);
+      load_tab<kIntegrateSlots>(d, o, vr, tc);
       pc = load_prefix(d, o, j);
     }
 
-    const T sdt = s.csdt * tc[FT_K];
+    const T k = tc[FT_K], fo = cld(d.f, o);
+    const T sdt = s.csdt * k;
     const T sice = T(2) * tc[FT_CG0] * s.ice;
-    const T crest = tc[FT_SBF] + sdt + tc[FT_SDV] - sice;
-    const T ctail = sdt + tc[FT_SDV] - sice;
-    const SinFreq<T> sf = sin_freq(s, tc[FT_LOGL2], tc[FT_CP0], tc[FT_FKOVG], sin_diss1, fs2);
-    const T cf = tpsf * cld(d.f, o);
-    const T coth = tc[FT_COTH], ok4 = tc[FT_OK4];
-    const T Lk = spow * Tr<T>::lg(tc[FT_K4]) + Tr<T>::lg(cf);  // sds before cothkd
+    const T ctail = sdt + nu4 * (k * k) - sice;  // sdt + sdv - sice
+    const T crest = tc[FT_SBF] + ctail;
+    const SinFreq<T> sf = sin_freq(s, tc[FT_LOGL2], tc[FT_CP0], fo * k * ig, sin_diss1, fs2);
+    const T cf = tpsf * fo;
+    const T coth = tc[FT_COTH];
+    const T lk4 = T(4) * Tr<T>::lg(k);        // lg(k^4)
+    const T Lk = spow * lk4 + Tr<T>::lg(cf);  // sds before cothkd
     const T c0 = -dtx * crest, dtc = dtx * coth;
     const T icfc = Tr<T>::rcp(cf * coth);
     const T q0 = T(1) + mssf * pc.a0, qc = mssf * pc.ac, qs = mssf * pc.as;
@@ -624,7 +629,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
             if (!prog) {
               const T num = fmax(ss, T(0)) - ctail;
               const T t = fmax(num, T(0)) * (icfc * Tr<T>::rcp(dm));
-              es_ = ok4 * Tr<T>::ex(ipow * Tr<T>::lg(t));
+              es_ = Tr<T>::ex(fma(ipow, Tr<T>::lg(t), -lk4));  // k^-4 t^(1/p)
             }
             bst(rs0, vtail, T(0.5) * (e + es_), p * rb);
           }
@@ -822,11 +827,15 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
   }
   if (valid) {
     const int jc = (j == 0) ? d.im : j - 1;  // cp0(o, i-1), src/umwm_stress.F90:219,316
-    const double icp = 1.0 / (double)bld<T>(cpr, (uint32_t)jc * E);
-    const double kdk = bld<T>(row_rsrc(d.kdk, o, ldi), vj);
+    const T icp = T(1) / bld<T>(cpr, (uint32_t)jc * E);
+    const T kdk = bld<T>(row_rsrc(d.kdk, o, ldi), vj);
     const rsrc_t sp = freq_rsrc(d.spart, o, 4, ldi);
-    bst16(sp, (uint32_t)j * 32u, f64x2{(double)tx * icp, (double)ty * icp});
-    bst(sp, (uint32_t)j * 32u + 16u, (double)hs * kdk);
+    if constexpr (sizeof(T) == 4) {
+      bst16(sp, (uint32_t)j * 16u, f32x4{tx * icp, ty * icp, hs * kdk, T(0)});
+    } else {
+      bst16(sp, (uint32_t)j * 32u, f64x2{tx * icp, ty * icp});
+      bst(sp, (uint32_t)j * 32u + 16u, hs * kdk);
+    }
   }
 }
 
@@ -853,8 +862,18 @@ __global__ __launch_bounds__(64) void k_stress(const DomainDev<T>* __restrict__ 
   for (int o = 0; o < om; ++o) {
     const double kdk = bld<T>(row_rsrc(d.kdk, o, ldi), (uint32_t)j * E);
     const rsrc_t sp = freq_rsrc(d.spart, o, 4, ldi);
-    const f64x2 txy = bld16<double>(sp, (uint32_t)j * 32u);
-    const double tx = txy[0], ty = txy[1];
+    double tx, ty, hso;
+    if constexpr (sizeof(T) == 4) {
+      const f32x4 v = bld16<float>(sp, (uint32_t)j * 16u);
+      tx = v[0];
+      ty = v[1];
+      hso = v[2];
+    } else {
+      const f64x2 v = bld16<double>(sp, (uint32_t)j * 32u);
+      tx = v[0];
+      ty = v[1];
+      hso = bld<double>(sp, (uint32_t)j * 32u + 16u);
+    }
     si.sx += tx * kdk;
     si.sy += ty * kdk;
     if (o >= oc) {
@@ -865,7 +884,7 @@ __global__ __launch_bounds__(64) void k_stress(const DomainDev<T>* __restrict__ 
       si.tomx = tx;
       si.tomy = ty;
     }
-    hs += bld<double>(sp, (uint32_t)j * 32u + 16u);
+    hs += hso;
   }
   si.wspd = w;
   si.wdir = pload(d.wdir, ldq, j);
