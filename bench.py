@@ -159,6 +159,65 @@ def cpu_baseline(wl_full, seconds_budget=20.0):
                       f"{threads} OpenMP threads"}
 
 
+def make_member(wl, precision):
+    d = hip.Domain(wl.params, wl.grid_input(), wl.forcing_input(0), precision=precision)
+    d.init_jonswap(wl.jonswap)
+    d.stokes_drift(True)
+    return d
+
+
+def batched_leg(n, seed=3, precision=4, sequential=True, scale=1.0):
+    """C3: n uneven regional domains (100x100..600x400, om 25-49, pm 24-72)
+    advanced together, one launch per kernel per substep replayed as a
+    hipGraph; the first hour (cold substep, graph capture) is untimed and the
+    second is timed.  With `sequential`, the same members are then advanced
+    one domain at a time for the same hour (each with the batch's tile) to
+    show what batching gains."""
+    t0 = time.perf_counter()
+    wls = configs.c3(seed=seed, n=n, hours=2, scale=scale)
+    doms = [make_member(wl, precision) for wl in wls]
+    setup = time.perf_counter() - t0
+    bins = [d.im * d.om * d.pm for d in doms]
+    log(f"bench: c3 {n} members, {sum(bins) / 1e9:.2f} G bins per substep set, set-up {setup:.1f} s")
+    # autotune on a probe built like the largest member
+    big = int(np.argmax(bins))
+    probe = make_member(wls[big], precision)
+    probe.substep()
+    tile, _ = autotune.autotune(probe, steps=2, verbose=log)
+    probe.close()
+    b = hip.Batch(doms)
+    b.set_tile(*tile)
+    b.run_dtg(use_graph=True)
+    t0 = time.perf_counter()
+    nsub = b.run_dtg(use_graph=True)
+    wall = time.perf_counter() - t0
+    upd = sum(n_ * b_ for n_, b_ in zip(nsub, bins))
+    out = {"value": upd / wall, "unit": "bin-updates/s", "members": n, "grid_scale": scale,
+           "bins_per_member_substep": [int(x) for x in bins], "substeps": nsub,
+           "wall_s": wall, "tile": tile, "graph": True, "dtype": "f32" if precision == 4 else "f64"}
+    log(f"bench: c3 batched {upd / wall / 1e9:.2f} G bin-updates/s over {wall:.2f} s")
+    b.close()
+    for d in doms:
+        d.close()
+    if sequential:
+        doms = [make_member(wl, precision) for wl in wls]
+        for d in doms:
+            d.set_tile(*tile)
+            d.run_dtg()
+        for d in doms:
+            d.synchronize()
+        t0 = time.perf_counter()
+        nseq = [d.run_dtg() for d in doms]
+        wall_s = time.perf_counter() - t0
+        upd_s = sum(n_ * b_ for n_, b_ in zip(nseq, bins))
+        out["sequential"] = {"value": upd_s / wall_s, "wall_s": wall_s, "substeps": nseq}
+        out["batched_vs_sequential"] = (upd / wall) / (upd_s / wall_s)
+        log(f"bench: c3 one domain at a time {upd_s / wall_s / 1e9:.2f} G bin-updates/s")
+        for d in doms:
+            d.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +234,8 @@ def main():
                     help="fixed tile shape (implies --no-tune)")
     ap.add_argument("--no-finish", action="store_true",
                     help="skip finishing the simulated hours (profiling runs)")
+    ap.add_argument("--c3-members", type=int, default=64,
+                    help="members of the batched ensemble leg (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device-to-device copy first (scales HBM counters)")
     args = ap.parse_args()
@@ -276,6 +337,13 @@ def main():
         "roofline": roofline(r32, 4, profile_traffic()),
         "roofline_substep": roofline_substep(r32, 4),
     }
+    if args.c3_members > 0:
+        log("bench: batched ensemble leg")
+        out["batched_c3"] = batched_leg(args.c3_members)
+        # the same draw with grids shrunk 4x per side: members too small to
+        # fill the chip alone, where one launch for all of them pays
+        log("bench: batched ensemble leg, small members")
+        out["batched_c3_small"] = batched_leg(args.c3_members, scale=0.25)
     if not args.no_cpu:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(wl)

@@ -257,13 +257,15 @@ int umwm_hip_batch_create(umwm_domain* const* doms, int32_t n,
                           umwm_batch** out);
 int umwm_hip_batch_destroy(umwm_batch* b);
 /* Inner loop of every member to its own sumt >= dtg, replaying one
- * hipGraph per substep when use_graph != 0.  nsub[k] receives member k's
- * substep count (array of n, may be NULL). */
+ * hipGraph per substep when use_graph != 0 (one launch per kernel per pm
+ * class of members).  nsub[k] receives member k's substep count (array of
+ * n, may be NULL). */
 int umwm_hip_batch_run_dtg(umwm_batch* b, int use_graph, int32_t* nsub);
 /* Tile shape of the batch's fused launches (layout of umwm_hip_set_tile). */
 int umwm_hip_batch_tile(const umwm_batch* b, int32_t out[5]);
 /* Sets the batch's tile shape (same rules as umwm_hip_set_tile over the
- * largest pm of the batch). */
+ * largest pm of the batch).  Members are launched in groups of equal pm;
+ * each group uses the tile's D1, D2 and NB with W = ceil(pm / D). */
 int umwm_hip_batch_set_tile(umwm_batch* b, const int32_t tile[5]);
 
 #ifdef __cplusplus

@@ -778,12 +778,18 @@ int set_field_impl(DomainImpl<T>& D, const std::string& name, const double* buf,
 // ---------------------------------------------------------------------
 template <typename T>
 struct BatchImpl {
+  // members with the same pm share one task list and launch shape: W waves
+  // of D directions each, W = ceil(pm / D) for the batch's D
+  struct Group {
+    int pm = 0, omax = 0, ntasks = 0;
+    BlockTask* tasks = nullptr;
+    Tile tile;
+  };
   std::vector<DomainImpl<T>*> doms;
   DeviceArena arena;
   DomainDev<T>* d_doms = nullptr;
-  BlockTask* tasks = nullptr;
-  int ntasks = 0;
-  Tile tile;
+  std::vector<Group> groups;
+  Tile tile;  // as set (D1, D2, NB are what the groups share)
   hipStream_t stream = nullptr;
   Scalars** d_scs = nullptr;
   ScalarsView* d_view = nullptr;
@@ -1179,37 +1185,52 @@ int batch_create_impl(umwm_domain* const* doms, int n, BatchImpl<T>*& out) {
     delete B;
     return fail(UMWM_ERR_HIP, "hipStreamCreate failed");
   }
-  // domain structs and concatenated task list, largest domains first
+  // domain structs; one task list per pm class, largest domains first and
+  // XCD-interleaved within the class
   std::vector<DomainDev<T>> hd(n);
+  for (int k = 0; k < n; ++k) hd[k] = B->doms[k]->h;
   std::vector<int> order(n);
   for (int k = 0; k < n; ++k) order[k] = k;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
     return (size_t)B->doms[a]->S.im * B->doms[a]->S.om * B->doms[a]->S.pm >
            (size_t)B->doms[b]->S.im * B->doms[b]->S.om * B->doms[b]->S.pm;
   });
-  std::vector<BlockTask> tk;
-  for (int k : order) {
-    hd[k] = B->doms[k]->h;
-    const int nb = hd[k].nblocks;
-    for (int b = 0; b < nb; ++b) tk.push_back(BlockTask{k, b * kLanes, b, 0});
-  }
-  // XCD-aware interleave over the whole list
-  std::vector<int> ord = xcd_order((int)tk.size());
-  std::vector<BlockTask> tk2(tk.size());
-  for (size_t b = 0; b < tk.size(); ++b) tk2[b] = tk[ord[b]];
-  B->ntasks = (int)tk2.size();
+  std::vector<int> pms;
+  for (int k : order)
+    if (std::find(pms.begin(), pms.end(), B->doms[k]->S.pm) == pms.end())
+      pms.push_back(B->doms[k]->S.pm);
   B->d_doms = B->arena.template alloc<DomainDev<T>>(n);
-  B->tasks = B->arena.template alloc<BlockTask>(std::max(B->ntasks, 1));
   B->d_scs = B->arena.template alloc<Scalars*>(n);
   B->d_view = B->arena.template alloc<ScalarsView>(n);
   if (!B->d_view) {
     delete B;
     return fail(UMWM_ERR_NOMEM, "batch allocation failed");
   }
+  for (int pm : pms) {
+    typename BatchImpl<T>::Group g;
+    g.pm = pm;
+    std::vector<BlockTask> tk;
+    for (int k : order) {
+      if (B->doms[k]->S.pm != pm) continue;
+      g.omax = std::max(g.omax, B->doms[k]->S.om);
+      for (int b = 0; b < hd[k].nblocks; ++b) tk.push_back(BlockTask{k, b * kLanes, b, 0});
+    }
+    std::vector<int> ord = xcd_order((int)tk.size());
+    std::vector<BlockTask> tk2(tk.size());
+    for (size_t b = 0; b < tk.size(); ++b) tk2[b] = tk[ord[b]];
+    g.ntasks = (int)tk2.size();
+    g.tasks = B->arena.template alloc<BlockTask>(std::max(g.ntasks, 1));
+    if (!g.tasks) {
+      delete B;
+      return fail(UMWM_ERR_NOMEM, "batch allocation failed");
+    }
+    UMWM_HIP_TRY(hipMemcpy(g.tasks, tk2.data(), tk2.size() * sizeof(BlockTask), hipMemcpyHostToDevice));
+    g.tile = choose_tile(pm);
+    B->groups.push_back(g);
+  }
   std::vector<Scalars*> scs(n);
   for (int k = 0; k < n; ++k) scs[k] = hd[k].sc;
   UMWM_HIP_TRY(hipMemcpy(B->d_doms, hd.data(), n * sizeof(DomainDev<T>), hipMemcpyHostToDevice));
-  UMWM_HIP_TRY(hipMemcpy(B->tasks, tk2.data(), tk2.size() * sizeof(BlockTask), hipMemcpyHostToDevice));
   UMWM_HIP_TRY(hipMemcpy(B->d_scs, scs.data(), n * sizeof(Scalars*), hipMemcpyHostToDevice));
   UMWM_HIP_TRY(hipHostMalloc((void**)&B->h_view, n * sizeof(ScalarsView), hipHostMallocDefault));
   out = B;
@@ -1232,9 +1253,10 @@ int batch_launch_substep(BatchImpl<T>& B, int first, int rotation) {
   for (auto* D : B.doms)
     if (any_forcing(*D))
       if (int rc = on_stream(*D, B.stream, launch_forcinginterpolate<T>)) return rc;
-  int rc = launch_fused<T>(B.d_doms, B.tasks, B.ntasks, B.omax, first, rotation, B.tile,
-                           B.stream, nullptr);
-  if (rc) return rc;
+  for (const auto& g : B.groups)
+    if (int rc = launch_fused<T>(B.d_doms, g.tasks, g.ntasks, g.omax, first, rotation, g.tile,
+                                 B.stream, nullptr))
+      return rc;
   for (auto* D : B.doms) D->ef_mirrors_e = true;
   const int n = (int)B.doms.size();
   hipLaunchKernelGGL(k_gather_scalars, dim3((n + 63) / 64), dim3(64), 0, B.stream,
@@ -1368,6 +1390,13 @@ int umwm_hip_batch_set_tile(umwm_batch* b, const int32_t tile[5]) {
     Tile t = tile_from(tile, pmmax);
     if (!tile_ok(t, pmmax)) return fail(UMWM_ERR_ARGUMENT, "tile does not cover pm");
     B->tile = t;
+    for (auto& g : B->groups) {
+      Tile gt = t;  // same D and NB, waves for this class's pm
+      gt.W1 = (g.pm + t.D1 - 1) / t.D1;
+      gt.W2 = (g.pm + t.D2 - 1) / t.D2;
+      gt.pmax = g.pm;
+      g.tile = tile_ok(gt, g.pm) ? gt : choose_tile(g.pm);
+    }
     if (B->exec) {  // the captured graph holds the old launch shape
       (void)hipGraphExecDestroy(B->exec);
       (void)hipGraphDestroy(B->graph);
