@@ -218,6 +218,38 @@ def batched_leg(n, seed=3, precision=4, sequential=True, scale=1.0):
     return out
 
 
+def hurricane_leg(steps, warmup, member=0):
+    """C5 member: 1500x1200 at 2 km, om = 49, pm = 72, shelf and coastline,
+    translating Holland vortex.  FP32 and FP64 each take a cold substep, the
+    autotuner, `warmup` and `steps` timed substeps, then finish the first hour
+    so that their Hs and ustar can be compared at the same time."""
+    wl = configs.c5(member=member, hours=1)
+    out, fields = {"member": member, "note": wl.note}, {}
+    for prec, tag in ((4, "fp32"), (8, "fp64")):
+        t0 = time.perf_counter()
+        d = make_domain(wl, prec)
+        log(f"bench: c5 {tag} created in {time.perf_counter() - t0:.1f} s, "
+            f"{d.im} sea points x {d.om}x{d.pm}")
+        bins = d.im * d.om * d.pm
+        wall, kms, tile = timed_steps(d, warmup, steps)
+        rate = steps * bins / wall
+        log(f"bench: c5 {tag} {rate / 1e9:.2f} G bin-updates/s")
+        while d.scalar("sumt") < wl.params.dtg:
+            d.substep()
+        d.diag()
+        fields[tag] = (d.get("ht"), d.get("ustar"))
+        out[tag] = {"value": rate, "ms_per_step": 1e3 * wall / steps,
+                    "kernel_ms": dict(zip(KERNELS, [round(x, 4) for x in kms])), "tile": tile}
+        out["sea_points"], out["bins_per_step"] = d.im, bins
+        d.close()
+    (h32, u32), (h64, u64) = fields["fp32"], fields["fp64"]
+    out["fp32_vs_fp64"] = out["fp32"]["value"] / out["fp64"]["value"]
+    out["errors_vs_fp64_after_1h"] = {
+        "hs_rmse_rel": float(np.sqrt(np.mean((h32 - h64) ** 2)) / np.sqrt(np.mean(h64 ** 2))),
+        "ustar_max_rel": float(np.max(np.abs(u32 - u64) / np.maximum(np.abs(u64), 1e-3)))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +266,7 @@ def main():
                     help="fixed tile shape (implies --no-tune)")
     ap.add_argument("--no-finish", action="store_true",
                     help="skip finishing the simulated hours (profiling runs)")
+    ap.add_argument("--no-c5", action="store_true", help="skip the hurricane member leg")
     ap.add_argument("--c3-members", type=int, default=64,
                     help="members of the batched ensemble leg (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -344,6 +377,9 @@ def main():
         # fill the chip alone, where one launch for all of them pays
         log("bench: batched ensemble leg, small members")
         out["batched_c3_small"] = batched_leg(args.c3_members, scale=0.25)
+    if not args.no_c5:
+        log("bench: hurricane member leg")
+        out["hurricane_c5"] = hurricane_leg(10, 2)
     if not args.no_cpu:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(wl)
