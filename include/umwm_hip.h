@@ -174,7 +174,12 @@ int umwm_hip_propagation(umwm_domain* dom);
 int umwm_hip_refraction(umwm_domain* dom);
 /* e = ef (src/umwm_top.F90:113). */
 int umwm_hip_update(umwm_domain* dom);
-/* stress(option), option "atm" or "ocn" (src/umwm_stress.F90:17). */
+/* stress(option), option "atm" or "ocn" (src/umwm_stress.F90:17).  "ocn"
+ * reads ssin, sds and snl of the last source evaluation, as the reference
+ * does at output time, so that substep must have run through the operator
+ * entry points; after a fused substep it returns UMWM_ERR_STATE.  Outputs
+ * (per sea point): tailocnx/y, taux/tauy_ocntop, taux/tauy_ocnbot,
+ * taux/tauy_snl, epsx/epsy_ocn, epsx/epsy_atm, taux1..3, tauy1..3. */
 int umwm_hip_stress(umwm_domain* dom, const char* option);
 /* diag (src/umwm_physics.F90:89-205). */
 int umwm_hip_diag(umwm_domain* dom);

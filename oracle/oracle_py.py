@@ -52,7 +52,7 @@ def lib():
         L.oracle_get_scalar.argtypes = [C.c_void_p, C.c_char_p]
         L.oracle_set_scalar.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
         for fn in ("forcinginterpolate", "sin_d12", "sds_d12", "snl_d12", "s_ice",
-                   "source", "propagation", "refraction", "update", "stress_atm",
+                   "source", "propagation", "refraction", "update", "stress_atm", "stress_ocn",
                    "diag", "substep"):
             getattr(L, "oracle_" + fn).argtypes = [C.c_void_p]
         L.oracle_forcinginput.argtypes = [C.c_void_p, P(abi.ForcingInput)]

@@ -29,14 +29,16 @@
  *            dtr bound); stokes_drift us/vs (textbook Stokes drift of one
  *            bin, deep and finite depth); source dts choice, E* = (e+E_s)/2
  *            and the diagnostic tail (breaking/input equilibrium);
- *            cold first substep.
+ *            cold first substep; stress('ocn') form-drag partitions
+ *            (sum to the form stress; zero bottom stress in deep water).
  *   partial  sin_d12 (signs, monotonicity, tail clamp; the magnitude of
  *            each factor only through its closed-form pieces).
  *   parity unpinned: the exponential integration of prognostic bins in
  *            source; refraction's rotation magnitude; the stress('atm')
  *            frequency/direction integral (incl. the cp0(o,i-1) quirk) and
- *            its tail; the Stokes e-folding depth; diag outputs other than
- *            Hs and Tm; forcinginterpolate.
+ *            its tail; stress('ocn') ocean-top, bottom, Snl-loss and energy
+ *            flux integrals; the Stokes e-folding depth; diag outputs other
+ *            than Hs and Tm; forcinginterpolate.
  */
 #include "umwm_oracle.h"
 
@@ -76,6 +78,8 @@ struct oracle_model {
   int *oc;
   double *taux, *tauy, *taux_form, *tauy_form, *taux_skin, *tauy_skin;
   double *taux_diag, *tauy_diag, *tailatmx, *tailatmy;
+  /* stress('ocn') outputs */
+  double *tailocnx, *tailocny, *taux_ocntop, *tauy_ocntop, *taux_ocnbot, *tauy_ocnbot, *taux_snl, *tauy_snl, *epsx_ocn, *epsy_ocn, *epsx_atm, *epsy_atm, *taux1, *tauy1, *taux2, *tauy2, *taux3, *tauy3;
   double *ht, *mwp, *mwd, *mss, *mwl, *momx, *momy, *cgmxx, *cgmxy, *cgmyy;
   double *dwd, *dwl, *dwp, *dcp0, *dcg0, *dcp, *dcg;
 
@@ -712,6 +716,7 @@ static const fdesc FIELDS[] = {
     FD(mss, 1), FD(mwl, 1), FD(momx, 1), FD(momy, 1), FD(cgmxx, 1),
     FD(cgmxy, 1), FD(cgmyy, 1), FD(dwd, 1), FD(dwl, 1), FD(dwp, 1),
     FD(dcp0, 1), FD(dcg0, 1), FD(dcp, 1), FD(dcg, 1), FD(ds, 1),
+    FD(tailocnx, 1), FD(tailocny, 1), FD(taux_ocntop, 1), FD(tauy_ocntop, 1), FD(taux_ocnbot, 1), FD(tauy_ocnbot, 1), FD(taux_snl, 1), FD(tauy_snl, 1), FD(epsx_ocn, 1), FD(epsy_ocn, 1), FD(epsx_atm, 1), FD(epsy_atm, 1), FD(taux1, 1), FD(tauy1, 1), FD(taux2, 1), FD(tauy2, 1), FD(taux3, 1), FD(tauy3, 1),
     FD(k, 2), FD(kdk, 2), FD(k3dk, 2), FD(k4, 2), FD(oneoverk4, 2),
     FD(fkovg, 2), FD(logl2overz, 2), FD(cothkd, 2), FD(invcp0, 2), FD(sbf, 2),
     FD(sdv, 2), FD(sdt, 2), FD(snl_arg, 2), {"bf1_renorm", offsetof(oracle_model, bf1r), 0, 2},
@@ -848,7 +853,7 @@ oracle_model* oracle_create(const umwm_params* P, const umwm_grid_input* g,
                     &m->tauy_skin, &m->taux_diag, &m->tauy_diag, &m->tailatmx, &m->tailatmy,
                     &m->ht, &m->mwp, &m->mwd, &m->mss, &m->mwl, &m->momx, &m->momy,
                     &m->cgmxx, &m->cgmxy, &m->cgmyy, &m->dwd, &m->dwl, &m->dwp, &m->dcp0,
-                    &m->dcg0, &m->dcp, &m->dcg, &m->ds};
+                    &m->dcg0, &m->dcp, &m->dcg, &m->ds, &m->tailocnx, &m->tailocny, &m->taux_ocntop, &m->tauy_ocntop, &m->taux_ocnbot, &m->tauy_ocnbot, &m->taux_snl, &m->tauy_snl, &m->epsx_ocn, &m->epsy_ocn, &m->epsx_atm, &m->epsy_atm, &m->taux1, &m->tauy1, &m->taux2, &m->tauy2, &m->taux3, &m->tauy3};
   for (size_t q = 0; q < sizeof(pp_) / sizeof(pp_[0]); ++q) *pp_[q] = ALLOCD(ns);
   m->oc = ALLOCI(ns);
   double** tb_[] = {&m->k, &m->kdk, &m->k3dk, &m->k4, &m->oneoverk4, &m->fkovg,
@@ -894,6 +899,7 @@ void oracle_free(oracle_model* m) {
       (void**)&m->taux, (void**)&m->tauy, (void**)&m->taux_form, (void**)&m->tauy_form,
       (void**)&m->taux_skin, (void**)&m->tauy_skin, (void**)&m->taux_diag,
       (void**)&m->tauy_diag, (void**)&m->tailatmx, (void**)&m->tailatmy,
+      (void**)&m->tailocnx, (void**)&m->tailocny, (void**)&m->taux_ocntop, (void**)&m->tauy_ocntop, (void**)&m->taux_ocnbot, (void**)&m->tauy_ocnbot, (void**)&m->taux_snl, (void**)&m->tauy_snl, (void**)&m->epsx_ocn, (void**)&m->epsy_ocn, (void**)&m->epsx_atm, (void**)&m->epsy_atm, (void**)&m->taux1, (void**)&m->tauy1, (void**)&m->taux2, (void**)&m->tauy2, (void**)&m->taux3, (void**)&m->tauy3,
       (void**)&m->ht, (void**)&m->mwp, (void**)&m->mwd, (void**)&m->mss,
       (void**)&m->mwl, (void**)&m->momx, (void**)&m->momy, (void**)&m->cgmxx,
       (void**)&m->cgmxy, (void**)&m->cgmyy, (void**)&m->dwd, (void**)&m->dwl,
@@ -1325,6 +1331,128 @@ void oracle_stress_atm(oracle_model* m) {
     m->tauy[i] = m->tauy_form[i] + m->tauy_skin[i];
     m->cd[i] = drag_coefficient(m->taux[i], m->tauy[i], rhoa, w);
     m->ustar[i] = sqrt(sqrt(m->taux[i] * m->taux[i] + m->tauy[i] * m->tauy[i]) / rhoa);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* src/umwm_stress.F90:34-214 stress('ocn'): momentum into the ocean top */
+/* and bottom, the Snl momentum loss, energy fluxes into the ocean and  */
+/* from the air, and the three form-drag partitions.  Uses the source  */
+/* arrays of the last source evaluation (sds after cothkd, snl, ssin)  */
+/* with the current e, as the reference does at output time.  The     */
+/* reference also overwrites dummy(o,p,i) with e ssin kdk / cp0 here;  */
+/* nothing reads that before sds_d12 rewrites it, so it is not kept.   */
+/* ------------------------------------------------------------------ */
+static void ocn_sum(const oracle_model* m, int i, const double* ux, const double* uy, double* sx,
+                    double* sy) {
+  *sx = 0;
+  *sy = 0;
+  for (int o = 1; o <= m->om; ++o) {
+    *sx += ux[o - 1] * TB(m->kdk, o, i);
+    *sy += uy[o - 1] * TB(m->kdk, o, i);
+  }
+}
+
+void oracle_stress_ocn(oracle_model* m) {
+  const int om = m->om, pm = m->pm, im = m->im;
+  const double bf1 = m->bf1, bf2 = m->bf2, dthg = m->dthg;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+  for (int i = 1; i <= im; ++i) {
+    double ux[512], uy[512]; /* om <= 512 */
+    double sx, sy;
+    const double tail = oracle_stress_tail(m->wspd[i], TB(m->k, om, i));
+    const double rw = m->rhow[i];
+    /* ocean top (:37-77) */
+    for (int o = 1; o <= om; ++o) ux[o - 1] = uy[o - 1] = 0;
+    for (int p = 1; p <= pm; ++p) {
+      for (int o = 1; o <= om; ++o) {
+        double v = SP(m->e, o, p, i) * (SP(m->sds, o, p, i) + TB(m->sdt, o, i) + TB(m->sdv, o, i));
+        ux[o - 1] = ux[o - 1] + v * m->cth[p - 1] * TB(m->invcp0, o, i);
+        uy[o - 1] = uy[o - 1] + v * m->sth[p - 1] * TB(m->invcp0, o, i);
+      }
+      for (int o = 3; o <= om; ++o) {
+        double v = SP(m->snl, o, p, i) * (bf1 * TB(m->cp0, o, i) * TB(m->invcp0, o - 1, i) +
+                                          bf2 * TB(m->cp0, o, i) * TB(m->invcp0, o - 2, i));
+        ux[o - 1] = ux[o - 1] - v * m->cth[p - 1] * TB(m->invcp0, o, i);
+        uy[o - 1] = uy[o - 1] - v * m->sth[p - 1] * TB(m->invcp0, o, i);
+      }
+    }
+    m->tailocnx[i] = ux[om - 1] * TB(m->k, om, i) * tail * rw * dthg;
+    m->tailocny[i] = uy[om - 1] * TB(m->k, om, i) * tail * rw * dthg;
+    ocn_sum(m, i, ux, uy, &sx, &sy);
+    m->taux_ocntop[i] = sx * rw * dthg + m->tailocnx[i] + m->taux_skin[i];
+    m->tauy_ocntop[i] = sy * rw * dthg + m->tailocny[i] + m->tauy_skin[i];
+    /* ocean bottom (:79-96) */
+    for (int o = 1; o <= om; ++o) ux[o - 1] = uy[o - 1] = 0;
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= om; ++o) {
+        ux[o - 1] = ux[o - 1] + SP(m->e, o, p, i) * TB(m->sbf, o, i) * m->cth[p - 1] * TB(m->invcp0, o, i);
+        uy[o - 1] = uy[o - 1] + SP(m->e, o, p, i) * TB(m->sbf, o, i) * m->sth[p - 1] * TB(m->invcp0, o, i);
+      }
+    ocn_sum(m, i, ux, uy, &sx, &sy);
+    m->taux_ocnbot[i] = sx * rw * dthg;
+    m->tauy_ocnbot[i] = sy * rw * dthg;
+    /* Snl momentum loss (:98-123) */
+    for (int o = 1; o <= om; ++o) ux[o - 1] = uy[o - 1] = 0;
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 3; o <= m->oc[i]; ++o) {
+        double v = SP(m->snl, o, p, i) * (bf1 * (1 - TB(m->cp0, o, i) * TB(m->invcp0, o - 1, i)) +
+                                          bf2 * (1 - TB(m->cp0, o, i) * TB(m->invcp0, o - 2, i)));
+        ux[o - 1] = ux[o - 1] + v * m->cth[p - 1] * TB(m->invcp0, o, i);
+        uy[o - 1] = uy[o - 1] + v * m->sth[p - 1] * TB(m->invcp0, o, i);
+      }
+    ocn_sum(m, i, ux, uy, &sx, &sy);
+    m->taux_snl[i] = sx * rw * dthg;
+    m->tauy_snl[i] = sy * rw * dthg;
+    /* energy flux into the ocean (:125-152) */
+    for (int o = 1; o <= om; ++o) ux[o - 1] = uy[o - 1] = 0;
+    for (int p = 1; p <= pm; ++p) {
+      for (int o = 1; o <= om; ++o) {
+        ux[o - 1] = ux[o - 1] + SP(m->e, o, p, i) * SP(m->sds, o, p, i) * m->cth[p - 1];
+        uy[o - 1] = uy[o - 1] + SP(m->e, o, p, i) * SP(m->sds, o, p, i) * m->sth[p - 1];
+      }
+      for (int o = 3; o <= om; ++o) {
+        double v = SP(m->snl, o, p, i) * (bf1 * TB(m->cp0, o, i) * TB(m->invcp0, o - 1, i) +
+                                          bf2 * TB(m->cp0, o, i) * TB(m->invcp0, o - 2, i));
+        ux[o - 1] = ux[o - 1] - v * m->cth[p - 1];
+        uy[o - 1] = uy[o - 1] - v * m->sth[p - 1];
+      }
+    }
+    ocn_sum(m, i, ux, uy, &sx, &sy);
+    m->epsx_ocn[i] = sx * rw * dthg;
+    m->epsy_ocn[i] = sy * rw * dthg;
+    /* energy flux from the air (:154-167) */
+    for (int o = 1; o <= om; ++o) ux[o - 1] = uy[o - 1] = 0;
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= om; ++o) {
+        ux[o - 1] = ux[o - 1] + SP(m->e, o, p, i) * SP(m->ssin, o, p, i) * m->cth[p - 1];
+        uy[o - 1] = uy[o - 1] + SP(m->e, o, p, i) * SP(m->ssin, o, p, i) * m->sth[p - 1];
+      }
+    ocn_sum(m, i, ux, uy, &sx, &sy);
+    m->epsx_atm[i] = sx * rw * dthg;
+    m->epsy_atm[i] = sy * rw * dthg;
+    /* form-drag partitions (:169-212) */
+    double t1x = 0, t1y = 0, t2x = 0, t2y = 0, t3x = 0, t3y = 0;
+    for (int p = 1; p <= pm; ++p)
+      for (int o = 1; o <= om; ++o) {
+        double dm = SP(m->e, o, p, i) * SP(m->ssin, o, p, i) * TB(m->invcp0, o, i) * TB(m->kdk, o, i);
+        if (SP(m->ssin, o, p, i) > 0) {
+          t1x = t1x + dm * m->cth[p - 1];
+          t1y = t1y + dm * m->sth[p - 1];
+        } else if (cos(m->wdir[i] - m->th[p - 1]) < 0) {
+          t2x = t2x + dm * m->cth[p - 1];
+          t2y = t2y + dm * m->sth[p - 1];
+        } else {
+          t3x = t3x + dm * m->cth[p - 1];
+          t3y = t3y + dm * m->sth[p - 1];
+        }
+      }
+    m->taux1[i] = t1x * rw * dthg;
+    m->tauy1[i] = t1y * rw * dthg;
+    m->taux2[i] = t2x * rw * dthg;
+    m->tauy2[i] = t2y * rw * dthg;
+    m->taux3[i] = t3x * rw * dthg;
+    m->tauy3[i] = t3y * rw * dthg;
   }
 }
 

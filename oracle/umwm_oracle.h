@@ -52,6 +52,7 @@ void oracle_propagation(oracle_model* m);
 void oracle_refraction(oracle_model* m);
 void oracle_update(oracle_model* m);
 void oracle_stress_atm(oracle_model* m);
+void oracle_stress_ocn(oracle_model* m);
 void oracle_diag(oracle_model* m);
 void oracle_stokes_drift(oracle_model* m, int init);
 void oracle_init_jonswap(oracle_model* m, const umwm_jonswap* js);

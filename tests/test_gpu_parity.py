@@ -39,6 +39,22 @@ def test_setup_maps_bit_exact(case, oracle_lib):
         assert abs(d.scalar(name) - o.scalar(name)) <= 1e-15 * abs(o.scalar(name)), name
 
 
+OCN_PAIRS = [("tailocnx", "tailocny"), ("taux_ocntop", "tauy_ocntop"),
+             ("taux_ocnbot", "tauy_ocnbot"), ("taux_snl", "tauy_snl"), ("epsx_ocn", "epsy_ocn"),
+             ("epsx_atm", "epsy_atm"), ("taux1", "tauy1"), ("taux2", "tauy2"), ("taux3", "tauy3")]
+
+
+def _vector_compare(o, d, pairs, tol, tag):
+    """Vector fields judged against the vector's magnitude (a component of
+    a field symmetric about an axis is round-off)."""
+    for x, y in pairs:
+        ox, oy = o.get(x), o.get(y)
+        scale = max(np.max(np.abs(ox)), np.max(np.abs(oy)))
+        for name, ref in ((x, ox), (y, oy)):
+            err = float(np.max(np.abs(d.get(name) - ref)))
+            assert err <= tol * scale, f"{tag}: {name} abs err {err:.3e} vs scale {scale:.3e}"
+
+
 def _op_compare(o, d, names, tol, tag):
     for name in names:
         err = relerr(d.get(name), o.get(name))
@@ -88,6 +104,9 @@ def test_operators_fp64(case, oracle_lib):
         d.stress("atm")
         _op_compare(o, d, ["ustar", "cd", "taux", "tauy", "taux_form", "tauy_form",
                            "taux_skin", "tauy_skin", "taux_diag", "tauy_diag"], TOL_OP, tag)
+        o.stress_ocn()
+        d.stress("ocn")
+        _vector_compare(o, d, OCN_PAIRS, TOL_OP, tag)
         if step == 0:
             o.diag()
             d.diag()
@@ -181,3 +200,23 @@ def test_fp32_tiny_tracks_fp64(case, oracle_lib):
     o.run_dtg()
     d.run_dtg()
     assert hs_rmse(d.get("ht"), o.get("ht")) <= 0.005
+
+
+def test_stress_ocn_needs_operator_sources():
+    """After a fused substep the source arrays of stress('ocn') were never
+    materialised: the call reports a state error instead of reading stale
+    arrays; after an operator-path substep it works."""
+    from umwm_amd import hip
+    wl = tiny_case("land_shelf")
+    d = hip.Domain(wl.params, wl.grid_input(), wl.forcing_input(0), precision=8)
+    d.init_jonswap(wl.jonswap)
+    d.substep()
+    with pytest.raises(hip.UmwmError, match="stress\\('ocn'\\)"):
+        d.stress("ocn")
+    for op in ("forcinginterpolate", "sin_d12", "sds_d12", "snl_d12", "s_ice", "source",
+               "propagation", "refraction", "update"):
+        getattr(d, op)()
+    d.stress("atm")
+    d.stress("ocn")
+    assert np.all(np.isfinite(d.get("taux_ocntop")))
+    d.close()

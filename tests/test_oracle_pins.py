@@ -549,3 +549,26 @@ def test_source_diagnostic_tail_is_an_equilibrium(O):
             assert np.all(ef[i, ~pos, q] == 0)
             checked += int(pos.sum())
     assert checked > 100
+
+
+# ------------------------------------------------------------ stress('ocn')
+def test_stress_ocn_partitions_add_up(O):
+    """src/umwm_stress.F90:169-212: the three form-drag partitions (wind
+    pushing waves, waves against the wind, swell overrunning it) add up to the
+    form stress integral sum(e ssin cth kdk / cp0) rhow dthg, which at constant
+    depth (cp0(o,i-1) = cp0(o,i)) is stress('atm')'s taux_form minus its tail;
+    in deep water the bottom stress vanishes."""
+    o = regional(O, mm=8, nm=7, om=37, pm=36, dpt=4000.0, wspd0=14.0, wdir0=0.7)
+    o.init_jonswap(abi.make_jonswap(fp=0.1, dir_from_wind=False, thw=2.5))  # swell vs wind
+    o.substep()
+    o.substep()
+    o.stress_ocn()
+    sx = o.get("taux1") + o.get("taux2") + o.get("taux3")
+    sy = o.get("tauy1") + o.get("tauy2") + o.get("tauy3")
+    assert np.allclose(sx, o.get("taux_form") - o.get("tailatmx"), rtol=1e-10, atol=1e-14)
+    assert np.allclose(sy, o.get("tauy_form") - o.get("tailatmy"), rtol=1e-10, atol=1e-14)
+    for name in ("taux1", "taux2", "taux3"):
+        assert np.all(np.abs(o.get(name)) > 0)
+    top = np.hypot(o.get("taux_ocntop"), o.get("tauy_ocntop"))
+    bot = np.hypot(o.get("taux_ocnbot"), o.get("tauy_ocnbot"))
+    assert np.all(bot < 1e-12 * top)

@@ -172,6 +172,11 @@ int ensure_ops(DomainImpl<T>& D) {
   h.rotr = D.arena.template alloc<T>(ns);
   h.sdt = D.arena.template alloc<T>(nt);
   h.sice = D.arena.template alloc<T>(nt);
+  for (auto& p : D.ocn.f) {
+    p = D.arena.template alloc<double>(h.ldi);
+    if (!p) return fail(UMWM_ERR_NOMEM, "device allocation of operator arrays failed");
+    if (int rc = dfill<double>(p, h.ldi, 0.0, D.stream)) return rc;
+  }
   if (!h.ssin || !h.sds || !h.snl || !h.dummy || !h.rotl || !h.rotr || !h.sdt || !h.sice)
     return fail(UMWM_ERR_NOMEM, "device allocation of operator arrays failed");
   for (T* p : {h.ssin, h.sds, h.snl, h.dummy, h.rotl, h.rotr})
@@ -620,6 +625,11 @@ bool find_field(DomainImpl<T>& D, const std::string& n, FieldRef& r) {
       {"hs_ice", h.hs_ice}};
   for (auto& t : ptd)
     if (n == t.first) return set(F_PT_D, t.second);
+  for (int q = 0; q < kOcnFields; ++q)
+    if (n == ocn_field_names()[q]) {
+      if (!D.ocn.f[q]) return false;  // before the operator arrays exist
+      return set(F_PT_D, D.ocn.f[q]);
+    }
   if (n == "oc") return set(F_PT_I, h.oc);
   if (n == "oc_rot") return set(F_PT_I, h.oc_rot);
   if (n == "us") return set(F_STOKES, D.stokes.us);
@@ -1010,11 +1020,15 @@ int umwm_hip_refraction(umwm_domain* dom) {
 int umwm_hip_stress(umwm_domain* dom, const char* option) {
   if (!option) return fail(UMWM_ERR_ARGUMENT, "null option");
   const std::string opt(option, strnlen(option, 3));
-  if (opt == "ocn") return fail(UMWM_ERR_ARGUMENT, "stress('ocn') is not implemented");
-  if (opt != "atm") return fail(UMWM_ERR_ARGUMENT, "stress option must be 'atm' or 'ocn'");
+  if (opt != "atm" && opt != "ocn")
+    return fail(UMWM_ERR_ARGUMENT, "stress option must be 'atm' or 'ocn'");
   return dispatch(dom, [&](auto& D) {
+    if (opt == "ocn" && (D.ef_mirrors_e || !D.ops_ready))
+      return fail(UMWM_ERR_STATE,
+                  "stress('ocn') reads ssin, sds and snl of the last source evaluation: "
+                  "run that substep through the operator entry points");
     int rc = ensure_ops(D);
-    if (!rc) rc = launch_op_stress_atm(D);
+    if (!rc) rc = opt == "atm" ? launch_op_stress_atm(D) : launch_op_stress_ocn(D);
     if (!rc) rc = sync_and_check(D);
     return rc;
   });

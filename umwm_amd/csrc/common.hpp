@@ -125,6 +125,17 @@ struct DiagDev {
   double *dwd, *dwl, *dwp, *dcp0, *dcg0, *dcp, *dcg;
 };
 
+// stress('ocn') outputs (src/umwm_stress.F90:34-214), FP64, per sea point,
+// in the order of kOcnFields.
+constexpr int kOcnFields = 18;
+struct OcnDev {
+  double* f[kOcnFields];
+};
+inline const char* const* ocn_field_names() {
+  static const char* const n[kOcnFields] = {"tailocnx", "tailocny", "taux_ocntop", "tauy_ocntop", "taux_ocnbot", "tauy_ocnbot", "taux_snl", "tauy_snl", "epsx_ocn", "epsy_ocn", "epsx_atm", "epsy_atm", "taux1", "tauy1", "taux2", "tauy2", "taux3", "tauy3"};
+  return n;
+}
+
 // Stokes drift outputs (src/umwm_stokes.F90:13-132), FP64.
 struct StokesDev {
   const double* depth;  // lm levels, negative down

@@ -70,6 +70,7 @@ struct DomainImpl : DomainBase {
   DomainDev<T>* d = nullptr;  // device copy (array of one)
   ForcingDev<T> fh{};
   DiagDev diag{};
+  OcnDev ocn{};  // allocated with the operator arrays
   StokesDev stokes{};
   BlockTask* tasks = nullptr;
   int ntasks = 0;
@@ -98,6 +99,7 @@ template <typename T> int launch_op_propagation(DomainImpl<T>& D);
 template <typename T> int launch_op_refraction(DomainImpl<T>& D);
 template <typename T> int launch_op_update(DomainImpl<T>& D);
 template <typename T> int launch_op_stress_atm(DomainImpl<T>& D);
+template <typename T> int launch_op_stress_ocn(DomainImpl<T>& D);
 template <typename T> int launch_diag(DomainImpl<T>& D);
 template <typename T> int launch_stokes(DomainImpl<T>& D);
 template <typename T> int launch_hs_ice(DomainImpl<T>& D);

@@ -448,6 +448,101 @@ __global__ void k_stress_atm(DomainDev<T> d) {
 }
 
 // ---------------------------------------------------------------------
+// src/umwm_stress.F90:34-214 stress('ocn'): momentum into the ocean top
+// and bottom, Snl momentum loss, energy fluxes into the ocean and from the
+// air, and the form-drag partitions, from the current e and the source
+// arrays (ssin, sds after cothkd, snl) of the last source evaluation.
+// FP64 sums; one thread per sea point.
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ void k_stress_ocn(DomainDev<T> d, OcnDev out, double bf1, double bf2) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= d.im) return;
+  const size_t ldi = d.ldi;
+  const int om = d.om, pm = d.pm, oc = d.oc[j];
+  const double rw = d.rhow[j], dthg = d.dthg;
+  auto tab = [&](const T* a, int o) { return (double)a[(size_t)o * ldi + j]; };
+  auto spec = [&](const T* a, int o, int p) { return (double)a[((size_t)o * pm + p) * ldi + j]; };
+  double top_x = 0, top_y = 0, bot_x = 0, bot_y = 0, snl_x = 0, snl_y = 0;
+  double eo_x = 0, eo_y = 0, ea_x = 0, ea_y = 0, tomx = 0, tomy = 0;
+  double t1x = 0, t1y = 0, t2x = 0, t2y = 0, t3x = 0, t3y = 0;
+  const double cwd = cos((double)d.wdir[j]), swd = sin((double)d.wdir[j]);
+  for (int o = 0; o < om; ++o) {
+    const double icp = tab(d.invcp0, o), cp = tab(d.cp0, o), kdk = tab(d.kdk, o);
+    const double sdt = tab(d.sdt, o), sdv = tab(d.sdv, o), sbf = tab(d.sbf, o);
+    // Snl weights of bins o-1, o-2 (o >= 2, 0-based)
+    const double wtop = o >= 2 ? bf1 * cp * tab(d.invcp0, o - 1) + bf2 * cp * tab(d.invcp0, o - 2) : 0;
+    const double wsnl =
+        o >= 2 ? bf1 * (1 - cp * tab(d.invcp0, o - 1)) + bf2 * (1 - cp * tab(d.invcp0, o - 2)) : 0;
+    double ux = 0, uy = 0, bx = 0, by = 0, nx = 0, ny = 0, qx = 0, qy = 0, ax = 0, ay = 0;
+    for (int p = 0; p < pm; ++p) {
+      const double ct = d.cth[p], st = d.sth[p];
+      const double e = spec(d.e, o, p), sds = spec(d.sds, o, p), snl = spec(d.snl, o, p);
+      const double ssin = spec(d.ssin, o, p);
+      const double v = e * (sds + sdt + sdv);
+      ux = ux + v * ct * icp;
+      uy = uy + v * st * icp;
+      if (o >= 2) {
+        ux = ux - snl * wtop * ct * icp;
+        uy = uy - snl * wtop * st * icp;
+      }
+      bx = bx + e * sbf * ct * icp;
+      by = by + e * sbf * st * icp;
+      if (o >= 2 && o <= oc) {
+        nx = nx + snl * wsnl * ct * icp;
+        ny = ny + snl * wsnl * st * icp;
+      }
+      qx = qx + e * sds * ct;
+      qy = qy + e * sds * st;
+      if (o >= 2) {
+        qx = qx - snl * wtop * ct;
+        qy = qy - snl * wtop * st;
+      }
+      ax = ax + e * ssin * ct;
+      ay = ay + e * ssin * st;
+      const double dm = e * ssin * icp * kdk;
+      if (ssin > 0) {
+        t1x += dm * ct;
+        t1y += dm * st;
+      } else if (cwd * ct + swd * st < 0) {  // cos(wdir - th) < 0
+        t2x += dm * ct;
+        t2y += dm * st;
+      } else {
+        t3x += dm * ct;
+        t3y += dm * st;
+      }
+    }
+    top_x += ux * kdk;
+    top_y += uy * kdk;
+    bot_x += bx * kdk;
+    bot_y += by * kdk;
+    snl_x += nx * kdk;
+    snl_y += ny * kdk;
+    eo_x += qx * kdk;
+    eo_y += qy * kdk;
+    ea_x += ax * kdk;
+    ea_y += ay * kdk;
+    if (o == om - 1) {
+      tomx = ux;
+      tomy = uy;
+    }
+  }
+  double w = d.wspd[j];
+  w = w > 1e-2 ? w : 1e-2;
+  const double kom = tab(d.k, om - 1);
+  const double tail = stress_tail(w, kom);
+  const double tlx = tomx * kom * tail * rw * dthg, tly = tomy * kom * tail * rw * dthg;
+  const double vals[kOcnFields] = {
+      tlx, tly,
+      top_x * rw * dthg + tlx + d.taux_skin[j], top_y * rw * dthg + tly + d.tauy_skin[j],
+      bot_x * rw * dthg, bot_y * rw * dthg, snl_x * rw * dthg, snl_y * rw * dthg,
+      eo_x * rw * dthg, eo_y * rw * dthg, ea_x * rw * dthg, ea_y * rw * dthg,
+      t1x * rw * dthg, t1y * rw * dthg, t2x * rw * dthg, t2y * rw * dthg,
+      t3x * rw * dthg, t3y * rw * dthg};
+  for (int q = 0; q < kOcnFields; ++q) out.f[q][j] = vals[q];
+}
+
+// ---------------------------------------------------------------------
 // src/umwm_physics.F90:89-205 diag
 // ---------------------------------------------------------------------
 template <typename T>
@@ -712,6 +807,12 @@ int launch_op_stress_atm(DomainImpl<T>& D) {
 }
 
 template <typename T>
+int launch_op_stress_ocn(DomainImpl<T>& D) {
+  LAUNCH(k_stress_ocn<T>, grid1(D.h.im), dim3(kBlock), D.h, D.ocn, D.S.bf1, D.S.bf2);
+  return 0;
+}
+
+template <typename T>
 int launch_diag(DomainImpl<T>& D) {
   LAUNCH(k_diag<T>, grid1(D.h.im), dim3(kBlock), D.h, D.diag);
   return 0;
@@ -753,6 +854,7 @@ int launch_jonswap(DomainImpl<T>& D, const umwm_jonswap& js) {
   template int launch_op_refraction<T>(DomainImpl<T>&);                  \
   template int launch_op_update<T>(DomainImpl<T>&);                      \
   template int launch_op_stress_atm<T>(DomainImpl<T>&);                  \
+  template int launch_op_stress_ocn<T>(DomainImpl<T>&);                  \
   template int launch_diag<T>(DomainImpl<T>&);                           \
   template int launch_hs_ice<T>(DomainImpl<T>&);                         \
   template int launch_stokes<T>(DomainImpl<T>&);                         \
