@@ -13,9 +13,10 @@
  * modules use NetCDF-Fortran and datetime-fortran, neither of which exists
  * in this image.
  *
- * Pinning (tests/test_oracle_pins.py; the reference ships no tests, golden
- * vectors or fixtures for this path, so every pin is a closed form, an
- * invariant, a textbook reduction or a hand-enumerated case):
+ * Pinning (tests/test_oracle_pins.py, tests/test_oracle_pins_fluxes.py; the
+ * reference ships no tests, golden vectors or fixtures for this path, so
+ * every pin is a closed form, an invariant, a textbook reduction or a
+ * hand-enumerated case):
  *   pinned   sheltering_coare35, stress_tail, friction_velocity_skin,
  *            drag_coefficient_skin, distance_haversine (closed forms);
  *            f, th, cth2pp, bf1/bf2, snl_arg, dom, cfllim, dtamin (closed
@@ -23,22 +24,34 @@
  *            relation, deep-water limits); masks/remap index maps
  *            (hand-enumerated regional and periodic masks, estuary fill);
  *            init_jonswap + diag Hs/Tm (closed form, Hs = 0.607 m);
+ *            diag's other outputs (single-component closed forms: Tm, mean
+ *            and dominant direction, wavelength, period, mss, momentum and
+ *            its flux, Doppler-shifted speeds; two-component bisector);
+ *            forcinginterpolate (linear interpolation in time through the
+ *            index map); sin_d12 (deep-water Donelan et al. 2012 form with
+ *            the sheltering, opposing-wind and swell branches, the ice
+ *            factor and the tail clamp); s_ice (Kohout et al. 2014);
  *            sds_d12 (isotropic prefix closed form); snl_d12 (energy
- *            telescoping); propagation (conservation, exact first moment,
- *            steady uniform field); refraction (per-frequency conservation,
- *            dtr bound); stokes_drift us/vs (textbook Stokes drift of one
- *            bin, deep and finite depth); source dts choice, E* = (e+E_s)/2
- *            and the diagnostic tail (breaking/input equilibrium);
- *            cold first substep; stress('ocn') form-drag partitions
- *            (sum to the form stress; zero bottom stress in deep water).
- *   partial  sin_d12 (signs, monotonicity, tail clamp; the magnitude of
- *            each factor only through its closed-form pieces).
- *   parity unpinned: the exponential integration of prognostic bins in
- *            source; refraction's rotation magnitude; the stress('atm')
- *            frequency/direction integral (incl. the cp0(o,i-1) quirk) and
- *            its tail; stress('ocn') ocean-top, bottom, Snl-loss and energy
- *            flux integrals; the Stokes e-folding depth; diag outputs other
- *            than Hs and Tm; forcinginterpolate.
+ *            telescoping); source dts choice, E* = (e+E_s)/2, the
+ *            exponential integration of prognostic bins (scipy's solution
+ *            of the frozen-rate ODE plus an Euler Snl increment) and the
+ *            diagnostic tail (breaking/input equilibrium); cold first
+ *            substep; propagation (conservation, exact first moment, steady
+ *            uniform field); refraction (per-frequency conservation, dtr
+ *            bound, the rotation rate as the phase-ray turning rate over a
+ *            slope and as the vorticity of a sheared current); stress('atm')
+ *            (single-bin wave-supported stress with the cp0(o,i-1) divisor,
+ *            the tail integral, skin stress on the relative wind, ustar, cd);
+ *            stress('ocn') (E = c M between energy and momentum fluxes, all
+ *            dissipation into the top, bottom-friction share, Snl momentum
+ *            loss and hand-down of a donor bin, form-drag partitions);
+ *            stokes_drift us/vs (textbook Stokes drift of one bin, deep and
+ *            finite depth) and the e-folding depth (exponential decay of one
+ *            deep-water component).
+ *   partial  stress('ocn'): the Snl corrections of bins at the third
+ *            frequency and above that receive energy (only donor bins are
+ *            pinned); the substep driver order (src/umwm_top.F90:89-141) is
+ *            a sequence of calls and has no pin of its own beyond the pieces.
  */
 #include "umwm_oracle.h"
 

@@ -1,0 +1,543 @@
+"""More pins of the FP64 oracle: the spectral integrals of diag, the time
+interpolation of the forcing, the wind input, sea-ice attenuation, the
+exponential integration of source, the
+rotation rate of refraction, the stress('atm') and stress('ocn') integrals
+and the Stokes e-folding depth.
+
+Each test reduces the oracle's output to something the mathematics or the
+physics fixes independently of the reference's expression: a spectrum with a
+single nonzero bin (whose integrals are closed forms), the exact solution of
+the frozen-rate ODE from scipy's integrator, numpy's linear interpolation, the
+ray equation with the depth derivative of the dispersion relation, and the
+momentum and energy carried by one wave component (M = E / c).  CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, optimize
+
+from umwm_amd import abi
+
+G = 9.80665
+SIGMA, RHOW = 0.07, 1030.0
+
+
+@pytest.fixture(scope="module")
+def O(oracle_lib):
+    return oracle_lib
+
+
+def regional(O, mm=8, nm=7, om=25, pm=24, depth=None, forcing=None, **kw):
+    p = abi.make_params(mm=mm, nm=nm, om=om, pm=pm, **kw)
+    grid = abi.make_grid_input(z=-depth) if depth is not None else None
+    if depth is not None:
+        p.topofromfile = 1
+    return O.Oracle(p, grid, forcing, threads=1)
+
+
+def single_bin(o, q, p, value):
+    e = o.get("e")
+    e[:] = 0
+    e[:, p, q] = value
+    return e
+
+
+def wrap(a):
+    return math.atan2(math.sin(a), math.cos(a))
+
+
+# --------------------------------------------------------------------- diag
+def test_diag_single_bin_closed_forms(O):
+    """src/umwm_physics.F90:89-205 for a spectrum whose only nonzero bin is
+    (f_q, th_p) with variance v = e kdk dth: Tm = 1/f, mean and dominant
+    direction th_p, mean and dominant wavelength 2 pi / k, mss = k^2 v, wave
+    momentum M = rho_w g v / cp along th_p, its flux cg M (cos, sin)(cos, sin),
+    dominant phase and group speeds Doppler-shifted by the current."""
+    o = regional(O, mm=7, nm=6, om=25, pm=24, dpt=35.0, wspd0=11.0, uc0=0.3, vc0=-0.2)
+    o.sin_d12()  # sets oc; the momentum sums run over o <= oc
+    oc = o.get("oc")[: o.im]
+    q, p = int(oc.min()) - 3, 7
+    single_bin(o, q, p, 0.02)
+    o.diag()
+    f, k, kdk = o.get("f")[q], o.get("k")[:, q], o.get("kdk")[:, q]
+    cp, cg = o.get("cp0")[:, q], o.get("cg0")[:, q]
+    th = o.get("th")[p]
+    dth = 2 * math.pi / 24
+    v = 0.02 * kdk * dth
+    rhow = o.get("rhow")[: o.im]
+    c, s = math.cos(th), math.sin(th)
+    mom = rhow * G * v / cp
+    assert np.allclose(o.get("ht"), 4 * np.sqrt(v), rtol=1e-14)
+    assert np.allclose(o.get("mwp"), 1 / f, rtol=1e-13)
+    assert np.allclose([wrap(a - th) for a in o.get("mwd")], 0, atol=1e-13)
+    assert np.allclose(o.get("mwl"), 2 * math.pi / k, rtol=1e-13)
+    assert np.allclose(o.get("mss"), k * k * v, rtol=1e-13)
+    assert np.allclose(o.get("momx"), mom * c, rtol=1e-13)
+    assert np.allclose(o.get("momy"), mom * s, rtol=1e-13)
+    assert np.allclose(o.get("cgmxx"), cg * mom * c * c, rtol=1e-13)
+    assert np.allclose(o.get("cgmxy"), cg * mom * c * s, rtol=1e-13)
+    assert np.allclose(o.get("cgmyy"), cg * mom * s * s, rtol=1e-13)
+    assert np.allclose(o.get("dwd"), th, rtol=1e-15)
+    assert np.allclose(o.get("dwl"), 2 * math.pi / k, rtol=1e-15)
+    assert np.allclose(o.get("dwp"), 1 / f, rtol=1e-15)
+    assert np.allclose(o.get("dcp0"), cp, rtol=1e-15)
+    assert np.allclose(o.get("dcg0"), cg, rtol=1e-15)
+    assert np.allclose(o.get("dcp"), cp + 0.3 * c - 0.2 * s, rtol=1e-14)
+    assert np.allclose(o.get("dcg"), cg + 0.3 * c - 0.2 * s, rtol=1e-14)
+
+    # a bin above the prognostic range carries variance but no momentum
+    single_bin(o, int(oc.max()) + 1, p, 0.02)
+    o.diag()
+    assert np.all(o.get("ht") > 0)
+    assert np.all(o.get("momx") == 0) and np.all(o.get("cgmyy") == 0)
+
+
+def test_diag_two_components(O):
+    """Two components of equal variance at th1 and th2: the mean direction is
+    their bisector; Tm02 = sqrt(m0 / m2) of the two-line spectrum; the
+    dominant bin is the one of larger variance."""
+    o = regional(O, mm=6, nm=5, om=25, pm=24, dpt=4000.0)
+    kdk = o.get("kdk")
+    q1, q2, p1, p2 = 6, 11, 3, 9
+    e = o.get("e")
+    e[:] = 0
+    e[:, p1, q1] = 0.01 / kdk[:, q1]
+    e[:, p2, q2] = 0.01 / kdk[:, q2]
+    o.diag()
+    th, f = o.get("th"), o.get("f")
+    assert np.allclose([wrap(a - 0.5 * (th[p1] + th[p2])) for a in o.get("mwd")], 0, atol=1e-12)
+    tm = math.sqrt(2 / (f[q1] ** 2 + f[q2] ** 2))
+    assert np.allclose(o.get("mwp"), tm, rtol=1e-12)
+    e[:, p2, q2] *= 1.5
+    o.diag()
+    assert np.allclose(o.get("dwp"), 1 / f[q2]) and np.allclose(o.get("dwd"), th[p2])
+
+
+# ------------------------------------------------------- forcinginterpolate
+def test_forcinginterpolate_is_linear_in_time(O):
+    """src/umwm_forcing.F90:40-99: every forcing field at model time sumt is
+    numpy's linear interpolation between the previous and the next snapshot
+    (t = 0 and t = dtg), taken through the unrolled index map; wind speed and
+    direction come from the interpolated components; rhorat = rhoa / rhow."""
+    mm, nm = 7, 6
+    depth = np.full((mm, nm), 80.0)
+    depth[3, 2] = -4.0  # an island
+    rng = np.random.default_rng(7)
+
+    def snap():
+        return {name: rng.uniform(lo, hi, size=(mm, nm)) for name, lo, hi in
+                (("uw", -15, 15), ("vw", -15, 15), ("uc", -1, 1), ("vc", -1, 1),
+                 ("rhoa", 1.1, 1.3), ("rhow", 1020, 1030), ("fice", 0, 1))}
+
+    s0, s1 = snap(), snap()
+    o = regional(O, mm=mm, nm=nm, depth=depth, forcing=abi.make_forcing_input(**s0),
+                 winds=1, currents=1, air_density=1, water_density=1, seaice=1)
+    o.forcinginput(abi.make_forcing_input(**s1))
+    mi, ni = o.get("mi")[: o.im] - 1, o.get("ni")[: o.im] - 1
+    dtg = 3600.0
+
+    def lerp(name, t):
+        a, b = s0[name][mi, ni], s1[name][mi, ni]
+        return np.array([np.interp(t, [0.0, dtg], [x, y]) for x, y in zip(a, b)])
+
+    for t in (0.0, 900.0, 1800.0, 2700.0, 3600.0):
+        o.set_scalar("sumt", t)
+        o.forcinginterpolate()
+        uw, vw = lerp("uw", t), lerp("vw", t)
+        assert np.allclose(o.get("wspd")[: o.im], np.hypot(uw, vw), rtol=1e-13)
+        assert np.allclose(o.get("wdir")[: o.im], np.arctan2(vw, uw), rtol=1e-12, atol=1e-14)
+        for name in ("uc", "vc", "rhoa", "rhow", "fice"):
+            assert np.allclose(o.get(name)[: o.im], lerp(name, t), rtol=1e-13, atol=1e-15), name
+        assert np.allclose(o.get("rhorat")[: o.im], lerp("rhoa", t) / lerp("rhow", t), rtol=1e-13)
+
+
+# ------------------------------------------------------------------ source
+@pytest.mark.parametrize("snl_fac", [0.0, 5.0])
+def test_source_integration_solves_the_frozen_rate_ode(O, snl_fac):
+    """src/umwm_physics.F90:59-68: over one substep the prognostic bins solve
+    dE/dt = (Sin - Sds - Sbf - Sdt - Sdv + Sice) E with the rates frozen
+    (checked against scipy's DOP853 solution of that ODE), and the nonlinear
+    transfer is added as a forward-Euler increment dts * snl."""
+    o = regional(O, mm=7, nm=6, om=37, pm=36, dpt=25.0, wspd0=16.0, wdir0=0.4,
+                 snl_fac=snl_fac, fice0=0.5)
+    o.init_jonswap(abi.make_jonswap(fp=0.12))
+    o.substep()  # cold substep (dts = 0)
+    for op in ("forcinginterpolate", "sin_d12", "sds_d12", "snl_d12", "s_ice"):
+        getattr(o, op)()
+    e0 = o.get("e").copy()
+    rate = (o.get("ssin") - o.get("sds") - (o.get("sbf") + o.get("sdt") + o.get("sdv")
+                                            - o.get("sice"))[:, None, :])
+    snl = o.get("snl").copy()
+    if snl_fac == 0:
+        assert np.all(snl == 0)
+    o.source()
+    dts = o.scalar("dts")
+    assert dts > 0
+    oc = o.get("oc")[: o.im]
+    prog = np.arange(o.om)[None, None, :] < oc[:, None, None]
+    prog = np.broadcast_to(prog, e0.shape)
+    assert np.all(o.get("sice") < 0)
+    r, y0 = rate[prog], e0[prog]
+    sol = integrate.solve_ivp(lambda t, y: r * y, (0.0, dts), y0, method="DOP853",
+                              rtol=1e-13, atol=1e-300)
+    exact = sol.y[:, -1]
+    es = o.get("ef")[prog]
+    assert np.allclose(es, exact + dts * snl[prog], rtol=1e-10, atol=1e-12 * float(y0.max()))
+    assert float(np.max(np.abs(r * dts))) > 0.05  # the exponential matters
+
+
+# ------------------------------------------------------------------ sin_d12
+def test_sin_d12_donelan_deep_water(O):
+    """src/umwm_source_functions.F90:16-82 in deep water reduces to Donelan et
+    al. (2012): Sin = A (rho_a / rho_w) (omega k / g) x |x| (1 - ice) (omega k / g
+    is omega / c^2 for gravity waves; the capillary correction of the
+    dispersion relation is left out of this factor) with
+    x = U(lambda/2) cos(phi - th) - c - U_c . k_hat, U(z) = U10 + u*/kappa
+    ln(z / 10) at z = lambda/2 (capped at 20 m), kappa = 0.4, and A the
+    sheltering coefficient for x > 0, sin_diss1 against the wind and sin_diss2
+    for swell overrunning it; above the cut-off only growth is kept.  c and
+    lambda come from the exact gravity-capillary root; the oracle's wavenumbers
+    are Newton-truncated (about 1e-4), hence the tolerance."""
+    U, phi, uc, vc, ice = 12.0, 0.3, 0.2, 0.1, 0.2
+    o = regional(O, mm=6, nm=5, om=37, pm=36, dpt=4000.0, wspd0=U, wdir0=phi, uc0=uc,
+                 vc0=vc, fice0=ice)
+    o.forcinginterpolate()
+    o.sin_d12()
+    f, th = o.get("f"), o.get("th")
+    ssin = o.get("ssin")
+    ustar = o.get("ustar")[0]
+    oc = int(o.get("oc")[0])
+    shelt = O.sheltering_coare35(U)
+    rr = 1.2 / 1030.0
+    worst = 0.0
+    for q in range(o.om):
+        w = 2 * math.pi * f[q]
+        F = lambda x: G * x * math.tanh(x * 4000.0) * (1 + SIGMA * x * x / (RHOW * G)) - w * w
+        k = optimize.brentq(F, 1e-8, 1e5, xtol=1e-15, rtol=1e-15)
+        c = w / k
+        z = min(math.pi / k, 20.0)
+        ul2 = U + ustar / 0.4 * math.log(z / 10.0)
+        x = ul2 * np.cos(phi - th) - c - uc * np.cos(th) - vc * np.sin(th)
+        a = np.where(x > 0, shelt, np.where(np.cos(phi - th) > 0, 0.001, 0.10))
+        expect = a * rr * (w * k / G) * x * np.abs(x) * (1 - ice)
+        if q + 1 > oc:
+            expect = np.maximum(expect, 0)
+        got = ssin[0, :, q]
+        scale = float(np.abs(expect).max())
+        worst = max(worst, float(np.abs(got - expect).max()) / max(scale, 1e-300))
+        assert np.all(np.sign(got) == np.sign(expect)), q
+    assert worst < 1e-3
+
+
+# ------------------------------------------------------------------- s_ice
+@pytest.mark.parametrize("scale,fice", [(1.0, 0.5), (50.0, 0.5), (1.0, 0.2)])
+def test_sice_kohout_attenuation(O, scale, fice):
+    """src/umwm_source_functions.F90:108-153, Kohout et al. (2014): in ice
+    above the lower threshold the wave height decays as dHs/dx = C1 Hs below
+    Hs = 3 m and C1 x 3 m above (C1 = -5.35e-6 /m), i.e. an energy decay rate
+    in time of 2 cg dHs/dx / 1; Hs from the JONSWAP closed form (0.607 m,
+    x sqrt(scale))."""
+    o = regional(O, mm=6, nm=5, om=37, pm=36, dpt=4000.0, wspd0=15.0, fice0=fice)
+    o.init_jonswap(abi.make_jonswap(alpha=0.01, gamma=3.3, fp=0.3, dir_from_wind=True))
+    o.get("e")[:] *= scale
+    o.s_ice()
+    f = o.get("f")
+    dlnf = math.log(f[1] / f[0])
+    sig = np.where(f <= 0.3, 0.07, 0.09)
+    S = (0.01 * G * G * (2 * math.pi) ** -4 * f ** -5 * np.exp(-1.25 * (0.3 / f) ** 4)
+         * 3.3 ** np.exp(-(f - 0.3) ** 2 / (2 * sig * sig * 0.09)))
+    hs = 4 * math.sqrt(scale * float(np.sum(S * f * dlnf)))
+    C1 = -5.35e-6
+    if fice <= 0.3:
+        expect = np.zeros_like(o.get("cg0"))
+    else:
+        expect = 2 * o.get("cg0") * (C1 * hs if hs < 3.0 else C1 * 3.0)
+    assert np.allclose(o.get("sice"), expect, rtol=1e-10, atol=0)
+
+
+# -------------------------------------------------------------- refraction
+def _phase_speed_and_depth_derivative(f, d):
+    """Phase speed c = w / k of the dispersion relation
+    w^2 = g k tanh(kd) (1 + sigma k^2 / (rho g)) and dc/dd from implicit
+    differentiation at fixed w."""
+    w = 2 * math.pi * f
+    F = lambda x: G * x * math.tanh(x * d) * (1 + SIGMA * x * x / (RHOW * G)) - w * w
+    k = optimize.brentq(F, 1e-8, 1e5, xtol=1e-15, rtol=1e-15)
+    t = math.tanh(k * d)
+    sur = 1 + SIGMA * k * k / (RHOW * G)
+    dF_dk = G * t * sur + G * k * (1 - t * t) * d * sur + G * k * t * 2 * SIGMA * k / (RHOW * G)
+    dF_dd = G * k * (1 - t * t) * k * sur
+    dk_dd = -dF_dd / dF_dk
+    return w / k, -w / (k * k) * dk_dd
+
+
+def test_refraction_rotation_follows_phase_rays(O):
+    """src/umwm_advection.F90:118-142 on a gently sloping bottom: the rotation
+    rate of direction th is the phase-ray turning rate
+    dth/dt = dc/dx sin(th) - dc/dy cos(th) (waves turn towards slower, shallower
+    water), with dc/dx from the depth derivative of the dispersion relation,
+    and rotl / rotr are its means over the edges with p+1 / p-1.  The centred
+    difference and the oracle's Newton-truncated wavenumbers leave a few 1e-3."""
+    mm, nm, dxy = 12, 9, 2000.0
+    a, b = 1.0e-3, -6.0e-4  # depth gradient (m per m) in x and y
+    m = np.arange(mm)[:, None]
+    n = np.arange(nm)[None, :]
+    depth = 45.0 + a * m * dxy + b * n * dxy
+    o = regional(O, mm=mm, nm=nm, om=25, pm=24, depth=depth, delx=dxy, dely=dxy, wspd0=10.0)
+    o.sin_d12()
+    o.set_scalar("first", 1)
+    o.set_scalar("dts", 60.0)
+    o.refraction()
+    mi, ni = o.get("mi")[: o.im], o.get("ni")[: o.im]
+    d, f, th = o.get("d")[: o.im], o.get("f"), o.get("th")
+    oc = o.get("oc")[: o.im]
+    rotl, rotr = o.get("rotl"), o.get("rotr")
+    inner = np.where((mi > 2) & (mi < mm - 1) & (ni > 2) & (ni < nm - 1))[0]
+    assert len(inner) >= 20
+    worst = 0.0
+    scale = 0.0
+    for i in inner:
+        for q in range(0, int(oc[i]), 3):
+            _, dcdd = _phase_speed_and_depth_derivative(f[q], d[i])
+            cx, cy = dcdd * a, dcdd * b
+            R = cx * np.sin(th) - cy * np.cos(th)
+            expect_l = 0.5 * (R + np.roll(R, -1))
+            expect_r = 0.5 * (R + np.roll(R, 1))
+            scale = max(scale, float(np.abs(R).max()))
+            worst = max(worst, float(np.abs(rotl[i, :, q] - expect_l).max()),
+                        float(np.abs(rotr[i, :, q] - expect_r).max()))
+    assert scale > 1e-4
+    assert worst < 5e-3 * scale
+    # waves heading north over a bottom deepening to the east turn west (th grows)
+    p_north = int(np.argmax(np.sin(th)))
+    assert math.sin(th[p_north]) > 0.99
+    i0 = inner[0]
+    assert rotl[i0, p_north, 0] + rotr[i0, p_north, 0] > 0
+
+
+def test_refraction_current_rotation_is_the_vorticity(O):
+    """With deep water (no depth refraction) and a sheared current
+    uc = alpha y, vc = beta x, the rotation rate in every direction is the
+    relative vorticity dvc/dx - duc/dy = beta - alpha; a uniform current adds
+    nothing."""
+    mm, nm, dxy = 9, 8, 3000.0
+    alpha, beta = 2.0e-5, -7.0e-6
+    x = np.arange(mm)[:, None] * dxy * np.ones((1, nm))
+    y = np.arange(nm)[None, :] * dxy * np.ones((mm, 1))
+    zero = np.zeros((mm, nm))
+    fz = abi.make_forcing_input(uc=0.4 + alpha * y, vc=-0.3 + beta * x)
+    o = regional(O, mm=mm, nm=nm, om=25, pm=24, depth=np.full((mm, nm), 4000.0), forcing=fz,
+                 currents=1, delx=dxy, dely=dxy, wspd0=10.0)
+    o.forcinginput(abi.make_forcing_input(uc=0.4 + alpha * y, vc=-0.3 + beta * x))
+    o.forcinginterpolate()
+    o.sin_d12()
+    o.set_scalar("first", 1)
+    o.set_scalar("dts", 60.0)
+    o.refraction()
+    mi, ni = o.get("mi")[: o.im], o.get("ni")[: o.im]
+    inner = np.where((mi > 2) & (mi < mm - 1) & (ni > 2) & (ni < nm - 1))[0]
+    oc = o.get("oc")[: o.im]
+    for i in inner:
+        for q in range(int(oc[i])):
+            assert np.allclose(o.get("rotl")[i, :, q], beta - alpha, rtol=1e-9)
+            assert np.allclose(o.get("rotr")[i, :, q], beta - alpha, rtol=1e-9)
+    del zero
+
+
+# ----------------------------------------------------------- stress('atm')
+def _tail_integral(w, kmax):
+    t = 0.000112 * w * w - 0.01451 * w - 1.0186
+    val, _ = integrate.quad(lambda kk: (kk / kmax) ** t, kmax, 1e3, limit=200,
+                            epsabs=0, epsrel=1e-13)
+    return val
+
+
+def test_stress_atm_single_bin_momentum_flux(O):
+    """src/umwm_stress.F90:216-264: the form stress is the wave-supported
+    stress rho_w g dth Sin e kdk / c of the one nonzero bin along its
+    direction, with c taken at the previous sea point (the dummy argument
+    cp(:,:) of stress_vector_integral has lower bound 1 over cp0(om,0:im), so
+    cp(o,i) reads cp0(o,i-1)); above the last bin the integrand continues as
+    k^tail up to k = 1000 (tail = k(om) x that integral); the skin stress
+    follows the wind relative to current + surface Stokes drift; ustar and cd
+    close the stress vector."""
+    mm, nm = 9, 7
+    depth = 12.0 + 6.0 * np.arange(mm)[:, None] * np.ones((1, nm))
+    o = regional(O, mm=mm, nm=nm, om=25, pm=24, depth=depth, wspd0=13.0, wdir0=0.5,
+                 uc0=0.25, vc0=-0.1)
+    o.stokes_drift(True)
+    cp_prev = o.get("cp0", ghost=True)[:-1]  # row i is point i's predecessor
+    th = o.get("th")
+    dth = 2 * math.pi / 24
+    rhow, rhoa = o.get("rhow")[: o.im], o.get("rhoa")[: o.im]
+    for q, p in ((8, 13), (24, 12)):  # th_13 is 0.1 rad from the wind, th_12 0.16
+        single_bin(o, q, p, 0.01)
+        o.stokes_drift(False)
+        o.forcinginterpolate()  # sets rhorat = rhoa / rhow
+        o.sin_d12()
+        o.stress_atm()
+        ssin, kdk, k = o.get("ssin")[:, p, q], o.get("kdk")[:, q], o.get("k")[:, q]
+        assert np.all(ssin > 0)
+        w = o.get("wspd")[: o.im]
+        util = 0.01 * ssin / cp_prev[:, q]
+        form = rhow * G * dth * util * kdk
+        if q == 24:
+            tail = np.array([util[i] * k[i] * _tail_integral(w[i], k[i]) * rhow[i] * G * dth
+                             for i in range(o.im)])
+        else:
+            tail = np.zeros(o.im)
+        assert np.allclose(o.get("tailatmx"), tail * math.cos(th[p]), rtol=1e-11, atol=0)
+        assert np.allclose(o.get("tailatmy"), tail * math.sin(th[p]), rtol=1e-11, atol=0)
+        tfx = (form + tail) * math.cos(th[p])
+        tfy = (form + tail) * math.sin(th[p])
+        assert np.allclose(o.get("taux_form"), tfx, rtol=1e-11, atol=0)
+        assert np.allclose(o.get("tauy_form"), tfy, rtol=1e-11, atol=0)
+        # diagnostic part: bins o >= oc (1-based)
+        oc = o.get("oc")[: o.im]
+        diag = np.where(q + 1 >= oc, form, 0.0)
+        assert np.allclose(o.get("taux_diag"), diag * math.cos(th[p]), rtol=1e-11, atol=0)
+        # skin stress and closure
+        us0, vs0 = o.get("us")[0], o.get("vs")[0]
+        wd = o.get("wdir")[: o.im]
+        urel = w * np.cos(wd) - (0.25 + us0)
+        vrel = w * np.sin(wd) - (-0.1 + vs0)
+        wrel = np.hypot(urel, vrel)
+        cdf = np.hypot(tfx, tfy) / (rhoa * w * w)
+        cds = np.array([O.drag_coefficient_skin(cdf[i], w[i], wrel[i], 10.0, 1.56e-5, 0.4)
+                        for i in range(o.im)])
+        sx = rhoa * cds * wrel ** 2 * urel / wrel
+        sy = rhoa * cds * wrel ** 2 * vrel / wrel
+        assert np.allclose(o.get("taux_skin"), sx, rtol=1e-11)
+        assert np.allclose(o.get("tauy_skin"), sy, rtol=1e-11)
+        tx, ty = tfx + sx, tfy + sy
+        assert np.allclose(o.get("ustar"), np.sqrt(np.hypot(tx, ty) / rhoa), rtol=1e-12)
+        assert np.allclose(o.get("cd"), np.hypot(tx, ty) / (rhoa * w * w), rtol=1e-12)
+    # the quirk matters here: the bottom slopes, so c differs between neighbours
+    assert np.max(np.abs(cp_prev[1:, 8] / o.get("cp0")[1:, 8] - 1)) > 1e-2
+
+
+# ----------------------------------------------------------- stress('ocn')
+def _ocn_case(O, depth, q, p, **kw):
+    o = regional(O, mm=7, nm=6, om=25, pm=24, dpt=depth, wspd0=14.0, wdir0=0.3, **kw)
+    o.stokes_drift(True)
+    e = single_bin(o, q, p, 0.0)
+    e[:, p, q] = 3e-3 * o.get("oneoverk4")[:, q]  # saturation e k^4 = 3e-3
+    for op in ("forcinginterpolate", "sin_d12", "sds_d12", "snl_d12", "s_ice"):
+        getattr(o, op)()
+    o.stress_atm()
+    o.stress_ocn()
+    return o
+
+
+@pytest.mark.parametrize("depth,q,dissipation", [(4000.0, 5, False), (12.0, 3, False),
+                                                 (30.0, 6, True)])
+def test_stress_ocn_single_bin_momentum_and_energy(O, depth, q, dissipation):
+    """src/umwm_stress.F90:34-103,135-181 with Snl off and one nonzero bin:
+    each flux is carried by one wave component, so energy flux = c x momentum
+    flux (E = c M) from the air (eps_atm against the three form-drag
+    partitions) and into the ocean.  The top stress (minus the skin stress)
+    is the momentum of all dissipation, Sds + Sdt + Sdv, while eps_ocn counts
+    breaking alone; the bottom stress is the share Sbf of the same momentum;
+    no Snl momentum loss."""
+    p = 2
+    kw = dict(snl_fac=0.0) if dissipation else dict(snl_fac=0.0, sdt_fac=0.0, nu_water=0.0)
+    o = _ocn_case(O, depth, q, p, **kw)
+    cp = o.get("cp0")[:, q]
+    th = o.get("th")[p]
+    assert np.all(o.get("ssin")[:, p, q] != 0)
+    parts_x = o.get("taux1") + o.get("taux2") + o.get("taux3")
+    parts_y = o.get("tauy1") + o.get("tauy2") + o.get("tauy3")
+    assert np.allclose(o.get("epsx_atm"), cp * parts_x, rtol=1e-12, atol=0)
+    assert np.allclose(o.get("epsy_atm"), cp * parts_y, rtol=1e-12, atol=0)
+    topx = o.get("taux_ocntop") - o.get("taux_skin") - o.get("tailocnx")
+    topy = o.get("tauy_ocntop") - o.get("tauy_skin") - o.get("tailocny")
+    assert np.all(o.get("tailocnx") == 0)
+    sds = o.get("sds")[:, p, q]
+    dis = sds + o.get("sdt")[:, q] + o.get("sdv")[:, q]
+    if dissipation:
+        assert np.all(o.get("sdt")[:, q] > 0) and np.all(o.get("sdv")[:, q] > 0)
+    else:
+        assert np.all(dis == sds)
+    assert np.allclose(o.get("epsx_ocn") * dis / sds, cp * topx, rtol=1e-10, atol=0)
+    assert np.allclose(o.get("epsy_ocn") * dis / sds, cp * topy, rtol=1e-10, atol=0)
+    assert np.allclose(topy, topx * math.tan(th), rtol=1e-10)
+    ratio = o.get("sbf")[:, q] / dis
+    assert np.allclose(o.get("taux_ocnbot"), topx * ratio, rtol=1e-10, atol=0)
+    assert np.allclose(o.get("tauy_ocnbot"), topy * ratio, rtol=1e-10, atol=0)
+    if depth < 100:
+        assert np.all(ratio > 1e-3)
+    assert np.all(o.get("taux_snl") == 0) and np.all(o.get("tauy_snl") == 0)
+
+
+def test_stress_ocn_snl_momentum_loss_of_a_donor(O):
+    """src/umwm_stress.F90:105-133 in deep water: a single bin at the third
+    frequency gives away snl_fac Sds e to the two next lower frequencies with
+    weights bf1, bf2 (the closed-form Gaussians of dlnf), keeping energy.  The
+    momentum that energy carries drops from 1/c0 to 1/c1 and 1/c2 per unit
+    energy, so Snl loses snl_fac Sds e kdk (bf1 (1/c0 - 1/c1) + bf2 (1/c0 -
+    1/c2)) of momentum along the bin's direction.  The two receiving bins lie
+    below the third frequency, where the reference's Snl corrections start,
+    so only the donor's terms enter."""
+    q, p = 2, 4
+    o = _ocn_case(O, 4000.0, q, p, snl_fac=5.0)
+    f = o.get("f")
+    dlnf = math.log(f[1] / f[0])
+    b1, b2 = math.exp(-16 * dlnf * dlnf), math.exp(-64 * dlnf * dlnf)
+    bf1, bf2 = b1 / (b1 + b2), b2 / (b1 + b2)
+    cp = o.get("cp0")
+    c0, c1, c2 = cp[:, 2], cp[:, 1], cp[:, 0]
+    sds = o.get("sds")[:, p, q] / o.get("cothkd")[:, q]  # snl used Sds before the coth factor
+    kdk = o.get("kdk")[:, q]
+    e = o.get("e")[:, p, q]
+    loss = 5.0 * sds * e * kdk * (bf1 * (1 / c0 - 1 / c1) + bf2 * (1 / c0 - 1 / c2))
+    rhow = o.get("rhow")[: o.im]
+    dth = 2 * math.pi / 24
+    th = o.get("th")[p]
+    assert np.allclose(o.get("taux_snl"), -loss * rhow * G * dth * math.cos(th), rtol=1e-11)
+    assert np.allclose(o.get("tauy_snl"), -loss * rhow * G * dth * math.sin(th), rtol=1e-11)
+    assert np.all(loss > 0)
+    # the ocean-top stress adds to the momentum of all dissipation at c0 the
+    # momentum the handed-down energy carries at the receiving frequencies
+    # (energy / phase speed there); eps_ocn is c0 times the breaking part of it
+    sds_c = o.get("sds")[:, p, q]
+    dis = sds_c + o.get("sdt")[:, q] + o.get("sdv")[:, q]
+    moved = 5.0 * sds * e * (bf1 / c1 + bf2 / c2)
+    top = rhow * G * dth * kdk * (e * dis / c0 + moved)
+    assert np.allclose(o.get("taux_ocntop") - o.get("taux_skin"), top * math.cos(th), rtol=1e-11)
+    assert np.allclose(o.get("tauy_ocntop") - o.get("tauy_skin"), top * math.sin(th), rtol=1e-11)
+    eps = rhow * G * dth * kdk * (e * sds_c + c0 * moved)
+    assert np.allclose(o.get("epsx_ocn"), eps * math.cos(th), rtol=1e-11)
+    assert np.allclose(o.get("epsy_ocn"), eps * math.sin(th), rtol=1e-11)
+
+
+# ------------------------------------------------------------------ Stokes
+def test_stokes_efolding_depth_deep_water(O):
+    """src/umwm_stokes.F90:102-116: for one deep-water component the drift
+    decays as exp(-2 k z), so it falls to 1/e of its value at the first level
+    z1 at z1 + 1/(2k); the reference interpolates linearly between the two
+    bracketing levels (held as z = -depths) and negates the result, so the
+    depth is reported positive.  Levels are placed so
+    that the crossing sits a fifth of the way into its interval, where
+    swapped interpolation weights would be off by 0.6 of a level spacing."""
+    probe = regional(O, mm=6, nm=5, om=25, pm=24, dpt=4000.0)
+    q, p = 10, 3
+    k = float(probe.get("k")[0, q])
+    z1 = 0.3
+    n = 20
+    h = (1 / (2 * k)) / (n + 0.2)
+    depths = [z1 + j * h for j in range(n + 6)]
+    o = regional(O, mm=6, nm=5, om=25, pm=24, dpt=4000.0, depths=depths)
+    o.stokes_drift(True)
+    single_bin(o, q, p, 0.03)
+    o.stokes_drift(False)
+    ds = o.get("ds")
+    expect = z1 + 1 / (2 * k)
+    # linear interpolation of an exponential over one level: error <= h^2 (2k) / 8
+    # (U''/U' = 2k, times e^(2kh) for where in the interval each is taken)
+    assert np.allclose(ds, expect, rtol=0, atol=h * h * 2 * k / 8 * math.exp(2 * k * h))
+    assert np.all(ds > 0)
+    # no drift, no depth
+    o.get("e")[:] = 0
+    o.stokes_drift(False)
+    assert np.all(o.get("ds") == 0)
