@@ -83,8 +83,10 @@ def make_domain(wl, precision):
     return d
 
 
-def timed_steps(d, warmup, steps, tune=True):
-    """Cold substep, autotuning and warmup, then `steps` timed substeps.
+def timed_steps(d, warmup, steps, tune=True, kernel_steps=5):
+    """Cold substep, autotuning and warmup, then `kernel_steps` substeps
+    with per-kernel HIP-event timing, then `steps` substeps launched back
+    to back (one host wait, as run_dtg runs them) for the wall-clock rate.
     Returns wall seconds, per-kernel mean ms, and the tile chosen."""
     d.substep()  # cold first substep (dts = 0), not counted
     tile = None
@@ -92,17 +94,23 @@ def timed_steps(d, warmup, steps, tune=True):
         tile, _ = autotune.autotune(d, steps=2, verbose=log)
     for _ in range(warmup):
         d.substep()
-    d.synchronize()
     d.set_timing(True)
     d.timing(reset=True)
-    t0 = time.perf_counter()
-    sumt = 0.0
-    for _ in range(steps):
-        sumt, _ = d.substep()
-    d.synchronize()
-    wall = time.perf_counter() - t0
+    for _ in range(kernel_steps):
+        d.substep()
     ms, cnt = d.timing(reset=True)
     d.set_timing(False)
+    d.synchronize()
+    n0 = int(d.scalar("nsub"))
+    t0 = time.perf_counter()
+    _, _, n1 = d.substeps(steps)
+    wall = time.perf_counter() - t0
+    # every timed substep must have advanced the model: one that starts at
+    # sumt >= dtg does nothing and would inflate the rate
+    if n1 - n0 != steps:
+        raise RuntimeError(f"only {n1 - n0} of {steps} timed substeps were inside the hour; "
+                           "lower --steps/--warmup")
+    log(f"bench: {n1} substeps taken so far this hour")
     return wall, [m / max(c, 1) for m, c in zip(ms, cnt)], tile
 
 

@@ -197,11 +197,24 @@ int umwm_hip_end_first(umwm_domain* dom);
  * fused kernels.  Returns the new sumt and the dts taken.
  */
 int umwm_hip_substep(umwm_domain* dom, double* sumt, double* dts);
+
+/*
+ * `n` iterations of the same inner-loop body (src/umwm_top.F90:91-141),
+ * launched back to back with no host synchronisation between them; the
+ * scalars are read once at the end.  A substep that starts at
+ * sumt >= dtg finds the model inactive and does nothing, so a caller may
+ * run ahead of the loop test of :89.  Returns sumt and the dts of the
+ * last active substep, and `nsub`, the substeps taken since the current
+ * dtg began.
+ */
+int umwm_hip_substeps(umwm_domain* dom, int32_t n, double* sumt, double* dts, int32_t* nsub);
 /*
  * The inner loop to sumt >= dtg (src/umwm_top.F90:88-150) followed by the
  * per-dtg stokes_drift (when stokes) and diag (:152-155).  `nsub` receives
  * the number of substeps taken.  The first call of a run includes the
- * cold (dts = 0) substep.
+ * cold (dts = 0) substep.  The loop runs ahead: it launches as many
+ * substeps as the last dts says remain (as umwm_hip_substeps) before it
+ * reads sumt back, so the host waits once per run of substeps.
  */
 int umwm_hip_run_dtg(umwm_domain* dom, int32_t* nsub);
 

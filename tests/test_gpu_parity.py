@@ -220,3 +220,64 @@ def test_stress_ocn_needs_operator_sources():
     d.stress("ocn")
     assert np.all(np.isfinite(d.get("taux_ocntop")))
     d.close()
+
+
+def _fresh(wl, precision=8):
+    from umwm_amd import hip
+    d = hip.Domain(wl.params, wl.grid_input(), wl.forcing_input(0), precision=precision)
+    if wl.jonswap is not None:
+        d.init_jonswap(wl.jonswap)
+    d.stokes_drift(True)
+    return d
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_substeps_back_to_back_equal_single_substeps(precision):
+    """umwm_hip_substeps(n) launches n substeps with one host wait: the
+    spectrum, sumt and substep count equal n single substeps bit for bit;
+    running past the end of the dtg does nothing more."""
+    wl = tiny_case("winds_currents", seed=3)
+    a, b = _fresh(wl, precision), _fresh(wl, precision)
+    try:
+        sa = [a.substep() for _ in range(6)]
+        sumt, dts, n = b.substeps(6)
+        assert n == 6 and sumt == sa[-1][0] and dts == sa[-1][1]
+        assert np.array_equal(a.get("e"), b.get("e"))
+        assert np.array_equal(a.get("ustar"), b.get("ustar"))
+        # to the end of the hour and beyond: extra substeps are no-ops
+        while a.scalar("sumt") < wl.params.dtg:
+            a.substep()
+        n_hour = int(a.scalar("nsub"))
+        sumt, _, n = b.substeps(n_hour - 6 + 5)
+        assert n == n_hour and sumt == wl.params.dtg
+        assert np.array_equal(a.get("e"), b.get("e"))
+    finally:
+        a.close()
+        b.close()
+
+
+def test_run_dtg_runs_ahead_like_single_substeps():
+    """run_dtg launches runs of substeps between host waits; its hours match
+    hours of single substeps bit for bit, substep counts included."""
+    wl = tiny_case("land_shelf", seed=11)
+    a, b = _fresh(wl), _fresh(wl)
+    try:
+        for hour in range(2):
+            f = wl.forcing_input(hour) if hour else None
+            if f is not None:
+                a.forcinginput(f)
+                b.forcinginput(f)
+            n_b = b.run_dtg()
+            a.set_scalar("sumt", 0.0)
+            n_a = 0
+            while a.scalar("sumt") < wl.params.dtg:
+                a.substep()
+                n_a += 1
+            a.stokes_drift(False)
+            a.diag()
+            assert n_a == n_b
+            assert np.array_equal(a.get("e"), b.get("e"))
+            assert np.array_equal(a.get("ht"), b.get("ht"))
+    finally:
+        a.close()
+        b.close()

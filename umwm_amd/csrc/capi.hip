@@ -541,18 +541,34 @@ int fused_substep(DomainImpl<T>& D) {
   return 0;
 }
 
+// n substeps back to back; the scalars are read once, after the last.
+template <typename T>
+int substeps_impl(DomainImpl<T>& D, int n) {
+  for (int k = 0; k < n; ++k)
+    if (int rc = fused_substep(D)) return rc;
+  return read_scalars(D);
+}
+
 template <typename T>
 int run_dtg_impl(DomainImpl<T>& D, int32_t* nsub) {
   hipLaunchKernelGGL(k_begin_dtg, dim3(1), dim3(64), 0, D.stream, D.h.sc);
   UMWM_HIP_TRY(hipGetLastError());
+  const double dtg = D.S.P.dtg;
+  double sumt = 0, dts = 0;
   int n = 0, rc;
   for (;;) {
-    if ((rc = fused_substep(D))) return rc;
-    ++n;
-    if ((rc = read_scalars(D))) return rc;
-    if (!(D.sc_host->sumt < D.S.P.dtg)) break;
+    // run ahead by the substeps the last dts says are left (at least one,
+    // at most 64); those past the end of the dtg do nothing
+    int ahead = 1;
+    if (dts > 0) ahead = (int)std::min(64.0, std::max(1.0, std::ceil((dtg - sumt) / dts)));
+    if ((rc = substeps_impl(D, ahead))) return rc;
+    n += ahead;
+    sumt = D.sc_host->sumt;
+    dts = D.sc_host->dts;
+    if (!(sumt < dtg)) break;
     if (n > 10000000) return fail(UMWM_ERR_STATE, "run_dtg: substep count runaway");
   }
+  n = D.sc_host->nsub;
   if (D.firstdtg) D.firstdtg = false;
   if (D.S.P.stokes && (rc = launch_stokes(D))) return rc;
   if ((rc = launch_diag(D))) return rc;
@@ -1055,6 +1071,17 @@ int umwm_hip_substep(umwm_domain* dom, double* sumt, double* dts) {
     if (rc) return rc;
     if (sumt) *sumt = D.sc_host->sumt;
     if (dts) *dts = D.sc_host->dts;
+    return 0;
+  });
+}
+
+int umwm_hip_substeps(umwm_domain* dom, int32_t n, double* sumt, double* dts, int32_t* nsub) {
+  if (n < 1) return fail(UMWM_ERR_ARGUMENT, "substeps: n must be >= 1");
+  return dispatch(dom, [&](auto& D) {
+    if (int rc = substeps_impl(D, n)) return rc;
+    if (sumt) *sumt = D.sc_host->sumt;
+    if (dts) *dts = D.sc_host->dts;
+    if (nsub) *nsub = D.sc_host->nsub;
     return 0;
   });
 }

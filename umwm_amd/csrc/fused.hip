@@ -35,7 +35,7 @@
 // snl(o) = bf1r(o) X(o+1) + bf2r(o) X(o+2) - snl_fac X(o), X = sds e
 // (src/umwm_source_functions.F90:165-171) is folded into the integration
 // as each X becomes known: k_integrate keeps per direction only the
-// partial E_s of the two previous frequencies and their E^n.
+// partial E* = (E^n + partial E_s)/2 of the two previous frequencies.
 //
 // Wind input per bin (src/umwm_source_functions.F90:38-80) is evaluated
 // as y = |x| x times one of three per-frequency factors (sheltering for
@@ -211,10 +211,10 @@ __device__ __forceinline__ rsrc_t freq_rsrc(const T* base, int o, int rows, size
 
 // First N slots of the record of (o, j); vr = j * FT_N * sizeof(T).
 template <int N, typename T>
-__device__ __forceinline__ void load_tab(const DomainDev<T>& d, int o, uint32_t vr,
+__device__ __forceinline__ void load_tab(const T* ftab, size_t ldi, int o, uint32_t vr,
                                          FreqTab<T>& r) {
   T tmp[N];
-  bldn<T, N>(freq_rsrc(d.ftab, o, FT_N, d.ldi), vr, tmp);
+  bldn<T, N>(freq_rsrc(ftab, o, FT_N, ldi), vr, tmp);
 #pragma unroll
   for (int i = 0; i < N; ++i) r.v[i] = tmp[i];
 #pragma unroll
@@ -278,6 +278,14 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
   const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
   const uint32_t vr = (uint32_t)j * (uint32_t)(FT_N * sizeof(T));
   const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);  // bytes of one direction row
+  // Arrays the frequency loop uses, held in registers: buffer stores are
+  // opaque to the compiler, which would otherwise reload these pointers
+  // from the domain record after every store.
+  const T* const eb = d.e;
+  T* const pfxb = d.pfx;
+  const T* const ftab = d.ftab;
+  const T* const fr = d.f;
+  const T* const dir4 = d.dir4;
   double pts = INFINITY, rotmax = -INFINITY;
 
   if (active) {
@@ -300,12 +308,17 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     const T tpsf = T(d.twopisds_fac), spow = T(d.sds_power), mssf = T(d.mss_fac);
     const T nu4 = T(4 * d.nu_water), ig = T(1 / d.g);
     const double hdth = 0.5 * d.dth;
-    // overrun factor per direction of this wave (frequency independent)
+    // overrun factor per direction of this wave (frequency independent).
+    // Slots q >= nd are computed but never used: their spectrum loads go
+    // through a zero-range descriptor and read 0, and they are masked out
+    // of the rate maximum.  With no per-slot branch the compiler can
+    // interleave the directions.
+    const rsrc_t rz = make_rsrc(eb, 0u);
     T mq[DMAX];
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) {
-      T ct = 0, st = 0, c2 = 0, s2 = 0;
-      if (q < nd) cld4(d.dir4, p0 + q, ct, st, c2, s2);
+      T ct, st, c2, s2;
+      cld4(dir4, q < nd ? p0 + q : pm - 1, ct, st, c2, s2);
       mq[q] = s.cw * ct + s.sw * st > T(0) ? fs2 : T(1);
     }
     double A0 = 0, AC = 0, AS = 0;
@@ -313,28 +326,28 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     T ec[DMAX];
     FreqTab<T> tc;
     {
-      const rsrc_t ep = freq_rsrc(d.e, 0, pm, ldi);
+      const rsrc_t ep = freq_rsrc(eb, 0, pm, ldi);
 #pragma unroll
-      for (int q = 0; q < DMAX; ++q) ec[q] = (q < nd) ? bld<T>(ep, vj, (p0 + q) * rb) : T(0);
+      for (int q = 0; q < DMAX; ++q) ec[q] = bld<T>(q < nd ? ep : rz, vj, (p0 + q) * rb);
     }
     // FP32 prefetches the next frequency's table record with its spectrum;
     // FP64 (library exp/log, many live registers) loads it when needed
     constexpr bool kPrefetchTab = sizeof(T) == 4;
-    if constexpr (kPrefetchTab) load_tab<kRatesSlots>(d, 0, vr, tc);
+    if constexpr (kPrefetchTab) load_tab<kRatesSlots>(ftab, ldi, 0, vr, tc);
     for (int o = 0; o < om; ++o) {
       // issue the next frequency's loads before computing this one
       const int on = o + 1 < om ? o + 1 : o;
       T en[DMAX];
       FreqTab<T> tn;
-      const rsrc_t epn = freq_rsrc(d.e, on, pm, ldi);
+      const rsrc_t epn = freq_rsrc(eb, on, pm, ldi);
 #pragma unroll
-      for (int q = 0; q < DMAX; ++q) en[q] = (q < nd) ? bld<T>(epn, vj, (p0 + q) * rb) : T(0);
+      for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(q < nd ? epn : rz, vj, (p0 + q) * rb);
       if constexpr (kPrefetchTab)
-        load_tab<kRatesSlots>(d, on, vr, tn);
+        load_tab<kRatesSlots>(ftab, ldi, on, vr, tn);
       else
-        load_tab<kRatesSlots>(d, o, vr, tc);
+        load_tab<kRatesSlots>(ftab, ldi, o, vr, tc);
 
-      const T k = tc[FT_K], k2 = k * k, fo = cld(d.f, o);
+      const T k = tc[FT_K], k2 = k * k, fo = cld(fr, o);
       const T sdt = s.csdt * k;
       const T sice = T(2) * tc[FT_CG0] * s.ice;
       const T crest = tc[FT_SBF] + sdt + nu4 * k2 - sice;  // sbf + sdt + sdv - sice
@@ -342,10 +355,10 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       const T pa0 = T(hdth * A0), pac = T(hdth * AC), pas = T(hdth * AS);
       if (valid && w == 0) {
         if constexpr (sizeof(T) == 4) {
-          bst16(freq_rsrc(d.pfx, o, 4, ldi), (uint32_t)j * 16u,
+          bst16(freq_rsrc(pfxb, o, 4, ldi), (uint32_t)j * 16u,
                 typename Vec16<T>::type{pa0, pac, pas, T(0)});
         } else {
-          const rsrc_t pf = freq_rsrc(d.pfx, o, 4, ldi);
+          const rsrc_t pf = freq_rsrc(pfxb, o, 4, ldi);
           bst16(pf, (uint32_t)j * 32u, typename Vec16<T>::type{pa0, pac});
           bst16(pf, (uint32_t)j * 32u + 16u, typename Vec16<T>::type{pas, T(0)});
         }
@@ -358,20 +371,18 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       T m0 = 0, mc = 0, ms = 0, rq = 0;
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) {
-        if (q < nd) {
-          T ct, st, c2, s2;
-          cld4(d.dir4, p0 + q, ct, st, c2, s2);
-          const T e = ec[q];
-          const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
-          T dm = fma(qc, c2, fma(qs, s2, q0));
-          dm = dm * dm;
-          const T sds = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fmax(e, T(0))), Lk));
-          const T r = ss - fma(sds, cs, crest);
-          rq = fmax(rq, fabs(r));
-          m0 += e;
-          mc = fma(e, c2, mc);
-          ms = fma(e, s2, ms);
-        }
+        T ct, st, c2, s2;
+        cld4(dir4, q < nd ? p0 + q : pm - 1, ct, st, c2, s2);
+        const T e = ec[q];
+        const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
+        T dm = fma(qc, c2, fma(qs, s2, q0));
+        dm = dm * dm;
+        const T sds = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fmax(e, T(0))), Lk));
+        const T r = ss - fma(sds, cs, crest);
+        rq = fmax(rq, q < nd ? fabs(r) : T(0));
+        m0 += e;
+        mc = fma(e, c2, mc);
+        ms = fma(e, s2, ms);
       }
       if (o <= oc) rmax = fmax(rmax, rq);  // rates of prognostic bins only
       if (rotation) {
@@ -499,8 +510,8 @@ struct Prefix {
 };
 
 template <typename T>
-__device__ __forceinline__ Prefix<T> load_prefix(const DomainDev<T>& d, int o, int j) {
-  const rsrc_t pf = freq_rsrc(d.pfx, o, 4, d.ldi);
+__device__ __forceinline__ Prefix<T> load_prefix(const T* pfx, size_t ldi, int o, int j) {
+  const rsrc_t pf = freq_rsrc(pfx, o, 4, ldi);
   Prefix<T> r;
   if constexpr (sizeof(T) == 4) {
     const auto v = bld16<T>(pf, (uint32_t)j * 16u);
@@ -535,6 +546,12 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
   const uint32_t vr = (uint32_t)j * (uint32_t)(FT_N * sizeof(T));
   const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);
+  // Arrays the frequency loop uses, held in registers (see k_rates).
+  const T* const eb = d.e;
+  T* const esb = d.es;
+  const T* const ftab = d.ftab;
+  const T* const pfxb = d.pfx;
+  const T* const fr = d.f;
 
   const PointState<T> s = point_stored(d, j);
   const int oc = s.oc;
@@ -542,48 +559,59 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   const T tpsf = T(d.twopisds_fac), spow = T(d.sds_power), ipow = T(d.inv_sds_power);
   const T mssf = T(d.mss_fac), dsnl = dts * T(d.snl_fac);
   const T nu4 = T(4 * d.nu_water), ig = T(1 / d.g);
-  T mq[DMAX];
+  // direction constants of this wave, loaded once.  Slots q >= nd are
+  // computed but never used: their loads and stores go through a
+  // zero-range descriptor (loads read 0, stores are dropped), so the
+  // direction loop has no per-slot branch and the compiler can interleave
+  // the directions.
+  const rsrc_t rz = make_rsrc(eb, 0u);
+  T mq[DMAX], ctq[DMAX], stq[DMAX], c2q[DMAX], s2q[DMAX];
 #pragma unroll
   for (int q = 0; q < DMAX; ++q) {
-    T ct = 0, st = 0, c2 = 0, s2 = 0;
-    if (q < nd) cld4(d.dir4, p0 + q, ct, st, c2, s2);
+    T ct, st, c2, s2;
+    cld4(d.dir4, q < nd ? p0 + q : pm - 1, ct, st, c2, s2);
     mq[q] = s.cw * ct + s.sw * st > T(0) ? fs2 : T(1);
+    ctq[q] = ct;
+    stq[q] = st;
+    c2q[q] = c2;
+    s2q[q] = s2;
   }
-  // per direction: partial E_s and E^n of the two previous frequencies
-  T G1[DMAX], G2[DMAX], E1[DMAX], E2[DMAX];
+  // per direction, the partial E* = (E^n + partial E_s) / 2 of the two
+  // previous frequencies, still waiting for their snl gains
+  T H1[DMAX], H2[DMAX];
 #pragma unroll
-  for (int q = 0; q < DMAX; ++q) G1[q] = G2[q] = E1[q] = E2[q] = 0;
+  for (int q = 0; q < DMAX; ++q) H1[q] = H2[q] = 0;
   T b1m1 = 0, b2m2 = 0, b2m1 = 0;  // bf1r(o-1), bf2r(o-2), bf2r(o-1)
   T ec[DMAX];
   {
-    const rsrc_t ep = freq_rsrc(d.e, 0, pm, ldi);
+    const rsrc_t ep = freq_rsrc(eb, 0, pm, ldi);
 #pragma unroll
-    for (int q = 0; q < DMAX; ++q) ec[q] = (q < nd) ? bld<T>(ep, vj, (p0 + q) * rb) : T(0);
+    for (int q = 0; q < DMAX; ++q) ec[q] = bld<T>(q < nd ? ep : rz, vj, (p0 + q) * rb);
   }
   FreqTab<T> tc;
   Prefix<T> pc{};
   constexpr bool kPrefetchTab = sizeof(T) == 4;  // as in k_rates
   if constexpr (kPrefetchTab) {
-    load_tab<kIntegrateSlots>(d, 0, vr, tc);
-    pc = load_prefix(d, 0, j);
+    load_tab<kIntegrateSlots>(ftab, ldi, 0, vr, tc);
+    pc = load_prefix(pfxb, ldi, 0, j);
   }
   for (int o = 0; o < om; ++o) {
     const int on = o + 1 < om ? o + 1 : o;
     T en[DMAX];
-    const rsrc_t epn = freq_rsrc(d.e, on, pm, ldi);
+    const rsrc_t epn = freq_rsrc(eb, on, pm, ldi);
 #pragma unroll
-    for (int q = 0; q < DMAX; ++q) en[q] = (q < nd) ? bld<T>(epn, vj, (p0 + q) * rb) : T(0);
+    for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(q < nd ? epn : rz, vj, (p0 + q) * rb);
     FreqTab<T> tn;
     Prefix<T> pn{};
     if constexpr (kPrefetchTab) {
-      load_tab<kIntegrateSlots>(d, on, vr, tn);
-      pn = load_prefix(d, on, j);
+      load_tab<kIntegrateSlots>(ftab, ldi, on, vr, tn);
+      pn = load_prefix(pfxb, ldi, on, j);
     } else {
-      load_tab<kIntegrateSlots>(d, o, vr, tc);
-      pc = load_prefix(d, o, j);
+      load_tab<kIntegrateSlots>(ftab, ldi, o, vr, tc);
+      pc = load_prefix(pfxb, ldi, o, j);
     }
 
-    const T k = tc[FT_K], fo = cld(d.f, o);
+    const T k = tc[FT_K], fo = cld(fr, o);
     const T sdt = s.csdt * k;
     const T sice = T(2) * tc[FT_CG0] * s.ice;
     const T ctail = sdt + nu4 * (k * k) - sice;  // sdt + sdv - sice
@@ -599,51 +627,63 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     const bool prog = o <= oc;
     const bool in1 = (o >= 1) && (o - 1 <= oc);  // bin o-1 is prognostic
     const bool fin = valid && (o >= 2) && (o - 2 <= oc);  // bin o-2 completes now
-    const T db1 = in1 ? dts * b1m1 : T(0), db2 = dts * b2m2;
+    const T hdb1 = in1 ? T(0.5) * dts * b1m1 : T(0), hdb2 = T(0.5) * dts * b2m2;
     const uint32_t vtail = (valid && !prog) ? vj : kDropOffset;
     const uint32_t vfin = fin ? vj : kDropOffset;
     const bool anyfin = __any(fin), anytail = __any(valid && !prog);
-    const rsrc_t rs0 = freq_rsrc(d.es, o, pm, ldi);
-    const rsrc_t rs2 = freq_rsrc(d.es, o >= 2 ? o - 2 : 0, pm, ldi);
+    const rsrc_t rs0 = freq_rsrc(esb, o, pm, ldi);
+    const rsrc_t rs2 = freq_rsrc(esb, o >= 2 ? o - 2 : 0, pm, ldi);
+    // One pass over this wave's directions; PROG / TAIL / FIN say whether
+    // any lane of the wave integrates a prognostic bin, fills a tail bin
+    // or completes bin o-2 at this frequency (wave-uniform, so each
+    // combination is a straight-line loop).
+    auto sweep = [&](auto PROG, auto TAIL, auto FIN) {
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) {
-        if (q < nd) {
-          const int p = p0 + q;
-          T ct, st, c2, s2;
-          cld4(d.dir4, p, ct, st, c2, s2);
-          const T e = ec[q];
-          const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
-          T dm = fma(qc, c2, fma(qs, s2, q0));
-          dm = dm * dm;
-          const T sdsp = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fmax(e, T(0))), Lk));
-          const T X = sdsp * e;
-          T G = 0;
-          if (prog) {
-            // E_s = e exp(dts (ssin - sds - sbf - sdt - sdv + sice)) + dts snl
-            const T a = fma(-dtc, sdsp, fma(dtx, ss, c0));
-            G = fma(-dsnl, X, e * Tr<T>::ex(a));
-          }
-          if (anytail) {
-            // diagnostic tail, src/umwm_physics.F90:70-80
-            T es_ = 0;
-            if (!prog) {
-              const T num = fmax(ss, T(0)) - ctail;
-              const T t = fmax(num, T(0)) * (icfc * Tr<T>::rcp(dm));
-              es_ = Tr<T>::ex(fma(ipow, Tr<T>::lg(t), -lk4));  // k^-4 t^(1/p)
-            }
-            bst(rs0, vtail, T(0.5) * (e + es_), p * rb);
-          }
-          const T g1 = fma(db1, X, G1[q]);  // bf1r(o-1) X(o) into bin o-1
-          if (anyfin) {
-            const T es_ = fma(db2, X, G2[q]);  // bf2r(o-2) X(o) completes bin o-2
-            bst(rs2, vfin, T(0.5) * (E2[q] + es_), p * rb);
-          }
-          G2[q] = g1;
-          G1[q] = G;
-          E2[q] = E1[q];
-          E1[q] = e;
+        const uint32_t so = (uint32_t)(p0 + q) * rb;
+        const T ct = ctq[q], st = stq[q], c2 = c2q[q], s2 = s2q[q];
+        const T e = ec[q];
+        const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
+        T dm = fma(qc, c2, fma(qs, s2, q0));
+        dm = dm * dm;
+        const T sdsp = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fmax(e, T(0))), Lk));
+        const T X = sdsp * e;
+        T G = 0;
+        if constexpr (decltype(PROG)::value) {
+          // E_s = e exp(dts (ssin - sds - sbf - sdt - sdv + sice)) + dts snl
+          const T a = fma(-dtc, sdsp, fma(dtx, ss, c0));
+          G = prog ? fma(-dsnl, X, e * Tr<T>::ex(a)) : T(0);
         }
+        if constexpr (decltype(TAIL)::value) {
+          // diagnostic tail, src/umwm_physics.F90:70-80
+          const T num = fmax(ss, T(0)) - ctail;
+          const T t = fmax(num, T(0)) * (icfc * Tr<T>::rcp(dm));
+          const T es_ = Tr<T>::ex(fma(ipow, Tr<T>::lg(t), -lk4));  // k^-4 t^(1/p)
+          bst(q < nd ? rs0 : rz, vtail, T(0.5) * (e + es_), so);
+        }
+        const T h1 = fma(hdb1, X, H1[q]);  // bf1r(o-1) X(o) into bin o-1
+        if constexpr (decltype(FIN)::value)  // bf2r(o-2) X(o) completes bin o-2
+          bst(q < nd ? rs2 : rz, vfin, fma(hdb2, X, H2[q]), so);
+        H2[q] = h1;
+        H1[q] = T(0.5) * (e + G);
       }
+    };
+    using Y = std::true_type;
+    using N = std::false_type;
+    const bool anyprog = __any(valid && prog);
+    if (anyprog) {
+      if (anytail) {
+        if (anyfin) sweep(Y{}, Y{}, Y{}); else sweep(Y{}, Y{}, N{});
+      } else {
+        if (anyfin) sweep(Y{}, N{}, Y{}); else sweep(Y{}, N{}, N{});
+      }
+    } else {
+      if (anytail) {
+        if (anyfin) sweep(N{}, Y{}, Y{}); else sweep(N{}, Y{}, N{});
+      } else {
+        if (anyfin) sweep(N{}, N{}, Y{}); else sweep(N{}, N{}, N{});
+      }
+    }
     b2m2 = b2m1;
     b2m1 = tc[FT_B2];
     b1m1 = tc[FT_B1];
@@ -705,6 +745,9 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
                                  bld<T>(row_rsrc(d.fkovg, o, ldi), vj), sin_diss1, fs2);
   const rsrc_t pe = freq_rsrc(d.e, o, pm, ldi);    // E^n in, E^{n+1} out
   const rsrc_t ps = freq_rsrc(d.es, o, pm, ldi);   // E*
+  // direction tables in registers (see k_rates)
+  const T* const cthb = d.cth;
+  const T* const sthb = d.sth;
   // stress_vector_integral partial sums over the directions of this
   // frequency (FP32 within the frequency in the FP32 path, FP64 across)
   T tx = 0, ty = 0, hs = 0;
@@ -727,7 +770,7 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
         bst(pe, vst, v[q], (pb + q) * rb);
-        stress_acc(cld(d.cth, pb + q), cld(d.sth, pb + q), v[q]);
+        stress_acc(cld(cthb, pb + q), cld(sthb, pb + q), v[q]);
       }
     }
   } else {
@@ -773,7 +816,7 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
     const T As = T(2) * (cgi + bld<T>(cgr, vis)) * dxs, An = T(2) * (cgi + bld<T>(cgr, vin)) * dxn;
     const T ccu = pload(d.ccurv, ldi, j), scu = pload(d.scurv, ldi, j);
     const T kr = dtr * T(d.oneovdth);
-    auto rotp = [&](int p) { return fma(rx, cld(d.sth, p), fma(ry, cld(d.cth, p), r0)); };
+    auto rotp = [&](int p) { return fma(rx, cld(sthb, p), fma(ry, cld(cthb, p), r0)); };
     auto load_block = [&](int pb, AdvLoads<T>* L) {
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
@@ -800,7 +843,7 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
         const T enext = q + 1 < NB ? A[(q + 1) < NB ? q + 1 : 0].eo : (more ? B[0].eo : efirst);
         const int pn = p + 1 < pm ? p + 1 : 0;
         const T rnext = rotp(pn);
-        const T ct = cld(d.cth, p), st = cld(d.sth, p);
+        const T ct = cld(cthb, p), st = cld(sthb, p);
         const T cc = fma(ct, ccu, -st * scu);  // cos(th + curv)
         const T sn = fma(st, ccu, ct * scu);   // sin(th + curv)
         const T e0 = A[q].eo, ee = A[q].ee, ew = A[q].ew, en = A[q].en, es = A[q].es;

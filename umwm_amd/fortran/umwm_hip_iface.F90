@@ -96,6 +96,15 @@ module umwm_hip_capi
       real(c_double), intent(out) :: sumt, dts
       integer(c_int) :: rc
     end function
+    function umwm_hip_substeps(dom, n, sumt, dts, nsub) result(rc) &
+        bind(C, name='umwm_hip_substeps')
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: dom
+      integer(c_int32_t), value :: n
+      real(c_double), intent(out) :: sumt, dts
+      integer(c_int32_t), intent(out) :: nsub
+      integer(c_int) :: rc
+    end function
     function umwm_hip_run_dtg(dom, nsub) result(rc) bind(C, name='umwm_hip_run_dtg')
       import :: c_int, c_ptr, c_int32_t
       type(c_ptr), value :: dom
