@@ -239,7 +239,12 @@ int umwm_hip_get_field(umwm_domain* dom, const char* name, double* buf,
 int umwm_hip_set_field(umwm_domain* dom, const char* name, const double* buf,
                        int64_t n);
 /* Scalars: sumt dts dta dtr dtamin cfllim cgmax mindelx dlnf dth bf1 bf2
- * first. */
+ * first nsub.  Settable: sumt dts dtr_temp first, and keep_sources: set to 1
+ * (it cannot be unset) before the substep that reaches dtg, it makes the
+ * fused path save that substep's E^n and incoming ustar (one more spectrum
+ * of device memory), so that stress("ocn") at output time can rebuild the
+ * ssin, sds and snl the reference's output step reads (src/umwm_top.F90:
+ * 160-170); the save costs one spectrum copy per dtg. */
 int umwm_hip_get_scalar(umwm_domain* dom, const char* name, double* v);
 int umwm_hip_set_scalar(umwm_domain* dom, const char* name, double v);
 /* Sizes in doubles of a named field (0 if unknown). */

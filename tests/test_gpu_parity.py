@@ -204,8 +204,8 @@ def test_fp32_tiny_tracks_fp64(case, oracle_lib):
 
 def test_stress_ocn_needs_operator_sources():
     """After a fused substep the source arrays of stress('ocn') were never
-    materialised: the call reports a state error instead of reading stale
-    arrays; after an operator-path substep it works."""
+    materialised: without keep_sources the call reports a state error
+    instead of reading stale arrays."""
     from umwm_amd import hip
     wl = tiny_case("land_shelf")
     d = hip.Domain(wl.params, wl.grid_input(), wl.forcing_input(0), precision=8)
@@ -213,13 +213,33 @@ def test_stress_ocn_needs_operator_sources():
     d.substep()
     with pytest.raises(hip.UmwmError, match="stress\\('ocn'\\)"):
         d.stress("ocn")
-    for op in ("forcinginterpolate", "sin_d12", "sds_d12", "snl_d12", "s_ice", "source",
-               "propagation", "refraction", "update"):
-        getattr(d, op)()
-    d.stress("atm")
-    d.stress("ocn")
-    assert np.all(np.isfinite(d.get("taux_ocntop")))
+    # keep_sources set mid-dtg: nothing is saved until the dtg ends
+    d.set_scalar("keep_sources", 1)
+    with pytest.raises(hip.UmwmError, match="keep_sources"):
+        d.stress("ocn")
     d.close()
+
+
+@pytest.mark.parametrize("case", ["land_shelf", "winds_currents"])
+def test_stress_ocn_after_fused_hour(case, oracle_lib):
+    """With keep_sources, the fused run_dtg saves E^n and ustar of the
+    substep that reaches dtg; stress('ocn') at the hour end then equals the
+    oracle's output-time stress('ocn') after the same hour (FP64)."""
+    wl = tiny_case(case, seed=9)
+    o, d = make_pair(wl)
+    d.set_scalar("keep_sources", 1)
+    for hour in range(2):
+        if hour and wl.snapshots is not None:  # the next level (make_pair loaded hour 1)
+            o.forcinginput(wl.forcing_input(hour + 1))
+            d.forcinginput(wl.forcing_input(hour + 1))
+        assert o.run_dtg() == d.run_dtg()
+        # forcing as the last active substep interpolated it (substeps run
+        # ahead past the dtg must not re-interpolate it at sumt = dtg)
+        _op_compare(o, d, ["wspd", "wdir", "uc", "vc"], TOL_RUN, f"{case} hour {hour}")
+        o.stress_ocn()
+        d.stress("ocn")
+        _vector_compare(o, d, OCN_PAIRS, TOL_RUN, f"{case} hour {hour}")
+        _op_compare(o, d, ["ssin", "sds", "snl"], TOL_RUN, f"{case} hour {hour}")
 
 
 def _fresh(wl, precision=8):

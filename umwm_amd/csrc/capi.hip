@@ -515,13 +515,53 @@ bool any_forcing(const DomainImpl<T>& D) {
   return P.winds || P.currents || P.air_density || P.water_density || P.seaice;
 }
 
+// Runs sin_d12, sds_d12 and snl_d12 (operator kernels) on the E^n and
+// ustar that k_keep saved from the substep that reached dtg, leaving ssin,
+// sds (after cothkd), snl and sdt as that substep's source evaluation left
+// them (src/umwm_top.F90:95-98); the forcing and the cut-off have not
+// changed since.
+template <typename T>
+int materialize_sources(DomainImpl<T>& D) {
+  T* const e = D.h.e;
+  double* const ust = D.h.ustar;
+  D.h.e = D.h.e_keep;
+  D.h.ustar = D.h.ustar_keep;
+  int rc = launch_op_sin(D);
+  if (!rc) rc = launch_op_sds(D);
+  if (!rc) rc = launch_op_snl(D);
+  D.h.e = e;
+  D.h.ustar = ust;
+  return rc;
+}
+
+// keep_sources: allocates the E^n / ustar save of the substep reaching dtg
+template <typename T>
+int enable_keep(DomainImpl<T>& D) {
+  if (!D.h.e_keep) {
+    const size_t ns = (size_t)D.h.om * D.h.pm * D.h.ldi;
+    D.h.e_keep = D.arena.template alloc<T>(ns);
+    D.h.ustar_keep = D.arena.template alloc<double>(D.h.ldi);
+    if (!D.h.e_keep || !D.h.ustar_keep) {
+      D.h.e_keep = nullptr;
+      return fail(UMWM_ERR_NOMEM, "device allocation of the kept sources failed");
+    }
+    UMWM_HIP_TRY(hipMemcpyAsync(D.d, &D.h, sizeof(DomainDev<T>), hipMemcpyHostToDevice, D.stream));
+  }
+  // nothing is saved yet: clear the flag of an earlier dtg end
+  if (int rc = read_scalars(D)) return rc;
+  Scalars s = *D.sc_host;
+  s.last = 0;
+  UMWM_HIP_TRY(hipMemcpy(D.h.sc, &s, sizeof(Scalars), hipMemcpyHostToDevice));
+  return 0;
+}
+
 template <typename T>
 int fused_substep(DomainImpl<T>& D) {
   int rc;
-  if (any_forcing(D) && (rc = launch_forcinginterpolate(D))) return rc;
+  if (any_forcing(D) && (rc = launch_forcinginterpolate(D, true))) return rc;
   const int rotation = (D.S.P.currents || D.first || !D.rot_valid) ? 1 : 0;
   rc = launch_fused<T>(D.d, D.tasks, D.ntasks, D.h.om, D.first ? 1 : 0, rotation, D.tile,
-                       D.stream, D.timing ? D.ev : nullptr);
+                       D.h.e_keep != nullptr, D.stream, D.timing ? D.ev : nullptr);
   if (rc) return rc;
   D.ef_mirrors_e = true;
   D.rot_valid = true;
@@ -956,7 +996,7 @@ int umwm_hip_stokes_drift(umwm_domain* dom, int init) {
 
 int umwm_hip_forcinginterpolate(umwm_domain* dom) {
   return dispatch(dom, [&](auto& D) {
-    int rc = launch_forcinginterpolate(D);
+    int rc = launch_forcinginterpolate(D, false);
     return rc ? rc : sync_and_check(D);
   });
 }
@@ -1039,11 +1079,24 @@ int umwm_hip_stress(umwm_domain* dom, const char* option) {
   if (opt != "atm" && opt != "ocn")
     return fail(UMWM_ERR_ARGUMENT, "stress option must be 'atm' or 'ocn'");
   return dispatch(dom, [&](auto& D) {
-    if (opt == "ocn" && (D.ef_mirrors_e || !D.ops_ready))
+    int rc = 0;
+    if (opt == "ocn" && D.ef_mirrors_e) {
+      // after fused substeps: rebuild the sources of the substep that
+      // reached dtg from its saved E^n and incoming ustar
+      if (D.h.e_keep && (rc = read_scalars(D))) return rc;
+      if (!D.h.e_keep || !D.sc_host->last)
+        return fail(UMWM_ERR_STATE,
+                    "stress('ocn') reads ssin, sds and snl of the last source evaluation: "
+                    "after fused substeps it needs keep_sources set before the substep "
+                    "that reaches dtg, or run that substep through the operator entry points");
+      if ((rc = ensure_ops(D))) return rc;
+      if ((rc = materialize_sources(D))) return rc;
+    } else if (opt == "ocn" && !D.ops_ready) {
       return fail(UMWM_ERR_STATE,
                   "stress('ocn') reads ssin, sds and snl of the last source evaluation: "
-                  "run that substep through the operator entry points");
-    int rc = ensure_ops(D);
+                  "no source terms have been evaluated yet");
+    }
+    rc = ensure_ops(D);
     if (!rc) rc = opt == "atm" ? launch_op_stress_atm(D) : launch_op_stress_ocn(D);
     if (!rc) rc = sync_and_check(D);
     return rc;
@@ -1146,6 +1199,7 @@ int umwm_hip_set_scalar(umwm_domain* dom, const char* name, double v) {
   return dispatch(dom, [&](auto& D) {
     const std::string n(name);
     if (n == "first") { D.first = v != 0; return 0; }
+    if (n == "keep_sources") return v != 0 ? enable_keep(D) : 0;
     int rc = read_scalars(D);
     if (rc) return rc;
     Scalars s = *D.sc_host;
@@ -1290,13 +1344,18 @@ int on_stream(DomainImpl<T>& D, hipStream_t s, int (*fn)(DomainImpl<T>&)) {
 }
 
 template <typename T>
+int launch_forcing_gated(DomainImpl<T>& D) {
+  return launch_forcinginterpolate(D, true);
+}
+
+template <typename T>
 int batch_launch_substep(BatchImpl<T>& B, int first, int rotation) {
   for (auto* D : B.doms)
     if (any_forcing(*D))
-      if (int rc = on_stream(*D, B.stream, launch_forcinginterpolate<T>)) return rc;
+      if (int rc = on_stream(*D, B.stream, launch_forcing_gated<T>)) return rc;
   for (const auto& g : B.groups)
     if (int rc = launch_fused<T>(B.d_doms, g.tasks, g.ntasks, g.omax, first, rotation, g.tile,
-                                 B.stream, nullptr))
+                                 false, B.stream, nullptr))
       return rc;
   for (auto* D : B.doms) D->ef_mirrors_e = true;
   const int n = (int)B.doms.size();

@@ -46,7 +46,7 @@ struct Scalars {
   uint32_t counter;  // arrival counter of the rates kernel's last block
   uint32_t counter_rot;
   int32_t anycur;    // any current != 0 (zerocurrents = !anycur)
-  int32_t pad;
+  int32_t last;      // the latest active substep reached sumt >= dtg
 };
 
 // Everything a kernel needs to know about one domain.
@@ -84,6 +84,11 @@ struct DomainDev {
   double *pts, *hs_ice;
   // stress('atm') outputs, FP64 in both precisions
   double *ustar, *cd, *taux, *tauy, *taux_form, *tauy_form, *taux_skin;
+  // when set, the substep that reaches dtg saves its E^n and its incoming
+  // ustar here, from which stress('ocn') rebuilds that substep's source
+  // arrays (null: not kept)
+  T* e_keep;
+  double* ustar_keep;
   double *tauy_skin, *taux_diag, *tauy_diag, *tailatmx, *tailatmy;
   double *us1, *vs1;  // surface Stokes drift used by stress('atm')
   // fused-path copy of the (o, j) tables, [om][ldi][FT_N]: the FT_N values

@@ -89,7 +89,8 @@ struct DomainImpl : DomainBase {
 };
 
 // Kernel launchers (ops.hip, fused.hip).
-template <typename T> int launch_forcinginterpolate(DomainImpl<T>& D);
+// gate: skip once sumt >= dtg (fused substeps, which may run past the dtg)
+template <typename T> int launch_forcinginterpolate(DomainImpl<T>& D, bool gate);
 template <typename T> int launch_op_sin(DomainImpl<T>& D);
 template <typename T> int launch_op_sds(DomainImpl<T>& D);
 template <typename T> int launch_op_snl(DomainImpl<T>& D);
@@ -108,6 +109,6 @@ template <typename T> int launch_jonswap(DomainImpl<T>& D, const umwm_jonswap& j
 // Fused substep kernels over a task list that may span many domains.
 template <typename T>
 int launch_fused(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int omax,
-                 int first, int rotation, Tile tile, hipStream_t s, hipEvent_t* ev);
+                 int first, int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev);
 
 }  // namespace umwm

@@ -495,6 +495,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       }
       sc->dtr = fmin(sc->dtr_temp, dts);
       sc->nsub = sc->nsub + 1;
+      sc->last = sc->sumt >= d.dtg ? 1 : 0;
     }
     sc->active = active ? 1 : 0;
     sc->counter = 0;
@@ -883,6 +884,25 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
 }
 
 // ---------------------------------------------------------------------
+// k_keep: on the substep that reaches dtg (and only when the domain keeps
+// them), saves E^n and the incoming ustar, from which stress('ocn')
+// rebuilds that substep's ssin, sds and snl at output time.  Launched
+// between k_integrate (which reads E^n) and k_advect (which overwrites it).
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64) void k_keep(const DomainDev<T>* __restrict__ doms,
+                                             const BlockTask* __restrict__ tasks) {
+  const BlockTask tk = tasks[blockIdx.x];
+  const DomainDev<T>& d = doms[tk.dom];
+  if (!d.sc->active || !d.sc->last || !d.e_keep) return;
+  const int j = tk.i0 + (int)threadIdx.x;
+  if (j >= d.im) return;
+  const size_t ldi = d.ldi, rows = (size_t)d.om * d.pm;
+  for (size_t r = 0; r < rows; ++r) d.e_keep[r * ldi + j] = d.e[r * ldi + j];
+  d.ustar_keep[j] = d.ustar[j];
+}
+
+// ---------------------------------------------------------------------
 // k_stress: frequency sums + stress('atm') chain per point
 // ---------------------------------------------------------------------
 template <typename T>
@@ -944,7 +964,7 @@ __global__ __launch_bounds__(64) void k_stress(const DomainDev<T>* __restrict__ 
 
 template <typename T, int D1, int D2>
 int launch_all(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int omax, int first,
-               int rotation, Tile tile, hipStream_t s, hipEvent_t* ev) {
+               int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev) {
   const dim3 grid(ntasks);
   const size_t sh1 = tile.W1 > 1 ? (size_t)2 * 3 * tile.W1 * 64 * sizeof(double)
                                  : (size_t)64 * sizeof(double);
@@ -953,6 +973,7 @@ int launch_all(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int
                      rotation);
   if (ev) (void)hipEventRecord(ev[1], s);
   hipLaunchKernelGGL((k_integrate<T, D2>), grid, dim3(64 * tile.W2), 0, s, doms, tasks);
+  if (keep) hipLaunchKernelGGL((k_keep<T>), grid, dim3(64), 0, s, doms, tasks);
   if (ev) (void)hipEventRecord(ev[2], s);
   // advect: one wave per (64 points, frequency), FW frequencies per workgroup
   const int fw = 4;
@@ -971,14 +992,18 @@ int launch_all(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int
 
 template <typename T, int D1>
 int launch_d2(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int omax, int first,
-              int rotation, Tile tile, hipStream_t s, hipEvent_t* ev) {
+              int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev) {
+#define UMWM_D2(D2_)                                                                         \
+  case D2_:                                                                                  \
+    return launch_all<T, D1, D2_>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev);
   switch (tile.D2) {
-    case 3: return launch_all<T, D1, 3>(doms, tasks, ntasks, omax, first, rotation, tile, s, ev);
-    case 6: return launch_all<T, D1, 6>(doms, tasks, ntasks, omax, first, rotation, tile, s, ev);
-    case 9: return launch_all<T, D1, 9>(doms, tasks, ntasks, omax, first, rotation, tile, s, ev);
-    case 12: return launch_all<T, D1, 12>(doms, tasks, ntasks, omax, first, rotation, tile, s, ev);
-    case 18: return launch_all<T, D1, 18>(doms, tasks, ntasks, omax, first, rotation, tile, s, ev);
+    UMWM_D2(3)
+    UMWM_D2(6)
+    UMWM_D2(9)
+    UMWM_D2(12)
+    UMWM_D2(18)
   }
+#undef UMWM_D2
   return fail(UMWM_ERR_ARGUMENT, "unsupported integrate tile D");
 }
 
@@ -988,22 +1013,22 @@ bool tile_supported(int D) { return D == 3 || D == 6 || D == 9 || D == 12 || D =
 
 template <typename T>
 int launch_fused(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int omax,
-                 int first, int rotation, Tile tile, hipStream_t s, hipEvent_t* ev) {
+                 int first, int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev) {
   if (ntasks <= 0) return 0;
   if (tile.W1 < 1 || tile.W1 > 8 || tile.W2 < 1 || tile.W2 > 8)
     return fail(UMWM_ERR_ARGUMENT, "tile W must be 1..8");
   switch (tile.D1) {
-    case 6: return launch_d2<T, 6>(doms, tasks, ntasks, omax, first, rotation, tile, s, ev);
-    case 9: return launch_d2<T, 9>(doms, tasks, ntasks, omax, first, rotation, tile, s, ev);
-    case 18: return launch_d2<T, 18>(doms, tasks, ntasks, omax, first, rotation, tile, s, ev);
-    case 36: return launch_d2<T, 36>(doms, tasks, ntasks, omax, first, rotation, tile, s, ev);
+    case 6: return launch_d2<T, 6>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev);
+    case 9: return launch_d2<T, 9>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev);
+    case 18: return launch_d2<T, 18>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev);
+    case 36: return launch_d2<T, 36>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev);
   }
   return fail(UMWM_ERR_ARGUMENT, "unsupported rates tile D");
 }
 
 template int launch_fused<float>(const DomainDev<float>*, const BlockTask*, int, int, int, int,
-                                 Tile, hipStream_t, hipEvent_t*);
+                                 Tile, bool, hipStream_t, hipEvent_t*);
 template int launch_fused<double>(const DomainDev<double>*, const BlockTask*, int, int, int, int,
-                                  Tile, hipStream_t, hipEvent_t*);
+                                  Tile, bool, hipStream_t, hipEvent_t*);
 
 }  // namespace umwm

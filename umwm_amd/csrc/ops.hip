@@ -20,11 +20,15 @@ inline dim3 grid1(int n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 // ---------------------------------------------------------------------
 // src/umwm_forcing.F90:40-99 forcinginterpolate (gustiness = 0)
 // ---------------------------------------------------------------------
+// With `gate` (the fused substep) it does nothing once sumt >= dtg: a
+// substep launched past the end of the dtg leaves the forcing as the last
+// active substep interpolated it, as the reference's loop test would.
 template <typename T>
-__global__ void k_forcing(DomainDev<T> d, ForcingDev<T> F) {
+__global__ void k_forcing(DomainDev<T> d, ForcingDev<T> F, int gate) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.imm) return;
   const double s = d.sc->sumt, dtg = d.dtg;
+  if (gate && !(s < dtg)) return;
   const size_t q2 = (size_t)(F.ni[q] - 1) * d.mm + (F.mi[q] - 1);
   const T wb = T(1 - s / dtg), wf = T(s), tg = T(dtg);
   if (F.winds) {
@@ -735,8 +739,8 @@ __global__ void k_jonswap(DomainDev<T> d, umwm_jonswap js, double dlnf) {
   } while (0)
 
 template <typename T>
-int launch_forcinginterpolate(DomainImpl<T>& D) {
-  LAUNCH(k_forcing<T>, grid1(D.h.imm), dim3(kBlock), D.h, D.fh);
+int launch_forcinginterpolate(DomainImpl<T>& D, bool gate) {
+  LAUNCH(k_forcing<T>, grid1(D.h.imm), dim3(kBlock), D.h, D.fh, gate ? 1 : 0);
   if (D.S.P.currents) {
     LAUNCH(k_set_anycur<T>, dim3(1), dim3(64), D.h, 0);
     LAUNCH(k_anycur<T>, grid1(D.h.imm), dim3(kBlock), D.h);
@@ -844,7 +848,7 @@ int launch_jonswap(DomainImpl<T>& D, const umwm_jonswap& js) {
 }
 
 #define INST(T)                                                          \
-  template int launch_forcinginterpolate<T>(DomainImpl<T>&);             \
+  template int launch_forcinginterpolate<T>(DomainImpl<T>&, bool);       \
   template int launch_op_sin<T>(DomainImpl<T>&);                         \
   template int launch_op_sds<T>(DomainImpl<T>&);                         \
   template int launch_op_snl<T>(DomainImpl<T>&);                         \
