@@ -37,7 +37,8 @@ typedef enum umwm_status {
   UMWM_ERR_NOMEM = -3,      /* device or host allocation failed           */
   UMWM_ERR_STATE = -4,      /* call made in the wrong order               */
   UMWM_ERR_UNKNOWN_FIELD = -5,
-  UMWM_ERR_SIZE = -6        /* buffer size does not match the field       */
+  UMWM_ERR_SIZE = -6,       /* buffer size does not match the field       */
+  UMWM_ERR_IO = -7          /* file cannot be opened, written or parsed   */
 } umwm_status;
 
 /* Floating-point type of the spectrum and of the source/advection math. */
@@ -238,6 +239,27 @@ int umwm_hip_get_field(umwm_domain* dom, const char* name, double* buf,
                        int64_t n);
 int umwm_hip_set_field(umwm_domain* dom, const char* name, const double* buf,
                        int64_t n);
+/* ---------------------------------------------------------------------- */
+/* Output and restart files                                               */
+/* ---------------------------------------------------------------------- */
+/*
+ * The reference writes NetCDF (umwmout_<time>.nc, umwmrst_<time>.nc);
+ * NetCDF is not available here, so these write the same variables (names,
+ * dimension names and order, units, single precision) into a raw container
+ * described in umwm_amd/csrc/rawio.hpp and read by umwm_amd/rawfile.py.
+ *
+ * umwm_hip_write_output: output_grid_nc (src/umwm_io.F90:317-854), every
+ * diagnostic on the (x, y) grid with NaN on land; the reference calls
+ * stress('ocn') first (src/umwm_top.F90:160-170), and so should the caller.
+ * umwm_hip_write_restart / umwm_hip_read_restart: restart_write and
+ * restart_read (src/umwm_restart.F90:18-145), the spectrum F(f,th,x) and
+ * ust(x) of the sea points.  Reading a restart into a domain created with
+ * restart = 1 continues the run without the cold first substep.
+ */
+int umwm_hip_write_output(umwm_domain* dom, const char* path);
+int umwm_hip_write_restart(umwm_domain* dom, const char* path);
+int umwm_hip_read_restart(umwm_domain* dom, const char* path);
+
 /* Scalars: sumt dts dta dtr dtamin cfllim cgmax mindelx dlnf dth bf1 bf2
  * first nsub.  Settable: sumt dts dtr_temp first, and keep_sources: set to 1
  * (it cannot be unset) before the substep that reaches dtg, it makes the

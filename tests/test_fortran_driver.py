@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle_py
-from umwm_amd import abi, build
+from umwm_amd import abi, build, rawfile
 
 EXE = os.path.join(build.HERE, "bin", "umwm_gpu")
 
@@ -22,10 +22,10 @@ NAMELIST = """&DOMAIN
   fmin = 0.0313
   fmax = 2.0
   fprog = 2.0
-  startTimeStr = '2012-01-01 00:00:00'
+  startTimeStr = '{start}'
   stopTimeStr  = '2012-01-01 02:00:00'
   dtg = 3600
-  restart = .false.
+  restart = {restart}
 /
 &PHYSICS
   g = 9.80665
@@ -109,6 +109,21 @@ def read_records(path, im):
     return out
 
 
+def run_driver(tmp_path, mode, start="2012-01-01 00:00:00", restart=".false.", tag="run",
+               template=NAMELIST):
+    """Runs umwm_gpu in tmp_path (which gets the reference's output/ and
+    restart/ folders); returns (record file path, completed process)."""
+    out = tmp_path / f"{tag}.bin"
+    nml = tmp_path / f"{tag}.nml"
+    nml.write_text(template.format(mode=mode, out=str(out), start=start, restart=restart))
+    for sub in ("output", "restart"):
+        (tmp_path / sub).mkdir(exist_ok=True)
+    r = subprocess.run([EXE, str(nml)], capture_output=True, text=True, timeout=300,
+                       cwd=str(tmp_path))
+    print(r.stdout)
+    return out, r
+
+
 def oracle_hours():
     p = abi.make_params(mm=14, nm=11, om=25, pm=24, delx=20000.0, dely=15000.0, dpt=800.0,
                         wspd0=12.0, wdir0=0.3, stokes=1,
@@ -128,11 +143,7 @@ def oracle_hours():
 @pytest.mark.parametrize("mode", ["operators", "fused"])
 def test_driver_matches_oracle(mode, tmp_path):
     assert os.path.exists(EXE), "Fortran driver not built (python -m umwm_amd.build)"
-    out = tmp_path / "out.bin"
-    nml = tmp_path / "main.nml"
-    nml.write_text(NAMELIST.format(mode=mode, out=str(out)))
-    r = subprocess.run([EXE, str(nml)], capture_output=True, text=True, timeout=300)
-    print(r.stdout)
+    out, r = run_driver(tmp_path, mode)
     assert r.returncode == 0, r.stdout + r.stderr
     im, ref = oracle_hours()
     recs = read_records(str(out), im)
@@ -141,6 +152,56 @@ def test_driver_matches_oracle(mode, tmp_path):
         for a, b, name in ((hs, hs0, "swh"), (tm, tm0, "mwp"), (ust, ust0, "ustar")):
             err = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
             assert err < 1e-10, f"{mode} hour {hour} {name} rel err {err:.3e}"
+    # gridded output every hour (outgrid = 1): swh and ust are the record's
+    # fields in single precision at the sea points
+    for hour, hs, tm, ust in recs:
+        f = rawfile.read(tmp_path / "output" / f"umwmout_2012-01-01_0{hour}:00:00.bin")
+        sea = ~np.isnan(f["swh"].data[:, :, 0])
+        assert sea.sum() == im
+        assert np.array_equal(np.sort(f["swh"].data[:, :, 0][sea]), np.sort(hs.astype(np.float32)))
+        assert np.array_equal(np.sort(f["ust"].data[:, :, 0][sea]), np.sort(ust.astype(np.float32)))
+        assert np.all(np.isfinite(f["taux_ocn"].data[:, :, 0][sea]))
+        assert np.any(f["taux_ocn"].data[:, :, 0][sea] != 0)
+
+
+@pytest.mark.gpu
+def test_driver_restart_continues_the_run(tmp_path):
+    """outrst = 1 writes restart/umwmrst_<time>.bin each hour.  A second run
+    with restart = .true. from 01:00 reads it into the state that
+    umwm_initialize built (so, as in the reference, the surface Stokes drift
+    of its first hour is still that of the initial spectrum) and takes no
+    cold substep; its hour matches the FP64 oracle started the same way from
+    the file's single-precision spectrum and ustar."""
+    assert os.path.exists(EXE), "Fortran driver not built (python -m umwm_amd.build)"
+    template = NAMELIST.replace("outrst = 6", "outrst = 1")
+    out, r = run_driver(tmp_path, "fused", tag="full", template=template)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rst = tmp_path / "restart" / "umwmrst_2012-01-01_01:00:00.bin"
+    assert rst.exists()
+    out2, r2 = run_driver(tmp_path, "fused", start="2012-01-01 01:00:00", restart=".true.",
+                          tag="cont", template=template)
+    assert r2.returncode == 0, r2.stdout + r2.stderr
+    f = rawfile.read(rst)
+    p = abi.make_params(mm=14, nm=11, om=25, pm=24, delx=20000.0, dely=15000.0, dpt=800.0,
+                        wspd0=12.0, wdir0=0.3, stokes=1, restart=1,
+                        depths=[0.1, 0.2, 0.5, 1.0, 2.0, 5.0, 10, 20, 50, 100])
+    o = oracle_py.Oracle(p, None, None, threads=2)
+    o.init_jonswap(abi.make_jonswap(alpha=0.01, gamma=3.3, fp=0.25, fp_from_wind=False,
+                                    dir_from_wind=True))
+    o.stokes_drift(True)
+    o.get("e")[:] = f["F"].data.T
+    o.get("ustar")[:] = f["ust"].data
+    o.run_dtg()
+    im = o.im
+    (hour, hs, tm, ust), = read_records(str(out2), im)
+    assert hour == 1
+    for a, b, name in ((hs, o.get("ht"), "swh"), (tm, o.get("mwp"), "mwp"),
+                       (ust, o.get("ustar"), "ustar")):
+        err = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        assert err < 1e-10, f"restart {name} rel err {err:.3e}"
+    # and it is a continuation: close to the uninterrupted run's 02:00
+    full = read_records(str(out), im)
+    assert float(np.max(np.abs(hs - full[1][1])) / np.max(full[1][1])) < 1e-3
 
 
 def test_driver_rejects_bad_namelist(tmp_path):
@@ -149,7 +210,8 @@ def test_driver_rejects_bad_namelist(tmp_path):
     if not os.path.exists(EXE):
         pytest.skip("Fortran driver not built")
     nml = tmp_path / "main.nml"
-    nml.write_text(NAMELIST.format(mode="fused", out=str(tmp_path / "o.bin")).replace(
+    nml.write_text(NAMELIST.format(mode="fused", out=str(tmp_path / "o.bin"),
+                                   start="2012-01-01 00:00:00", restart=".false.").replace(
         "pm = 24", "pm = 30"))
     r = subprocess.run([EXE, str(nml)], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0

@@ -23,7 +23,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 FLANG = os.path.join(ROCM, "bin", "amdflang")
 
 HIP_SOURCES = ["ops.hip", "fused.hip", "capi.hip"]
-CXX_SOURCES = ["setup.cpp"]
+CXX_SOURCES = ["setup.cpp", "rawio.cpp"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

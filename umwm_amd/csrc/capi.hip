@@ -12,6 +12,7 @@
 
 #include "domain.hpp"
 #include "physics.hpp"
+#include "rawio.hpp"
 
 namespace umwm {
 
@@ -918,6 +919,7 @@ const char* umwm_hip_error_string(int status) {
     case UMWM_ERR_STATE: return "call made in the wrong state";
     case UMWM_ERR_UNKNOWN_FIELD: return "unknown field";
     case UMWM_ERR_SIZE: return "buffer size mismatch";
+    case UMWM_ERR_IO: return "file error";
   }
   return "unknown status";
 }
@@ -1146,6 +1148,230 @@ int umwm_hip_run_dtg(umwm_domain* dom, int32_t* nsub) {
 int umwm_hip_get_field(umwm_domain* dom, const char* name, double* buf, int64_t n) {
   if (!name || !buf) return fail(UMWM_ERR_ARGUMENT, "null argument");
   return dispatch(dom, [&](auto& D) { return get_field_impl(D, name, buf, n); });
+}
+
+}  // extern "C"
+
+namespace {
+// ---------------------------------------------------------------------
+// gridded output and restart files (raw container, rawio.hpp)
+// ---------------------------------------------------------------------
+// A field of the domain as doubles in the reference's shape, or zeros when
+// it does not exist yet (the reference's module arrays start at zero).
+template <typename T>
+int fetch_field(DomainImpl<T>& D, const std::string& name, int64_t n, std::vector<double>& out) {
+  FieldRef r;
+  out.assign((size_t)n, 0.0);
+  if (!find_field(D, name, r)) return 0;
+  const int64_t have = field_size(D, r.kind);
+  std::vector<double> tmp((size_t)have);
+  if (int rc = get_field_impl(D, name, tmp.data(), have)) return rc;
+  std::copy(tmp.begin(), tmp.begin() + std::min(have, n), out.begin());
+  return 0;
+}
+
+// src/umwm_io.F90:317-854 output_grid_nc: every field on the (x, y) grid,
+// sea points through the unrolled index and NaN on land (gatherfield,
+// :857-901); lon, lat, seamask and depth as 2-D arrays.
+template <typename T>
+int write_output_impl(DomainImpl<T>& D, const std::string& path) {
+  const HostSetup& S = D.S;
+  const int64_t mm = S.mm, nm = S.nm, im = S.im, lm = std::max(S.lm, 1);
+  const double nan = std::nan("");
+  std::vector<RawVar> vars;
+  auto add = [&](const char* name, const char* units, const char* desc, int32_t type,
+                 std::vector<std::string> dims, std::vector<int64_t> shape,
+                 std::vector<double> data) {
+    RawVar v;
+    v.name = name;
+    v.units = units;
+    v.description = desc;
+    v.type = type;
+    v.dims = std::move(dims);
+    v.shape = std::move(shape);
+    v.data = std::move(data);
+    vars.push_back(std::move(v));
+  };
+  auto to_grid = [&](const std::vector<double>& sea, size_t off) {
+    std::vector<double> g((size_t)(mm * nm), nan);
+    for (int64_t j = 0; j < im; ++j)
+      g[(size_t)((S.ni[j] - 1) * mm + (S.mi[j] - 1))] = sea[off + (size_t)j];
+    return g;
+  };
+  std::vector<double> buf;
+  auto grid_var = [&](const char* name, const char* src, const char* units, const char* desc) {
+    if (int rc = fetch_field(D, src, im, buf)) return rc;
+    add(name, units, desc, RAW_F32, {"x", "y", "time"}, {mm, nm, 1}, to_grid(buf, 0));
+    return 0;
+  };
+  const bool stokes = S.P.stokes != 0;
+  if (stokes) {
+    std::vector<double> z((size_t)lm);
+    for (int64_t l = 0; l < lm; ++l) z[(size_t)l] = S.depth_l[(size_t)l];  // -depths
+    add("z", "m", "depth", RAW_F32, {"z"}, {lm}, z);
+    for (const char* c : {"us", "vs"}) {
+      if (int rc = fetch_field(D, c, im * lm, buf)) return rc;
+      std::vector<double> g;
+      for (int64_t l = 0; l < lm; ++l) {
+        const auto gl = to_grid(buf, (size_t)(l * im));
+        g.insert(g.end(), gl.begin(), gl.end());
+      }
+      add(c[0] == 'u' ? "u_stokes" : "v_stokes", "m/s",
+          c[0] == 'u' ? "Stokes drift, x" : "Stokes drift, y", RAW_F32, {"x", "y", "z", "time"},
+          {mm, nm, lm, 1}, g);
+    }
+    if (int rc = grid_var("d_stokes", "ds", "m", "Stokes drift e-folding depth")) return rc;
+  }
+  add("frequency", "hz", "frequency", RAW_F32, {"f"}, {(int64_t)S.om}, S.f);
+  add("theta", "rad", "direction", RAW_F32, {"th"}, {(int64_t)S.pm}, S.th);
+  add("lon", "degrees east", "longitude", RAW_F32, {"x", "y", "time"}, {mm, nm, 1}, S.lon);
+  add("lat", "degrees north", "latitude", RAW_F32, {"x", "y", "time"}, {mm, nm, 1}, S.lat);
+  add("seamask", "non-dimensional", "sea mask", RAW_I32, {"x", "y", "time"}, {mm, nm, 1},
+      std::vector<double>(S.mask.begin(), S.mask.end()));
+  add("depth", "m", "water depth", RAW_F32, {"x", "y", "time"}, {mm, nm, 1}, S.d2);
+  struct G { const char *name, *src, *units, *desc; };
+  static const G grid[] = {
+      {"wspd", "wspd", "m/s", "wind speed"},
+      {"wdir", "wdir", "rad", "wind direction"},
+      {"uc", "uc", "m/s", "ocean current, x"},
+      {"vc", "vc", "m/s", "ocean current, y"},
+      {"rhoa", "rhoa", "kg/m^3", "air density"},
+      {"rhow", "rhow", "kg/m^3", "water density"},
+      {"fice", "fice", "non-dimensional", "sea ice fraction"},
+      {"psim", "psim", "non-dimensional", "stability function (coupled runs only)"},
+      {"momx", "momx", "kgm/s", "wave momentum, x"},
+      {"momy", "momy", "kgm/s", "wave momentum, y"},
+      {"cgmxx", "cgmxx", "kgm^2/s^2", "momentum flux xx"},
+      {"cgmxy", "cgmxy", "kgm^2/s^2", "momentum flux xy"},
+      {"cgmyy", "cgmyy", "kgm^2/s^2", "momentum flux yy"},
+      {"shelt", "shelt", "non-dimensional", "sheltering coefficient"},
+      {"epsx_atm", "epsx_atm", "kg/s^3", "energy flux from air, x"},
+      {"epsy_atm", "epsy_atm", "kg/s^3", "energy flux from air, y"},
+      {"epsx_ocn", "epsx_ocn", "kg/s^3", "energy flux into ocean, x"},
+      {"epsy_ocn", "epsy_ocn", "kg/s^3", "energy flux into ocean, y"},
+      {"taux_form", "taux_form", "n/m^2", "form drag, x"},
+      {"tauy_form", "tauy_form", "n/m^2", "form drag, y"},
+      {"taux_form_1", "taux1", "n/m^2", "form drag, wind pushing waves, x"},
+      {"tauy_form_1", "tauy1", "n/m^2", "form drag, wind pushing waves, y"},
+      {"taux_form_2", "taux2", "n/m^2", "form drag, waves against wind, x"},
+      {"tauy_form_2", "tauy2", "n/m^2", "form drag, waves against wind, y"},
+      {"taux_form_3", "taux3", "n/m^2", "form drag, waves overrunning wind, x"},
+      {"tauy_form_3", "tauy3", "n/m^2", "form drag, waves overrunning wind, y"},
+      {"taux_skin", "taux_skin", "n/m^2", "skin drag, x"},
+      {"tauy_skin", "tauy_skin", "n/m^2", "skin drag, y"},
+      {"taux_diag", "taux_diag", "n/m^2", "form drag of the diagnostic tail, x"},
+      {"tauy_diag", "tauy_diag", "n/m^2", "form drag of the diagnostic tail, y"},
+      {"taux_ocn", "taux_ocntop", "n/m^2", "stress into ocean top, x"},
+      {"tauy_ocn", "tauy_ocntop", "n/m^2", "stress into ocean top, y"},
+      {"taux_bot", "taux_ocnbot", "n/m^2", "stress into ocean bottom, x"},
+      {"tauy_bot", "tauy_ocnbot", "n/m^2", "stress into ocean bottom, y"},
+      {"taux_snl", "taux_snl", "n/m^2", "momentum loss by Snl, x"},
+      {"tauy_snl", "tauy_snl", "n/m^2", "momentum loss by Snl, y"},
+      {"tailatmx", "tailatmx", "n/m^2", "form drag tail, x"},
+      {"tailatmy", "tailatmy", "n/m^2", "form drag tail, y"},
+      {"tailocnx", "tailocnx", "n/m^2", "ocean stress tail, x"},
+      {"tailocny", "tailocny", "n/m^2", "ocean stress tail, y"},
+      {"cd", "cd", "non-dimensional", "drag coefficient"},
+      {"ust", "ustar", "m/s", "friction velocity"},
+      {"swh", "ht", "m", "significant wave height"},
+      {"mss", "mss", "non-dimensional", "mean squared slope"},
+      {"mwp", "mwp", "s", "mean wave period"},
+      {"mwl", "mwl", "m", "mean wavelength"},
+      {"mwd", "mwd", "rad", "mean wave direction"},
+      {"dwp", "dwp", "s", "dominant wave period"},
+      {"dwl", "dwl", "m", "dominant wavelength"},
+      {"dwd", "dwd", "rad", "dominant wave direction"},
+      {"dcp0", "dcp0", "m/s", "dominant phase speed, intrinsic"},
+      {"dcg0", "dcg0", "m/s", "dominant group speed, intrinsic"},
+      {"dcp", "dcp", "m/s", "dominant phase speed"},
+      {"dcg", "dcg", "m/s", "dominant group speed"},
+      {"physics_time_step", "physics_time_step", "s", "physics time step"}};
+  for (const G& g : grid)
+    if (int rc = grid_var(g.name, g.src, g.units, g.desc)) return rc;
+  std::string err;
+  if (raw_write(path, vars, err)) return fail(UMWM_ERR_IO, err);
+  return 0;
+}
+
+// src/umwm_restart.F90:55-145 restart_write: per sea point lon, lat, ust,
+// wavenumber(f,x) and the spectrum F(f,th,x), single precision as there.
+template <typename T>
+int write_restart_impl(DomainImpl<T>& D, const std::string& path) {
+  const HostSetup& S = D.S;
+  const int64_t im = S.im, om = S.om, pm = S.pm;
+  std::vector<RawVar> vars(7);
+  std::vector<double> lon((size_t)im), lat((size_t)im);
+  for (int64_t j = 0; j < im; ++j) {
+    const size_t q = (size_t)((S.ni[j] - 1) * S.mm + (S.mi[j] - 1));
+    lon[(size_t)j] = S.lon[q];
+    lat[(size_t)j] = S.lat[q];
+  }
+  auto set = [&](int k, const char* name, const char* units, const char* desc,
+                 std::vector<std::string> dims, std::vector<int64_t> shape,
+                 std::vector<double> data) {
+    vars[k].name = name;
+    vars[k].units = units;
+    vars[k].description = desc;
+    vars[k].type = RAW_F32;
+    vars[k].dims = std::move(dims);
+    vars[k].shape = std::move(shape);
+    vars[k].data = std::move(data);
+  };
+  std::vector<double> ust, kk, e;
+  int rc = fetch_field(D, "ustar", im, ust);
+  if (!rc) rc = fetch_field(D, "k", om * im, kk);
+  if (!rc) rc = fetch_field(D, "e", om * pm * im, e);
+  if (rc) return rc;
+  set(0, "lon", "degrees east", "longitude", {"x"}, {im}, lon);
+  set(1, "lat", "degrees north", "latitude", {"x"}, {im}, lat);
+  set(2, "frequency", "hz", "frequency", {"f"}, {om}, S.f);
+  set(3, "theta", "rad", "direction", {"th"}, {pm}, S.th);
+  set(4, "ust", "m/s", "friction velocity", {"x"}, {im}, ust);
+  set(5, "wavenumber", "rad/m", "wavenumber", {"f", "x"}, {om, im}, kk);
+  set(6, "F", "m^2/rad/(rad/m)", "wave variance spectrum", {"f", "th", "x"}, {om, pm, im}, e);
+  std::string err;
+  if (raw_write(path, vars, err)) return fail(UMWM_ERR_IO, err);
+  return 0;
+}
+
+// src/umwm_restart.F90:18-52 restart_read: e and ustar of the sea points.
+template <typename T>
+int read_restart_impl(DomainImpl<T>& D, const std::string& path) {
+  const HostSetup& S = D.S;
+  std::vector<RawVar> vars;
+  std::string err;
+  if (raw_read(path, vars, err)) return fail(UMWM_ERR_IO, err);
+  const RawVar *F = nullptr, *u = nullptr;
+  for (const RawVar& v : vars) {
+    if (v.name == "F") F = &v;
+    if (v.name == "ust") u = &v;
+  }
+  if (!F || !u) return fail(UMWM_ERR_IO, path + ": no F or ust variable");
+  const std::vector<int64_t> want_f = {S.om, S.pm, S.im}, want_u = {S.im};
+  if (F->shape != want_f || u->shape != want_u)
+    return fail(UMWM_ERR_SIZE, path + ": F or ust does not match the domain's (om, pm, im)");
+  int rc = set_field_impl(D, "e", F->data.data(), (int64_t)F->data.size());
+  if (!rc) rc = set_field_impl(D, "ustar", u->data.data(), (int64_t)u->data.size());
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int umwm_hip_write_output(umwm_domain* dom, const char* path) {
+  if (!path) return fail(UMWM_ERR_ARGUMENT, "null path");
+  return dispatch(dom, [&](auto& D) { return write_output_impl(D, path); });
+}
+
+int umwm_hip_write_restart(umwm_domain* dom, const char* path) {
+  if (!path) return fail(UMWM_ERR_ARGUMENT, "null path");
+  return dispatch(dom, [&](auto& D) { return write_restart_impl(D, path); });
+}
+
+int umwm_hip_read_restart(umwm_domain* dom, const char* path) {
+  if (!path) return fail(UMWM_ERR_ARGUMENT, "null path");
+  return dispatch(dom, [&](auto& D) { return read_restart_impl(D, path); });
 }
 
 int umwm_hip_set_field(umwm_domain* dom, const char* name, const double* buf, int64_t n) {

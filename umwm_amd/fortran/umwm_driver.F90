@@ -13,9 +13,13 @@
 !> from &FORCING_CONSTANT, bathymetry (topoFromFile) from a raw file of
 !> mm*nm float64 values, m fastest (&GPU topo_file).  An optional &GPU group
 !> selects the precision, the mode, a JONSWAP start instead of the cold start
-!> and the output file, which receives after each dtg the record
-!>   int32 im, int32 hour, float64 swh(im), mwp(im), ustar(im)
-!> (the fields of the reference's gridded output at sea points).
+!> and a record file, which receives after each dtg the record
+!>   int32 im, int32 hour, float64 swh(im), mwp(im), ustar(im).
+!> Gridded output and restart files follow &OUTPUT outgrid / outrst as in
+!> src/umwm_top.F90:157-195 (output/umwmout_<time>.bin after stress('ocn'),
+!> restart/umwmrst_<time>.bin, and with restart = .true. the start reads
+!> restart/umwmrst_<startTimeStr>.bin), in the raw container of
+!> umwm_amd/csrc/rawio.hpp instead of NetCDF.
 
 !> &STOKES lives in its own scope: its group name is also the name of the
 !> &OUTPUT switch (src/umwm_stokes.F90:231-240 reads it inside umwm_stokes).
@@ -93,6 +97,33 @@ contains
     logical, intent(in) :: l
     b2i = merge(1, 0, l)
   end function b2i
+
+  !> 'YYYY-MM-DD_HH:MM:SS' of seconds since 1970-01-01 (inverse of
+  !> epoch_seconds), the time stamp of the output file names.
+  function time_string(t) result(s)
+    integer(8), intent(in) :: t
+    character(19) :: s
+    integer(8) :: days, sod, z, era, doe, yoe, y, doy, mp, d, m
+    days = t / 86400
+    sod = t - days * 86400
+    if (sod < 0) then
+      sod = sod + 86400
+      days = days - 1
+    end if
+    z = days + 719468
+    era = z / 146097
+    if (z < 0 .and. mod(z, 146097_8) /= 0) era = era - 1
+    doe = z - era * 146097
+    yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365
+    y = yoe + era * 400
+    doy = doe - (365 * yoe + yoe / 4 - yoe / 100)
+    mp = (5 * doy + 2) / 153
+    d = doy - (153 * mp + 2) / 5 + 1
+    m = mp + merge(3_8, -9_8, mp < 10)
+    if (m <= 2) y = y + 1
+    write(s, '(i4.4,a,i2.2,a,i2.2,a,i2.2,a,i2.2,a,i2.2)') y, '-', m, '-', d, '_', &
+      sod / 3600, ':', mod(sod, 3600_8) / 60, ':', mod(sod, 60_8)
+  end function time_string
 
   !> Seconds since 1970-01-01 of 'YYYY-MM-DD HH:MM:SS' (proleptic Gregorian).
   integer(8) function epoch_seconds(s)
@@ -208,6 +239,13 @@ contains
       call check(umwm_hip_init_jonswap(dom, js), 'umwm_initialize')
     end if
     call stokes_drift('init')
+    ! gridded output calls stress('ocn'), which after fused substeps needs
+    ! the sources of the substep that reaches dtg kept
+    if (outgrid /= 0 .and. trim(mode) == 'fused') &
+      call check(umwm_hip_set_scalar(dom, c_str('keep_sources'), 1d0), 'umwm_initialize')
+    ! src/umwm_top.F90:66: restart_read(starttimestr)
+    if (restart) call check(umwm_hip_read_restart(dom, &
+      c_str('restart/umwmrst_'//time_string(epoch_seconds(startTimeStr))//'.bin')), 'restart_read')
     nsteps = int((epoch_seconds(stopTimeStr) - epoch_seconds(startTimeStr)) / nint(dtg))
     n = len_trim(out_file)
     if (n > 0) then
@@ -280,8 +318,33 @@ contains
       write(*, '(a,i4,a,i6,a,f9.3,a,f9.5,a,f9.5)') 'umwm_gpu: hour', hour, ' substeps', nsub, &
         ' last dts', dts, ' mean swh', sum(swh) / im, ' max swh', v
       call write_fields(hour)
+      call output_and_restart(epoch_seconds(startTimeStr) + int(hour, 8) * nint(dtg, 8))
     end do
   end subroutine umwm_run
+
+  !> src/umwm_top.F90:157-195: gridded output every outgrid hours (every
+  !> dtg for -1) after stress('ocn'), restart every outrst hours, on full
+  !> hours of the model clock.
+  subroutine output_and_restart(t)
+    integer(8), intent(in) :: t
+    integer(8) :: hod
+    logical :: fullhour
+    character(19) :: ts
+    ts = time_string(t)
+    hod = mod(t, 86400_8) / 3600
+    fullhour = mod(t, 3600_8) == 0
+    if ((outgrid > 0 .and. fullhour .and. mod(hod, int(max(outgrid, 1), 8)) == 0) .or. &
+        outgrid == -1) then
+      call stress('ocn')
+      call check(umwm_hip_write_output(dom, c_str('output/umwmout_'//ts//'.bin')), &
+                 'output_grid')
+    end if
+    if (outrst > 0 .and. fullhour) then
+      if (mod(hod, int(outrst, 8)) == 0) &
+        call check(umwm_hip_write_restart(dom, c_str('restart/umwmrst_'//ts//'.bin')), &
+                   'restart_write')
+    end if
+  end subroutine output_and_restart
 
   !> src/umwm_top.F90:199-206
   subroutine umwm_finalize()

@@ -96,6 +96,24 @@ module umwm_hip_capi
       real(c_double), intent(out) :: sumt, dts
       integer(c_int) :: rc
     end function
+    function umwm_hip_write_output(dom, path) result(rc) bind(C, name='umwm_hip_write_output')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: dom
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: rc
+    end function
+    function umwm_hip_write_restart(dom, path) result(rc) bind(C, name='umwm_hip_write_restart')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: dom
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: rc
+    end function
+    function umwm_hip_read_restart(dom, path) result(rc) bind(C, name='umwm_hip_read_restart')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: dom
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int) :: rc
+    end function
     function umwm_hip_substeps(dom, n, sumt, dts, nsub) result(rc) &
         bind(C, name='umwm_hip_substeps')
       import :: c_int, c_ptr, c_double, c_int32_t

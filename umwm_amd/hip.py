@@ -27,6 +27,7 @@ EXPORTS = [
     "umwm_hip_get_timing", "umwm_hip_synchronize", "umwm_hip_batch_create",
     "umwm_hip_batch_destroy", "umwm_hip_batch_run_dtg", "umwm_hip_batch_tile",
     "umwm_hip_set_tile", "umwm_hip_get_tile", "umwm_hip_batch_set_tile",
+    "umwm_hip_write_output", "umwm_hip_write_restart", "umwm_hip_read_restart",
 ]
 
 
@@ -63,6 +64,8 @@ def lib():
                "propagation", "refraction", "update", "diag", "end_first", "synchronize"):
         getattr(L, "umwm_hip_" + fn).argtypes = [vp]
     L.umwm_hip_stress.argtypes = [vp, C.c_char_p]
+    for fn in ("write_output", "write_restart", "read_restart"):
+        getattr(L, "umwm_hip_" + fn).argtypes = [vp, C.c_char_p]
     L.umwm_hip_substep.argtypes = [vp, P(C.c_double), P(C.c_double)]
     L.umwm_hip_substeps.argtypes = [vp, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_int32)]
     L.umwm_hip_run_dtg.argtypes = [vp, P(C.c_int32)]
@@ -236,6 +239,16 @@ class Domain:
         s, d = C.c_double(), C.c_double()
         check(lib().umwm_hip_substep(self.h, C.byref(s), C.byref(d)))
         return s.value, d.value
+
+    def write_output(self, path):
+        """Gridded output (the reference's output_grid_nc variables)."""
+        check(lib().umwm_hip_write_output(self.h, str(path).encode()))
+
+    def write_restart(self, path):
+        check(lib().umwm_hip_write_restart(self.h, str(path).encode()))
+
+    def read_restart(self, path):
+        check(lib().umwm_hip_read_restart(self.h, str(path).encode()))
 
     def substeps(self, n):
         """n substeps back to back (one host wait); returns (sumt, dts,
