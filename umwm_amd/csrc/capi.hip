@@ -222,6 +222,8 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   h.kappa = P.kappa; h.fice_lth = P.fice_lth; h.fice_uth = P.fice_uth;
   h.nu_water = P.nu_water;
   h.tiny = sizeof(T) == 8 ? DBL_MIN : (double)FLT_MIN;
+  h.sfct = P.sfct; h.rhow0 = P.rhow0; h.sbf_fac = P.sbf_fac; h.sbp_fac = P.sbp_fac;
+  h.bf1 = S.bf1; h.bf2 = S.bf2;
 
   const int om = S.om, pm = S.pm, ldi = S.ldi, ldq = S.ldq;
   const size_t ns = (size_t)om * pm * ldi, nt = (size_t)om * ldi;
@@ -290,10 +292,15 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
     if (!dd) { delete D; return fail(UMWM_ERR_NOMEM, "device allocation failed"); }
     if ((rc = upload<T>(dd, d4, (size_t)4 * pm, s))) { delete D; return rc; }
     h.dir4 = dd;
+    T* dw = A.template alloc<T>((size_t)om);
+    if (!dw) { delete D; return fail(UMWM_ERR_NOMEM, "device allocation failed"); }
+    if ((rc = upload<T>(dw, S.dom, (size_t)om, s))) { delete D; return rc; }
+    h.dom = dw;
   }
   h.pl = pl; h.pr = pr;
-  // per-(frequency, point) table records for the fused kernels, [o][ldi][FT_N]
-  {
+  // per-(frequency, point) table records for the fused FP64 kernels,
+  // [o][ldi][FT_N] (the FP32 kernels derive them from k)
+  if (sizeof(T) == 8) {
     const std::vector<double>* ft[FT_PAD] = {&S.cp0, &S.logl2overz, &S.cothkd, &S.sbf, &S.k,
                                              &S.cg0, &S.bf1r, &S.bf2r, &S.kdk};
     std::vector<double> blk((size_t)FT_N * nt, 0.0);

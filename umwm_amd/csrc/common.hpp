@@ -59,6 +59,9 @@ struct DomainDev {
   double sin_fac, fieldscale1, fieldscale2, sds_power, inv_sds_power;
   double mss_fac, snl_fac, sdt_fac, z, nu_air, kappa, fice_lth, fice_uth, nu_water;
   double tiny;  // tiny(e) of the working precision
+  // dispersion-table constants for the FP32 sweeps, which derive their
+  // (frequency, point) tables from k and the depth
+  double sfct, rhow0, sbf_fac, sbp_fac, bf1, bf2;
   // spectra
   T *e, *ef, *es;
   T *ssin, *sds, *snl, *dummy, *rotl, *rotr;  // operator path only
@@ -71,6 +74,7 @@ struct DomainDev {
   // spectral constants
   const T *f, *th, *cth, *sth, *c2th, *s2th, *cth2pp;
   const T* dir4;  // per direction {cth, sth, cos 2th, sin 2th}, one 4-value record
+  const T* dom;   // frequency increment d omega per bin
   const int32_t *pl, *pr;
   // per unrolled point (ldq)
   T *wspd, *wdir, *uc, *vc, *rhoa, *rhow, *rhorat, *fice;

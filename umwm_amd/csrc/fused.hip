@@ -221,6 +221,104 @@ __device__ __forceinline__ void load_tab(const T* ftab, size_t ldi, int o, uint3
   for (int i = N; i < FT_N; ++i) r.v[i] = T(0);
 }
 
+// FP32 sweeps derive the (frequency, point) tables from k and the depth
+// (src/umwm_init.F90:1480-1533) rather than load them: the sweeps are
+// bound by memory traffic, and one k per (frequency, point) replaces a
+// 9-value record.  With u = e^{-2 kd}, kd = min(k d, 20):
+//   sinh(2 kd) = (1/u - u)/2,  cosh(kd) = (1/sqrt(u) + sqrt(u))/2,
+//   coth(0.2 kd) = (1 + w)/(1 - w) with w = e^{-0.4 kd},
+//   cp0 = 2 pi f / k, cg0 = cp0 (1/2 + k d / sinh(2 kd) + s k^2/(rho g + s k^2)),
+//   kdk = k dom / |cg0|, bf1r(o) = snl_fac bf1 kdk(o+1)/kdk(o) (o <= om-3),
+//   log(l2/z) with l2 = cp0/(2 f) capped at 20 m.
+// It keeps k of the next three frequencies and kdk of the next two, so
+// bf1r and bf2r need no look-back.  FP64 keeps the record tables: its
+// library exp would cost more than the traffic saved.
+struct TabGen {
+  const float* kb;
+  const float *fr, *domr;
+  size_t ldi;
+  uint32_t vj;
+  int om;
+  float dep, sgr, sbf, sbp, s1, s2, l2z, lz;
+  float k0, k1, k2, kn;   // k(o), k(o+1), k(o+2), k(o+3) (loaded ahead)
+  float kdk0, kdk1, kdk2;
+
+  __device__ float load_k(int o) const {
+    return bld<float>(row_rsrc(kb, (size_t)min(o, om - 1), ldi), vj);
+  }
+  __device__ float kdk_of(float k, int o) const {
+    o = min(o, om - 1);
+    const float kd = fminf(k * dep, 20.f);
+    const float u = __builtin_amdgcn_exp2f(-2.f * 1.4426950408889634f * kd);
+    const float sinh2 = 0.5f * (__builtin_amdgcn_rcpf(u) - u);
+    const float cp = 6.283185307179586f * cld(fr, o) * __builtin_amdgcn_rcpf(k);
+    const float sk2 = sgr * k * k;
+    const float cg = cp * (0.5f + k * dep * __builtin_amdgcn_rcpf(sinh2) +
+                           sk2 * __builtin_amdgcn_rcpf(1.f + sk2));
+    return k * cld(domr, o) * __builtin_amdgcn_rcpf(fabsf(cg));
+  }
+  template <typename D>
+  __device__ void init(const D& d, int j, uint32_t vj_) {
+    kb = d.k;
+    fr = d.f;
+    domr = d.dom;
+    ldi = d.ldi;
+    vj = vj_;
+    om = d.om;
+    dep = fmaxf(pload(d.depth, ldi, j), 1.f);  // lanes past im read 0
+    sgr = float(d.sfct / (d.rhow0 * d.g));
+    sbf = float(d.sbf_fac);
+    sbp = float(d.sbp_fac);
+    s1 = float(d.snl_fac * d.bf1);
+    s2 = float(d.snl_fac * d.bf2);
+    lz = float(log(d.z));
+    l2z = float(log(20.0 / d.z));
+    k0 = load_k(0);
+    k1 = load_k(1);
+    k2 = load_k(2);
+    kn = load_k(3);
+    kdk0 = kdk_of(k0, 0);
+    kdk1 = kdk_of(k1, 1);
+    kdk2 = kdk_of(k2, 2);
+  }
+  // slots of frequency o (the FreqTab layout of the record tables)
+  __device__ void table(int o, FreqTab<float>& r) const {
+    const float k = k0, fo = cld(fr, o);
+    const float kd = fminf(k * dep, 20.f);
+    const float u = __builtin_amdgcn_exp2f(-2.f * 1.4426950408889634f * kd);
+    const float iu = __builtin_amdgcn_rcpf(u);
+    const float isinh2 = __builtin_amdgcn_rcpf(0.5f * (iu - u));
+    const float v = __builtin_amdgcn_sqrtf(u);
+    const float ch = 0.5f * (__builtin_amdgcn_rcpf(v) + v);
+    const float w = __builtin_amdgcn_exp2f(-0.4f * 1.4426950408889634f * kd);
+    const float cp = 6.283185307179586f * fo * __builtin_amdgcn_rcpf(k);
+    const float sk2 = sgr * k * k;
+    const float l2 = 0.5f * cp * __builtin_amdgcn_rcpf(fo);
+    r.v[FT_CP0] = cp;
+    r.v[FT_LOGL2] = l2 > 20.f ? l2z : 0.6931471805599453f * __builtin_amdgcn_logf(l2) - lz;
+    r.v[FT_COTH] = (1.f + w) * __builtin_amdgcn_rcpf(1.f - w);
+    r.v[FT_SBF] = sbf * k * isinh2 + sbp * k * __builtin_amdgcn_rcpf(ch * ch);
+    r.v[FT_K] = k;
+    r.v[FT_CG0] = cp * (0.5f + k * dep * isinh2 + sk2 * __builtin_amdgcn_rcpf(1.f + sk2));
+    const bool bo = o <= om - 3;
+    const float ik = __builtin_amdgcn_rcpf(kdk0);
+    r.v[FT_B1] = bo ? s1 * kdk1 * ik : 0.f;
+    r.v[FT_B2] = bo ? s2 * kdk2 * ik : 0.f;
+    r.v[FT_KDK] = kdk0;
+    for (int q = FT_PAD; q < FT_N; ++q) r.v[q] = 0.f;
+  }
+  // moves to frequency o+1; `next` is k(o+4), loaded during iteration o
+  __device__ void advance(int o, float next) {
+    k0 = k1;
+    k1 = k2;
+    k2 = kn;
+    kn = next;
+    kdk0 = kdk1;
+    kdk1 = kdk2;
+    kdk2 = kdk_of(k2, o + 3);
+  }
+};
+
 // Transcendentals of the sweeps.  FP32 works in base 2 on the hardware
 // log2/exp2 (exponents pre-scaled by log2 e where a natural exponential is
 // meant); FP64 uses the device library's natural log and exp.
@@ -330,22 +428,24 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) ec[q] = bld<T>(q < nd ? ep : rz, vj, (p0 + q) * rb);
     }
-    // FP32 prefetches the next frequency's table record with its spectrum;
-    // FP64 (library exp/log, many live registers) loads it when needed
-    constexpr bool kPrefetchTab = sizeof(T) == 4;
-    if constexpr (kPrefetchTab) load_tab<kRatesSlots>(ftab, ldi, 0, vr, tc);
+    // FP32 derives the tables from k (TabGen); FP64 loads the record
+    constexpr bool kDerive = sizeof(T) == 4;
+    TabGen tg;
+    if constexpr (kDerive) tg.init(d, j, vj);
     for (int o = 0; o < om; ++o) {
       // issue the next frequency's loads before computing this one
       const int on = o + 1 < om ? o + 1 : o;
       T en[DMAX];
-      FreqTab<T> tn;
       const rsrc_t epn = freq_rsrc(eb, on, pm, ldi);
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(q < nd ? epn : rz, vj, (p0 + q) * rb);
-      if constexpr (kPrefetchTab)
-        load_tab<kRatesSlots>(ftab, ldi, on, vr, tn);
-      else
+      float knext = 0.f;
+      if constexpr (kDerive) {
+        knext = tg.load_k(o + 4);
+        tg.table(o, tc);
+      } else {
         load_tab<kRatesSlots>(ftab, ldi, o, vr, tc);
+      }
 
       const T k = tc[FT_K], k2 = k * k, fo = cld(fr, o);
       const T sdt = s.csdt * k;
@@ -421,7 +521,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       AS += k3 * s2s;
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) ec[q] = en[q];
-      if constexpr (kPrefetchTab) tc = tn;
+      if constexpr (kDerive) tg.advance(o, knext);
     }
     // max |rate| of the point over all waves
     __syncthreads();
@@ -591,9 +691,12 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   }
   FreqTab<T> tc;
   Prefix<T> pc{};
-  constexpr bool kPrefetchTab = sizeof(T) == 4;  // as in k_rates
-  if constexpr (kPrefetchTab) {
-    load_tab<kIntegrateSlots>(ftab, ldi, 0, vr, tc);
+  // FP32 derives the tables from k (TabGen) and prefetches the prefix;
+  // FP64 loads both when needed (as in k_rates)
+  constexpr bool kDerive = sizeof(T) == 4;
+  TabGen tg;
+  if constexpr (kDerive) {
+    tg.init(d, j, vj);
     pc = load_prefix(pfxb, ldi, 0, j);
   }
   for (int o = 0; o < om; ++o) {
@@ -602,11 +705,12 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     const rsrc_t epn = freq_rsrc(eb, on, pm, ldi);
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(q < nd ? epn : rz, vj, (p0 + q) * rb);
-    FreqTab<T> tn;
     Prefix<T> pn{};
-    if constexpr (kPrefetchTab) {
-      load_tab<kIntegrateSlots>(ftab, ldi, on, vr, tn);
+    float knext = 0.f;
+    if constexpr (kDerive) {
+      knext = tg.load_k(o + 4);
       pn = load_prefix(pfxb, ldi, on, j);
+      tg.table(o, tc);
     } else {
       load_tab<kIntegrateSlots>(ftab, ldi, o, vr, tc);
       pc = load_prefix(pfxb, ldi, o, j);
@@ -690,8 +794,8 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     b1m1 = tc[FT_B1];
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) ec[q] = en[q];
-    if constexpr (kPrefetchTab) {
-      tc = tn;
+    if constexpr (kDerive) {
+      tg.advance(o, knext);
       pc = pn;
     }
   }
@@ -975,8 +1079,9 @@ int launch_all(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int
   hipLaunchKernelGGL((k_integrate<T, D2>), grid, dim3(64 * tile.W2), 0, s, doms, tasks);
   if (keep) hipLaunchKernelGGL((k_keep<T>), grid, dim3(64), 0, s, doms, tasks);
   if (ev) (void)hipEventRecord(ev[2], s);
-  // advect: one wave per (64 points, frequency), FW frequencies per workgroup
-  const int fw = 4;
+  // advect: one wave per (64 points, frequency), FW frequencies per
+  // workgroup (2 measured 1-2 % faster than 4 or 8 on C2)
+  const int fw = 2;
   const dim3 ga(ntasks, (omax + fw - 1) / fw);
   if (tile.NB == 4)
     hipLaunchKernelGGL((k_advect<T, 4>), ga, dim3(64 * fw), 0, s, doms, tasks);
