@@ -47,8 +47,9 @@
 // All per-point streams go through raw buffer loads and stores (bufio.hpp):
 // one descriptor per frequency in scalar registers, 32-bit lane offsets
 // computed once per thread, directions reached with a scalar offset.  The
-// (frequency, point) tables are one 16-value record per lane (a few 16-byte
-// loads); the prefix moments and stress partial sums are 4-value records.
+// FP32 sweeps derive their (frequency, point) tables from k and the depth
+// (TabGen); FP64 reads a 12-value record per lane (three 16-byte loads).
+// The prefix moments and stress partial sums are 4-value records.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
