@@ -478,7 +478,10 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
         const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
         T dm = fma(qc, c2, fma(qs, s2, q0));
         dm = dm * dm;
-        const T sds = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fmax(e, T(0))), Lk));
+        // (e k4)^p of |e|: a bin emptied by round-off to about -1e-30 gives a
+        // vanishing sds, as the 0 the reference's power of a non-positive
+        // base is taken to be here, without a clamp instruction
+        const T sds = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk));
         const T r = ss - fma(sds, cs, crest);
         rq = fmax(rq, q < nd ? fabs(r) : T(0));
         m0 += e;
@@ -752,7 +755,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
         const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
         T dm = fma(qc, c2, fma(qs, s2, q0));
         dm = dm * dm;
-        const T sdsp = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fmax(e, T(0))), Lk));
+        const T sdsp = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk));  // |e|: see k_rates
         const T X = sdsp * e;
         T G = 0;
         if constexpr (decltype(PROG)::value) {
@@ -851,6 +854,10 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
                                  bld<T>(row_rsrc(d.fkovg, o, ldi), vj), sin_diss1, fs2);
   const rsrc_t pe = freq_rsrc(d.e, o, pm, ldi);    // E^n in, E^{n+1} out
   const rsrc_t ps = freq_rsrc(d.es, o, pm, ldi);   // E*
+  // the stress partials' cp0(o, i-1) (src/umwm_stress.F90:219,316) and kdk,
+  // loaded here so that their latency hides behind the direction loop
+  const T cp_prev = bld<T>(cpr, (uint32_t)((j == 0) ? d.im : j - 1) * E);
+  const T kdk = bld<T>(row_rsrc(d.kdk, o, ldi), vj);
   // direction tables in registers (see k_rates)
   const T* const cthb = d.cth;
   const T* const sthb = d.sth;
@@ -975,9 +982,7 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
     }
   }
   if (valid) {
-    const int jc = (j == 0) ? d.im : j - 1;  // cp0(o, i-1), src/umwm_stress.F90:219,316
-    const T icp = T(1) / bld<T>(cpr, (uint32_t)jc * E);
-    const T kdk = bld<T>(row_rsrc(d.kdk, o, ldi), vj);
+    const T icp = T(1) / cp_prev;
     const rsrc_t sp = freq_rsrc(d.spart, o, 4, ldi);
     if constexpr (sizeof(T) == 4) {
       bst16(sp, (uint32_t)j * 16u, f32x4{tx * icp, ty * icp, hs * kdk, T(0)});
