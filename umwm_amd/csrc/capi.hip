@@ -281,16 +281,17 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   h.snl_arg = sarg; h.bf1r = b1; h.bf2r = b2; h.dwn = dwn; h.cth_curv = ccv; h.sth_curv = scv;
   h.f = f; h.th = th; h.cth = ct; h.sth = st; h.c2th = c2; h.s2th = s2; h.cth2pp = c2pp;
   {
-    std::vector<double> d4((size_t)4 * pm);
+    const size_t n4 = (size_t)4 * (pm + kDirPad);
+    std::vector<double> d4(n4, 0.0);
     for (int p = 0; p < pm; ++p) {
       d4[4 * p] = S.cth[p];
       d4[4 * p + 1] = S.sth[p];
       d4[4 * p + 2] = S.c2th[p];
       d4[4 * p + 3] = S.s2th[p];
     }
-    T* dd = A.template alloc<T>((size_t)4 * pm);
+    T* dd = A.template alloc<T>(n4);
     if (!dd) { delete D; return fail(UMWM_ERR_NOMEM, "device allocation failed"); }
-    if ((rc = upload<T>(dd, d4, (size_t)4 * pm, s))) { delete D; return rc; }
+    if ((rc = upload<T>(dd, d4, n4, s))) { delete D; return rc; }
     h.dir4 = dd;
     T* dw = A.template alloc<T>((size_t)om);
     if (!dw) { delete D; return fail(UMWM_ERR_NOMEM, "device allocation failed"); }

@@ -73,7 +73,10 @@ struct DomainDev {
   const T *cth_curv, *sth_curv;
   // spectral constants
   const T *f, *th, *cth, *sth, *c2th, *s2th, *cth2pp;
-  const T* dir4;  // per direction {cth, sth, cos 2th, sin 2th}, one 4-value record
+  // per direction {cth, sth, cos 2th, sin 2th}, one 4-value record, then
+  // kDirPad zero records so that a wave's unused direction slots index it
+  // without a clamp
+  const T* dir4;
   const T* dom;   // frequency increment d omega per bin
   const int32_t *pl, *pr;
   // per unrolled point (ldq)
@@ -119,6 +122,11 @@ enum FusedTable {
 };
 constexpr int kRatesSlots = 12;
 constexpr int kIntegrateSlots = 8;
+
+// Zero records after the pm records of DomainDev::dir4.  A sweep wave w of
+// W <= 8 reads slots w*ceil(pm/W) + q, q < DMAX <= 36, which stay below
+// pm + W - 1 + 36 <= pm + 43.
+constexpr int kDirPad = 48;
 
 // One workgroup of a fused launch: which domain, which 64 sea points.
 struct BlockTask {

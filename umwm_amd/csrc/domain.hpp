@@ -19,7 +19,9 @@ struct Tile {
   int D1 = 9, W1 = 4;
   int D2 = 9, W2 = 4;
   int NB = 2;     // advect: directions per prefetch block
-  int pmax = 36;  // largest pm of the domains launched together
+  // pm of the domains launched together (a batch launches each pm class
+  // on its own); W D = pmax selects the sweeps' mask-free variants
+  int pmax = 36;
 };
 
 bool tile_supported(int D);

@@ -358,7 +358,7 @@ __device__ __forceinline__ T sin_m(const SinFreq<T>& q, T y, T m) {
 // ---------------------------------------------------------------------
 // k_rates
 // ---------------------------------------------------------------------
-template <typename T, int DMAX>
+template <typename T, int DMAX, bool FULL>
 __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ doms,
                                                const BlockTask* __restrict__ tasks, int first,
                                                int rotation) {
@@ -372,8 +372,9 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
   const bool valid = active && j < d.im;
   const int pm = d.pm, om = d.om;
   const size_t ldi = d.ldi, ldq = d.ldq;
-  const int D = (pm + W - 1) / W, p0 = w * D;
-  const int nd = max(0, min(D, pm - p0));  // directions of this wave (uniform)
+  const int D = FULL ? DMAX : (pm + W - 1) / W, p0 = w * D;
+  // directions of this wave (uniform); FULL launches have W * DMAX = pm
+  const int nd = FULL ? DMAX : max(0, min(D, pm - p0));
   const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
   const uint32_t vr = (uint32_t)j * (uint32_t)(FT_N * sizeof(T));
   const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);  // bytes of one direction row
@@ -409,15 +410,16 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     const double hdth = 0.5 * d.dth;
     // overrun factor per direction of this wave (frequency independent).
     // Slots q >= nd are computed but never used: their spectrum loads go
-    // through a zero-range descriptor and read 0, and they are masked out
-    // of the rate maximum.  With no per-slot branch the compiler can
-    // interleave the directions.
+    // through a zero-range descriptor and read 0, their direction records
+    // are dir4's zero padding, and they are masked out of the rate
+    // maximum.  With no per-slot branch the compiler can interleave the
+    // directions.
     const rsrc_t rz = make_rsrc(eb, 0u);
     T mq[DMAX];
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) {
       T ct, st, c2, s2;
-      cld4(dir4, q < nd ? p0 + q : pm - 1, ct, st, c2, s2);
+      cld4(dir4, p0 + q, ct, st, c2, s2);
       mq[q] = s.cw * ct + s.sw * st > T(0) ? fs2 : T(1);
     }
     double A0 = 0, AC = 0, AS = 0;
@@ -427,7 +429,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     {
       const rsrc_t ep = freq_rsrc(eb, 0, pm, ldi);
 #pragma unroll
-      for (int q = 0; q < DMAX; ++q) ec[q] = bld<T>(q < nd ? ep : rz, vj, (p0 + q) * rb);
+      for (int q = 0; q < DMAX; ++q) ec[q] = bld<T>(FULL || q < nd ? ep : rz, vj, (p0 + q) * rb);
     }
     // FP32 derives the tables from k (TabGen); FP64 loads the record
     constexpr bool kDerive = sizeof(T) == 4;
@@ -439,7 +441,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       T en[DMAX];
       const rsrc_t epn = freq_rsrc(eb, on, pm, ldi);
 #pragma unroll
-      for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(q < nd ? epn : rz, vj, (p0 + q) * rb);
+      for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(FULL || q < nd ? epn : rz, vj, (p0 + q) * rb);
       float knext = 0.f;
       if constexpr (kDerive) {
         knext = tg.load_k(o + 4);
@@ -473,7 +475,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) {
         T ct, st, c2, s2;
-        cld4(dir4, q < nd ? p0 + q : pm - 1, ct, st, c2, s2);
+        cld4(dir4, p0 + q, ct, st, c2, s2);
         const T e = ec[q];
         const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
         T dm = fma(qc, c2, fma(qs, s2, q0));
@@ -483,7 +485,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
         // base is taken to be here, without a clamp instruction
         const T sds = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk));
         const T r = ss - fma(sds, cs, crest);
-        rq = fmax(rq, q < nd ? fabs(r) : T(0));
+        rq = fmax(rq, FULL || q < nd ? fabs(r) : T(0));
         m0 += e;
         mc = fma(e, c2, mc);
         ms = fma(e, s2, ms);
@@ -632,7 +634,7 @@ __device__ __forceinline__ Prefix<T> load_prefix(const T* pfx, size_t ldi, int o
   return r;
 }
 
-template <typename T, int DMAX>
+template <typename T, int DMAX, bool FULL>
 __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restrict__ doms,
                                                    const BlockTask* __restrict__ tasks) {
   const BlockTask tk = tasks[blockIdx.x];
@@ -644,8 +646,8 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   const bool valid = j < d.im;
   const int pm = d.pm, om = d.om;
   const size_t ldi = d.ldi;
-  const int D = (pm + W - 1) / W, p0 = w * D;
-  const int nd = max(0, min(D, pm - p0));
+  const int D = FULL ? DMAX : (pm + W - 1) / W, p0 = w * D;
+  const int nd = FULL ? DMAX : max(0, min(D, pm - p0));
   const T dts = T(sc->dts);
   const T dtx = dts * Tr<T>::kExpScale;  // dts in the exponential's base
   const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
@@ -674,7 +676,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
 #pragma unroll
   for (int q = 0; q < DMAX; ++q) {
     T ct, st, c2, s2;
-    cld4(d.dir4, q < nd ? p0 + q : pm - 1, ct, st, c2, s2);
+    cld4(d.dir4, p0 + q, ct, st, c2, s2);
     mq[q] = s.cw * ct + s.sw * st > T(0) ? fs2 : T(1);
     ctq[q] = ct;
     stq[q] = st;
@@ -691,7 +693,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   {
     const rsrc_t ep = freq_rsrc(eb, 0, pm, ldi);
 #pragma unroll
-    for (int q = 0; q < DMAX; ++q) ec[q] = bld<T>(q < nd ? ep : rz, vj, (p0 + q) * rb);
+    for (int q = 0; q < DMAX; ++q) ec[q] = bld<T>(FULL || q < nd ? ep : rz, vj, (p0 + q) * rb);
   }
   FreqTab<T> tc;
   Prefix<T> pc{};
@@ -708,7 +710,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     T en[DMAX];
     const rsrc_t epn = freq_rsrc(eb, on, pm, ldi);
 #pragma unroll
-    for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(q < nd ? epn : rz, vj, (p0 + q) * rb);
+    for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(FULL || q < nd ? epn : rz, vj, (p0 + q) * rb);
     Prefix<T> pn{};
     float knext = 0.f;
     if constexpr (kDerive) {
@@ -768,11 +770,11 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
           const T num = fmax(ss, T(0)) - ctail;
           const T t = fmax(num, T(0)) * (icfc * Tr<T>::rcp(dm));
           const T es_ = Tr<T>::ex(fma(ipow, Tr<T>::lg(t), -lk4));  // k^-4 t^(1/p)
-          bst(q < nd ? rs0 : rz, vtail, T(0.5) * (e + es_), so);
+          bst(FULL || q < nd ? rs0 : rz, vtail, T(0.5) * (e + es_), so);
         }
         const T h1 = fma(hdb1, X, H1[q]);  // bf1r(o-1) X(o) into bin o-1
         if constexpr (decltype(FIN)::value)  // bf2r(o-2) X(o) completes bin o-2
-          bst(q < nd ? rs2 : rz, vfin, fma(hdb2, X, H2[q]), so);
+          bst(FULL || q < nd ? rs2 : rz, vfin, fma(hdb2, X, H2[q]), so);
         H2[q] = h1;
         H1[q] = T(0.5) * (e + G);
       }
@@ -1078,11 +1080,20 @@ int launch_all(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int
   const dim3 grid(ntasks);
   const size_t sh1 = tile.W1 > 1 ? (size_t)2 * 3 * tile.W1 * 64 * sizeof(double)
                                  : (size_t)64 * sizeof(double);
+  // a launch whose waves all hold DMAX directions (W D = pm) takes the
+  // variant with no per-slot masks
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL((k_rates<T, D1>), grid, dim3(64 * tile.W1), sh1, s, doms, tasks, first,
-                     rotation);
+  if (tile.W1 * D1 == tile.pmax)
+    hipLaunchKernelGGL((k_rates<T, D1, true>), grid, dim3(64 * tile.W1), sh1, s, doms, tasks,
+                       first, rotation);
+  else
+    hipLaunchKernelGGL((k_rates<T, D1, false>), grid, dim3(64 * tile.W1), sh1, s, doms, tasks,
+                       first, rotation);
   if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL((k_integrate<T, D2>), grid, dim3(64 * tile.W2), 0, s, doms, tasks);
+  if (tile.W2 * D2 == tile.pmax)
+    hipLaunchKernelGGL((k_integrate<T, D2, true>), grid, dim3(64 * tile.W2), 0, s, doms, tasks);
+  else
+    hipLaunchKernelGGL((k_integrate<T, D2, false>), grid, dim3(64 * tile.W2), 0, s, doms, tasks);
   if (keep) hipLaunchKernelGGL((k_keep<T>), grid, dim3(64), 0, s, doms, tasks);
   if (ev) (void)hipEventRecord(ev[2], s);
   // advect: one wave per (64 points, frequency), FW frequencies per
