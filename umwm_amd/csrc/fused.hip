@@ -672,7 +672,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   // direction loop has no per-slot branch and the compiler can interleave
   // the directions.
   const rsrc_t rz = make_rsrc(eb, 0u);
-  T mq[DMAX], ctq[DMAX], stq[DMAX], c2q[DMAX], s2q[DMAX];
+  T mq[DMAX], ctq[DMAX], stq[DMAX];
 #pragma unroll
   for (int q = 0; q < DMAX; ++q) {
     T ct, st, c2, s2;
@@ -680,8 +680,6 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     mq[q] = s.cw * ct + s.sw * st > T(0) ? fs2 : T(1);
     ctq[q] = ct;
     stq[q] = st;
-    c2q[q] = c2;
-    s2q[q] = s2;
   }
   // per direction, the partial E* = (E^n + partial E_s) / 2 of the two
   // previous frequencies, still waiting for their snl gains
@@ -734,7 +732,11 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     const T Lk = spow * lk4 + Tr<T>::lg(cf);  // sds before cothkd
     const T c0 = -dtx * crest, dtc = dtx * coth;
     const T icfc = Tr<T>::rcp(cf * coth);
-    const T q0 = T(1) + mssf * pc.a0, qc = mssf * pc.ac, qs = mssf * pc.as;
+    // dm = q0 + qc cos 2th + qs sin 2th = (q0 - qc) + ct (2 qc ct + 2 qs st),
+    // with cos 2th = 2 ct^2 - 1 and sin 2th = 2 ct st: the sweep keeps
+    // only ct and st per direction
+    const T qc = mssf * pc.ac;
+    const T q0m = T(1) + mssf * pc.a0 - qc, qc2 = T(2) * qc, qs2 = T(2) * mssf * pc.as;
     const bool prog = o <= oc;
     const bool in1 = (o >= 1) && (o - 1 <= oc);  // bin o-1 is prognostic
     const bool fin = valid && (o >= 2) && (o - 2 <= oc);  // bin o-2 completes now
@@ -743,7 +745,9 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     const uint32_t vfin = fin ? vj : kDropOffset;
     const bool anyfin = __any(fin), anytail = __any(valid && !prog);
     const rsrc_t rs0 = freq_rsrc(esb, o, pm, ldi);
-    const rsrc_t rs2 = freq_rsrc(esb, o >= 2 ? o - 2 : 0, pm, ldi);
+    // max(o, 2) - 2 rather than a conditional: the latter becomes a vector
+    // saturating subtract, which moves the descriptor to vector registers
+    const rsrc_t rs2 = freq_rsrc(esb, max(o, 2) - 2, pm, ldi);
     // One pass over this wave's directions; PROG / TAIL / FIN say whether
     // any lane of the wave integrates a prognostic bin, fills a tail bin
     // or completes bin o-2 at this frequency (wave-uniform, so each
@@ -752,10 +756,10 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) {
         const uint32_t so = (uint32_t)(p0 + q) * rb;
-        const T ct = ctq[q], st = stq[q], c2 = c2q[q], s2 = s2q[q];
+        const T ct = ctq[q], st = stq[q];
         const T e = ec[q];
         const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
-        T dm = fma(qc, c2, fma(qs, s2, q0));
+        T dm = fma(ct, fma(qc2, ct, qs2 * st), q0m);
         dm = dm * dm;
         const T sdsp = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk));  // |e|: see k_rates
         const T X = sdsp * e;
