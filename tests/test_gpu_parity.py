@@ -301,3 +301,48 @@ def test_run_dtg_runs_ahead_like_single_substeps():
     finally:
         a.close()
         b.close()
+
+
+# Tile shapes of the two sweep kernels at pm = 36 and pm = 24: each wave
+# holding exactly DMAX directions (with directions paired with their
+# opposite when DMAX is even, unpaired when it is odd), waves holding fewer
+# directions than DMAX, and a wave holding none.
+TILES = {
+    36: [(36, 1, 18, 2, 4), (18, 2, 12, 3, 2), (6, 6, 6, 6, 4), (9, 4, 9, 4, 2),
+         (36, 2, 18, 3, 2), (9, 5, 6, 7, 2)],
+    24: [(6, 4, 6, 4, 2), (18, 2, 12, 2, 4), (36, 1, 3, 8, 2)],
+}
+
+
+@pytest.mark.parametrize("pm,tile", [(pm, t) for pm, ts in TILES.items() for t in ts])
+def test_fused_tiles_fp64(pm, tile, oracle_lib):
+    """The fused substep under each tile shape against the oracle."""
+    wl = tiny_case("winds_currents", seed=13, pm=pm)
+    o, d = make_pair(wl)
+    d.set_tile(*tile)
+    for step in range(4):
+        o.substep()
+        d.substep()
+        err = relerr(d.get("e"), o.get("e"))
+        assert err <= TOL_RUN, f"tile {tile} step {step}: e rel err {err:.3e}"
+        err = pointwise_relerr(d.get("ustar"), o.get("ustar"), floor=1e-12)
+        assert err <= TOL_RUN, f"tile {tile} step {step}: ustar rel err {err:.3e}"
+    d.close()
+
+
+@pytest.mark.parametrize("tile", TILES[36][:3])
+def test_fused_tiles_fp32(tile):
+    """FP32 under other tiles against FP32 under the default tile: only the
+    order of the directional sums differs."""
+    wl = tiny_case("land_shelf", seed=17, pm=36)
+    a, b = _fresh(wl, 4), _fresh(wl, 4)
+    try:
+        b.set_tile(*tile)
+        for _ in range(4):
+            a.substep()
+            b.substep()
+        assert relerr(b.get("e"), a.get("e")) <= 1e-5
+        assert pointwise_relerr(b.get("ustar"), a.get("ustar"), floor=1e-12) <= 1e-5
+    finally:
+        a.close()
+        b.close()

@@ -415,11 +415,21 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     // maximum.  With no per-slot branch the compiler can interleave the
     // directions.
     const rsrc_t rz = make_rsrc(eb, 0u);
+    // FULL launches with an even DMAX pair each direction p with p + pm/2
+    // (theta + pi): cos and sin change sign while cos 2theta and sin 2theta
+    // do not, so a pair shares its direction record, the mss factor and one
+    // update of the moments.  Wave w then holds pairs w DMAX/2 + h in slots
+    // h and h + DMAX/2, which also halves the direction records the loop
+    // keeps in scalar registers.
+    constexpr bool PAIR = FULL && DMAX % 2 == 0;
+    constexpr int H = DMAX / 2;
+    const int ph = w * H, half = pm / 2;
+    auto dirp = [&](int q) { return PAIR ? (q < H ? ph + q : ph + q - H + half) : p0 + q; };
     T mq[DMAX];
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) {
       T ct, st, c2, s2;
-      cld4(dir4, p0 + q, ct, st, c2, s2);
+      cld4(dir4, dirp(q), ct, st, c2, s2);
       mq[q] = s.cw * ct + s.sw * st > T(0) ? fs2 : T(1);
     }
     double A0 = 0, AC = 0, AS = 0;
@@ -429,7 +439,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     {
       const rsrc_t ep = freq_rsrc(eb, 0, pm, ldi);
 #pragma unroll
-      for (int q = 0; q < DMAX; ++q) ec[q] = bld<T>(FULL || q < nd ? ep : rz, vj, (p0 + q) * rb);
+      for (int q = 0; q < DMAX; ++q) ec[q] = bld<T>(FULL || q < nd ? ep : rz, vj, dirp(q) * rb);
     }
     // FP32 derives the tables from k (TabGen); FP64 loads the record
     constexpr bool kDerive = sizeof(T) == 4;
@@ -441,7 +451,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       T en[DMAX];
       const rsrc_t epn = freq_rsrc(eb, on, pm, ldi);
 #pragma unroll
-      for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(FULL || q < nd ? epn : rz, vj, (p0 + q) * rb);
+      for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(FULL || q < nd ? epn : rz, vj, dirp(q) * rb);
       float knext = 0.f;
       if constexpr (kDerive) {
         knext = tg.load_k(o + 4);
@@ -472,23 +482,44 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       const T cs = T(1) - (tc[FT_B1] + tc[FT_B2]);                // snl_arg
       const T q0 = T(1) + mssf * pa0, qc = mssf * pac, qs = mssf * pas;
       T m0 = 0, mc = 0, ms = 0, rq = 0;
+      // (e k4)^p of |e|: a bin emptied by round-off to about -1e-30 gives a
+      // vanishing sds, as the 0 the reference's power of a non-positive
+      // base is taken to be here, without a clamp instruction
+      auto sds_of = [&](T dm, T e) { return dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk)); };
+      if constexpr (PAIR) {
 #pragma unroll
-      for (int q = 0; q < DMAX; ++q) {
-        T ct, st, c2, s2;
-        cld4(dir4, p0 + q, ct, st, c2, s2);
-        const T e = ec[q];
-        const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
-        T dm = fma(qc, c2, fma(qs, s2, q0));
-        dm = dm * dm;
-        // (e k4)^p of |e|: a bin emptied by round-off to about -1e-30 gives a
-        // vanishing sds, as the 0 the reference's power of a non-positive
-        // base is taken to be here, without a clamp instruction
-        const T sds = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk));
-        const T r = ss - fma(sds, cs, crest);
-        rq = fmax(rq, FULL || q < nd ? fabs(r) : T(0));
-        m0 += e;
-        mc = fma(e, c2, mc);
-        ms = fma(e, s2, ms);
+        for (int h = 0; h < H; ++h) {
+          T ct, st, c2, s2;
+          cld4(dir4, ph + h, ct, st, c2, s2);
+          const T ea = ec[h], eb2 = ec[h + H];
+          const T u = fma(sf.A, ct, sf.B * st);  // x of theta + pi is -u - cp0
+          const T xa = u - sf.cp0, xb = -u - sf.cp0;
+          const T ssa = sin_m(sf, fabs(xa) * xa, mq[h]), ssb = sin_m(sf, fabs(xb) * xb, mq[h + H]);
+          T dm = fma(qc, c2, fma(qs, s2, q0));
+          dm = dm * dm;
+          const T ra = ssa - fma(sds_of(dm, ea), cs, crest);
+          const T rb2 = ssb - fma(sds_of(dm, eb2), cs, crest);
+          rq = fmax(rq, fmax(fabs(ra), fabs(rb2)));
+          const T ep2 = ea + eb2;
+          m0 += ep2;
+          mc = fma(ep2, c2, mc);
+          ms = fma(ep2, s2, ms);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < DMAX; ++q) {
+          T ct, st, c2, s2;
+          cld4(dir4, p0 + q, ct, st, c2, s2);
+          const T e = ec[q];
+          const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
+          T dm = fma(qc, c2, fma(qs, s2, q0));
+          dm = dm * dm;
+          const T r = ss - fma(sds_of(dm, e), cs, crest);
+          rq = fmax(rq, FULL || q < nd ? fabs(r) : T(0));
+          m0 += e;
+          mc = fma(e, c2, mc);
+          ms = fma(e, s2, ms);
+        }
       }
       if (o <= oc) rmax = fmax(rmax, rq);  // rates of prognostic bins only
       if (rotation) {
@@ -500,7 +531,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
         if (o <= oc) {
           rm = -INFINITY;
           for (int q = 0; q < nd; ++q) {
-            const int p = p0 + q;
+            const int p = dirp(q);
             rm = fmax(rm, (double)rot_dir(dcx, dcy, cld(d.sth, p), cld(d.cth, p), dvc, duc, odx,
                                           ody));
           }
