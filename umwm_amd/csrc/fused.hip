@@ -70,9 +70,6 @@ __device__ __forceinline__ void pstore(U* a, size_t n, int j, U v) {
   bst(make_rsrc(a, (uint32_t)(n * sizeof(U))), (uint32_t)j * (uint32_t)sizeof(U), v);
 }
 
-__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
-
 // Per-point inputs of the source terms (src/umwm_source_functions.F90:21-52,
 // :174-177).
 template <typename T>
