@@ -47,6 +47,9 @@ struct Scalars {
   uint32_t counter_rot;
   int32_t anycur;    // any current != 0 (zerocurrents = !anycur)
   int32_t last;      // the latest active substep reached sumt >= dtg
+  // the rates kernel's running grid min of pts and max of the rotation
+  // rate as order-preserving keys (0 = none yet; its last block clears them)
+  unsigned long long pts_key, rot_key;
 };
 
 // Everything a kernel needs to know about one domain.

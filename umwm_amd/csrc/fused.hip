@@ -570,7 +570,15 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     }
     if (!valid) rotmax = -INFINITY;
   }
-  // block reductions -> per-domain partials; the last block finishes
+  // Block reductions, then the grid-wide ones through two agent-scope
+  // atomics per block on order-preserving integer keys (okey) kept in the
+  // domain's Scalars: an atomic max of ~okey(min pts) and of okey(max
+  // rotation rate).  The last block to arrive reads and clears them.  There
+  // are no per-block partials in memory, hence no release fence: on this
+  // multi-XCD GPU an agent-scope release writes back and invalidates the
+  // L2 and cost about a fifth of the kernel.  Device atomics are performed
+  // at the device's coherence point; the wait for their completion before
+  // the arrival count orders them ahead of it.
   __shared__ double bred[2][16];
   __shared__ int is_last;
   __syncthreads();
@@ -586,35 +594,20 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       mn = fmin(mn, bred[0][q]);
       mx = fmax(mx, bred[1][q]);
     }
-    d.part_min[tk.local] = mn;
-    d.part_max[tk.local] = mx;
-    __threadfence();
+    atomicMax(&sc->pts_key, ~okey(mn));
+    atomicMax(&sc->rot_key, okey(mx));
+    // both atomics complete (s_waitcnt 0) before the arrival count
+    __builtin_amdgcn_s_waitcnt(0);
     const unsigned tkt = atomicAdd(&sc->counter, 1u);
     is_last = (tkt == (unsigned)d.nblocks - 1);
   }
   __syncthreads();
   if (!is_last) return;
-  __threadfence();
-  double mn = INFINITY, mx = -INFINITY;
-  for (int q = threadIdx.x; q < d.nblocks; q += blockDim.x) {
-    mn = fmin(mn, __hip_atomic_load(&d.part_min[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    mx = fmax(mx, __hip_atomic_load(&d.part_max[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  }
-  mn = wave_min(mn);
-  mx = wave_max(mx);
-  __syncthreads();
-  if (lane == 0) {
-    bred[0][w] = mn;
-    bred[1][w] = mx;
-  }
-  __syncthreads();
   if (threadIdx.x == 0) {
-    mn = INFINITY;
-    mx = -INFINITY;
-    for (int q = 0; q < W; ++q) {
-      mn = fmin(mn, bred[0][q]);
-      mx = fmax(mx, bred[1][q]);
-    }
+    const unsigned long long kmin = atomicExch(&sc->pts_key, 0ull);
+    const unsigned long long kmax = atomicExch(&sc->rot_key, 0ull);
+    const double mn = kmin ? okey_inv(~kmin) : INFINITY;
+    const double mx = kmax ? okey_inv(kmax) : -INFINITY;
     if (active) {
       // src/umwm_physics.F90:44-58
       double dts = fmin(fmin(mn, d.dtamin), d.dtg - sc->sumt);

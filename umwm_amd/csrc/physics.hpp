@@ -215,4 +215,15 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
+// Order-preserving map of doubles (NaN aside) onto unsigned 64-bit keys, so
+// that an integer atomic max combines doubles; ~okey(x) does the same for a
+// min.  No double maps to key 0, which therefore stands for "no value".
+__device__ inline unsigned long long okey(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ inline double okey_inv(unsigned long long k) {
+  return __longlong_as_double((long long)((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k));
+}
+
 }  // namespace umwm
