@@ -405,7 +405,6 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     const T tpsf = T(d.twopisds_fac), spow = T(d.sds_power), mssf = T(d.mss_fac);
     const T nu4 = T(4 * d.nu_water), ig = T(1 / d.g);
     const double hdth = 0.5 * d.dth;
-    // overrun factor per direction of this wave (frequency independent).
     // Slots q >= nd are computed but never used: their spectrum loads go
     // through a zero-range descriptor and read 0, their direction records
     // are dir4's zero padding, and they are masked out of the rate
@@ -422,12 +421,19 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     constexpr int H = DMAX / 2;
     const int ph = w * H, half = pm / 2;
     auto dirp = [&](int q) { return PAIR ? (q < H ? ph + q : ph + q - H + half) : p0 + q; };
+    // overrun factor of slot q: fieldscale2 where the wind overruns the
+    // direction, else 1; the paired loop reads it as bit q of mb (one
+    // register instead of DMAX)
+    using MB = std::conditional_t<(DMAX > 32), uint64_t, uint32_t>;
+    MB mb = 0;
     T mq[DMAX];
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) {
       T ct, st, c2, s2;
       cld4(dir4, dirp(q), ct, st, c2, s2);
-      mq[q] = s.cw * ct + s.sw * st > T(0) ? fs2 : T(1);
+      const bool over = s.cw * ct + s.sw * st > T(0);
+      mq[q] = over ? fs2 : T(1);
+      if (over) mb |= MB(1) << q;
     }
     double A0 = 0, AC = 0, AS = 0;
     T rmax = 0;
@@ -483,6 +489,11 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       // vanishing sds, as the 0 the reference's power of a non-positive
       // base is taken to be here, without a clamp instruction
       auto sds_of = [&](T dm, T e) { return dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk)); };
+      // an opaque copy per frequency (an empty asm statement): without it the
+      // compiler hoists the DMAX bit tests out of the frequency loop into
+      // DMAX registers, which costs a wave per SIMD at DMAX = 36
+      MB mbl = mb;
+      asm volatile("" : "+v"(mbl));
       if constexpr (PAIR) {
 #pragma unroll
         for (int h = 0; h < H; ++h) {
@@ -491,7 +502,10 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
           const T ea = ec[h], eb2 = ec[h + H];
           const T u = fma(sf.A, ct, sf.B * st);  // x of theta + pi is -u - cp0
           const T xa = u - sf.cp0, xb = -u - sf.cp0;
-          const T ssa = sin_m(sf, fabs(xa) * xa, mq[h]), ssb = sin_m(sf, fabs(xb) * xb, mq[h + H]);
+          const T ya = fabs(xa) * xa, yb = fabs(xb) * xb;
+          const T na = ((mbl >> h) & 1) ? sf.neg2 : sf.neg1;
+          const T nb = ((mbl >> (h + H)) & 1) ? sf.neg2 : sf.neg1;
+          const T ssa = ya * (ya > T(0) ? sf.pos : na), ssb = yb * (yb > T(0) ? sf.pos : nb);
           T dm = fma(qc, c2, fma(qs, s2, q0));
           dm = dm * dm;
           const T ra = ssa - fma(sds_of(dm, ea), cs, crest);
