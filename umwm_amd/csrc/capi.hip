@@ -398,8 +398,6 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   // reduction scratch and scalars
   h.nblocks = (S.im + kLanes - 1) / kLanes;
   h.block0 = 0;
-  h.part_min = A.template alloc<double>(std::max(h.nblocks, 1));
-  h.part_max = A.template alloc<double>(std::max(h.nblocks, 1));
   h.pfx = A.template alloc<T>(4 * nt);
   h.spart = A.template alloc<T>(4 * nt);
   h.sc = A.template alloc<Scalars>(1);

@@ -111,8 +111,6 @@ struct DomainDev {
   T* ftab;
   T* pfx;
   T* spart;
-  // reduction scratch
-  double *part_min, *part_max;
   Scalars* sc;
 };
 

@@ -197,27 +197,21 @@ __device__ inline void last_block_dts(const DomainDev<T>& d, double bmin, int fi
   bmin = wave_min(bmin);
   if (lane == 0) red[w] = bmin;
   __syncthreads();
+  // grid min through an atomic max of ~okey (as in the fused rates kernel:
+  // no per-block partials, so no L2-flushing release fence)
   if (threadIdx.x == 0) {
     double m = red[0];
     for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmin(m, red[q]);
-    d.part_min[blockIdx.x] = m;
-    __threadfence();
+    atomicMax(&d.sc->pts_key, ~okey(m));
+    __builtin_amdgcn_s_waitcnt(0);  // the atomic completes before the arrival count
     unsigned t = atomicAdd(&d.sc->counter, 1u);
     is_last = (t == gridDim.x - 1);
   }
   __syncthreads();
   if (!is_last) return;
-  __threadfence();
-  double m = INFINITY;
-  for (int q = threadIdx.x; q < (int)gridDim.x; q += blockDim.x)
-    m = fmin(m, __hip_atomic_load(&d.part_min[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  m = wave_min(m);
-  __syncthreads();
-  if (lane == 0) red[w] = m;
-  __syncthreads();
   if (threadIdx.x == 0) {
-    double mn = red[0];
-    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) mn = fmin(mn, red[q]);
+    const unsigned long long k = atomicExch(&d.sc->pts_key, 0ull);
+    const double mn = k ? okey_inv(~k) : INFINITY;
     Scalars* sc = d.sc;
     double dts = fmin(fmin(mn, d.dtamin), d.dtg - sc->sumt);
     if (!d.restart && first) dts = 0;
@@ -359,17 +353,15 @@ __global__ void k_rotation(DomainDev<T> d) {
   if (threadIdx.x == 0) {
     double m = red[0];
     for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmax(m, red[q]);
-    d.part_max[blockIdx.x] = m;
-    __threadfence();
+    atomicMax(&d.sc->rot_key, okey(m));
+    __builtin_amdgcn_s_waitcnt(0);  // the atomic completes before the arrival count
     unsigned t = atomicAdd(&d.sc->counter_rot, 1u);
     is_last = (t == gridDim.x - 1);
   }
   __syncthreads();
   if (!is_last || threadIdx.x != 0) return;
-  __threadfence();
-  double m = -INFINITY;
-  for (int q = 0; q < (int)gridDim.x; ++q)
-    m = fmax(m, __hip_atomic_load(&d.part_max[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  const unsigned long long k = atomicExch(&d.sc->rot_key, 0ull);
+  const double m = k ? okey_inv(k) : -INFINITY;
   Scalars* sc = d.sc;
   sc->maxrot = m;
   sc->dtr_temp = (m <= d.tiny) ? sc->dts : d.dth / m;
