@@ -882,7 +882,9 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
   const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);
   constexpr uint32_t E = sizeof(T);
   const T kdta = T(0.25) * T(sc->dta), dtr = T(sc->dtr);
-  const bool anycur = sc->anycur != 0;
+  // wave-uniform in a scalar register, so that the current terms below are
+  // a scalar branch rather than an exec-masked block issued for every bin
+  const bool anycur = __builtin_amdgcn_readfirstlane(sc->anycur) != 0;
 
   const PointState<T> s = point_stored(d, j);
   const bool prog = o <= s.oc, rot = o <= pload(d.oc_rot, ldi, j);
