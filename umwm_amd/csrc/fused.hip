@@ -2,11 +2,12 @@
 //
 // One substep of the reference (src/umwm_top.F90:95-115) is four kernels:
 //   k_rates      sin + sds + rate + per-point max|rate| -> physics_time_step,
-//                then a grid-wide min (last-block reduction) -> dts, dta,
-//                sumt, dtr (src/umwm_physics.F90:22-58,
-//                src/umwm_advection.F90:329-365).  Reads E^n once and
-//                writes the exclusive frequency prefix of the directional
-//                cos^2 convolution (3 values per frequency and point).
+//                then a grid-wide min (atomic max on order-preserving keys,
+//                finished by the last block) -> dts, dta, sumt, dtr
+//                (src/umwm_physics.F90:22-58, src/umwm_advection.F90:329-365).
+//                Reads E^n once and writes the exclusive frequency prefix of
+//                the directional cos^2 convolution (3 values per frequency
+//                and point).
 //   k_integrate  sin + sds + snl + exponential integration + diagnostic
 //                tail (src/umwm_physics.F90:60-84); writes only
 //                E* = (E^n + E_s)/2 (es).
@@ -30,7 +31,9 @@
 // moments per frequency, exchanged between waves through LDS when W > 1,
 // which turns the O(pm^2) convolution into O(pm).  Each sweep prefetches
 // the next frequency's spectrum (and, in FP32, its tables) while computing
-// the current one.
+// the current one.  Launches whose waves all hold exactly DMAX directions
+// take a variant without per-slot masks, in which k_rates also pairs each
+// direction with its opposite.
 //
 // snl(o) = bf1r(o) X(o+1) + bf2r(o) X(o+2) - snl_fac X(o), X = sds e
 // (src/umwm_source_functions.F90:165-171) is folded into the integration
