@@ -346,3 +346,26 @@ def test_fused_tiles_fp32(tile):
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("precision", [4, 8])
+def test_dts_is_the_exact_grid_min(precision):
+    """The rates kernel's grid-wide reduction over about 700 workgroups
+    spread over every XCD: dts = min(min_i pts(i), dtamin, dtg - sumt)
+    exactly, with pts the per-point physics_time_step the same substep
+    stored (src/umwm_physics.F90:41-58)."""
+    wl = configs.c2(seed=4, mm=360, nm=180, hours=1)
+    d = _fresh(wl, precision)
+    try:
+        assert d.im > 40000
+        d.substep()  # cold first substep: dts = 0
+        assert d.scalar("dts") == 0.0
+        for _ in range(3):
+            sumt0 = d.scalar("sumt")
+            d.substep()
+            pts = np.asarray(d.get("physics_time_step"))[: d.im]
+            want = min(float(pts.min()), d.scalar("dtamin"), wl.params.dtg - sumt0)
+            assert d.scalar("dts") == want
+            assert d.scalar("sumt") == sumt0 + want
+    finally:
+        d.close()
