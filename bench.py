@@ -312,6 +312,10 @@ def main():
                                       tune=not (args.no_tune or args.tile))
         tile = tile or (tuple(args.tile) if args.tile else None)
         rate = args.steps * bins / wall
+        # prognostic bins (o <= oc, the rest is the diagnostic tail), for
+        # information: sum over sea points of (oc + 1) pm, oc 0-based
+        oc = np.asarray(d.get("oc"))[: d.im].astype(np.int64)
+        prog = int((oc + 1).sum()) * d.pm
         log(f"bench: {tag} {rate / 1e9:.2f} G bin-updates/s, kernels ms "
             + ", ".join(f"{k} {v:.3f}" for k, v in zip(KERNELS, kms)))
         if not args.no_finish:
@@ -319,7 +323,7 @@ def main():
             t0 = time.perf_counter()
             finish_run(d, wl, args.hours)
             log(f"bench: {tag} run done in {time.perf_counter() - t0:.1f} s")
-        res[tag] = dict(rate=rate, wall=wall, kms=kms, bins=bins, im=d.im,
+        res[tag] = dict(rate=rate, wall=wall, kms=kms, bins=bins, im=d.im, prog=prog,
                         hs=d.get("ht"), tm=d.get("mwp"), ust=d.get("ustar"),
                         us=d.get("us")[0], vs=d.get("vs")[0], tile=tile)
         d.close()
@@ -366,6 +370,7 @@ def main():
                                 "log-normal depth, GRF winds 0-25 m/s, JONSWAP start, "
                                 f"{args.hours} h") if not args.small else "c2 quarter size",
                    "sea_points": r32["im"], "bins_per_step": r32["bins"],
+                   "prognostic_bins_per_step": r32["prog"],
                    "chosen": "fp32 fused", "plain": "fp64 fused"},
         "kernel_ms": dict(zip(KERNELS, [round(x, 4) for x in r32["kms"]])),
         "tile": r32["tile"],
