@@ -280,6 +280,10 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device-to-device copy first (scales HBM counters)")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU (how the path would shard over "
+              "several is a design note only), so --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
     if args.copy_probe:
         import torch
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -288,11 +292,6 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log("bench: copy probe 3 x 1 GiB done")
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU sharding is a design note "
-              "only (see DESIGN.md), so --gpus must be 1", file=sys.stderr)
-        sys.exit(2)
-
     mm, nm = (720, 360) if not args.small else (180, 90)
     log(f"bench: building c2 {mm}x{nm}")
     wl = configs.c2(seed=2, mm=mm, nm=nm, hours=args.hours)
