@@ -69,6 +69,35 @@ __device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, int32_t v, uint32_t
   __builtin_amdgcn_raw_buffer_store_b32((uint32_t)v, r, (int)voff, (int)soff, 0);
 }
 
+// Streaming variants for spectrum rows a kernel touches once: the
+// cache-policy operand of the gfx940-family buffer instructions takes nt
+// (bit 1, non-temporal) and sc1 (bit 4, device-scope write-through).  k_advect
+// writes E^(n+1) with sc1|nt, so its 1 GB of stores leaves no dirty lines in
+// the L2s for the end-of-kernel write-back, and reads E^n, which nothing
+// reads again in the substep, with nt.
+constexpr int kCpolNt = 2, kCpolSc1 = 16;
+template <typename T>
+__device__ __forceinline__ T bld_nt(rsrc_t r, uint32_t voff, uint32_t soff = 0);
+template <>
+__device__ __forceinline__ float bld_nt<float>(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff,
+                                                                        kCpolNt));
+}
+template <>
+__device__ __forceinline__ double bld_nt<double>(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff,
+                                                                         kCpolNt));
+}
+__device__ __forceinline__ void bst_stream(rsrc_t r, uint32_t voff, float v, uint32_t soff = 0) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff,
+                                        (int)soff, kCpolSc1 | kCpolNt);
+}
+__device__ __forceinline__ void bst_stream(rsrc_t r, uint32_t voff, double v, uint32_t soff = 0) {
+  using u2 = __attribute__((ext_vector_type(2))) uint32_t;
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)voff, (int)soff,
+                                        kCpolSc1 | kCpolNt);
+}
+
 // 16-byte accesses.  The builtins' unsigned vectors are always bit-cast as
 // a whole to a typed vector: extracting single elements of the builtin's
 // own return value does not reliably keep the wide load.

@@ -925,10 +925,10 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
       T v[NB];
 #pragma unroll
       for (int q = 0; q < NB; ++q)
-        v[q] = fma(T(2), bld<T>(ps, vj, (pb + q) * rb), -bld<T>(pe, vj, (pb + q) * rb));
+        v[q] = fma(T(2), bld<T>(ps, vj, (pb + q) * rb), -bld_nt<T>(pe, vj, (pb + q) * rb));
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
-        bst(pe, vst, v[q], (pb + q) * rb);
+        bst_stream(pe, vst, v[q], (pb + q) * rb);
         stress_acc(cld(cthb, pb + q), cld(sthb, pb + q), v[q]);
       }
     }
@@ -985,7 +985,7 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
         L[q].ew = bld<T>(ps, viw, so);
         L[q].en = bld<T>(ps, vin, so);
         L[q].es = bld<T>(ps, vis, so);
-        L[q].en0 = bld<T>(pe, vj, so);
+        L[q].en0 = bld_nt<T>(pe, vj, so);
       }
     };
     AdvLoads<T> A[NB], B[NB];
@@ -1017,7 +1017,7 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
         const T fr = fma(rl, rl > T(0) ? e0 : enext, -rr * (rr > T(0) ? eprev : e0));
         T v = fma(-kr, fr, fma(-kk, flux, Es));
         v = prog ? v * keep : Es;
-        bst(pe, vst, v, p * rb);
+        bst_stream(pe, vst, v, p * rb);
         stress_acc(ct, st, v);
         eprev = e0;
         rprev = rcur;
