@@ -896,8 +896,20 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
   // table), where 64 neighbours share a few cache lines
   const rsrc_t cpr = row_rsrc(d.cp0, o, ldi), cgr = row_rsrc(d.cg0, o, ldi);
   const T cp0 = bld<T>(cpr, vj);
-  const SinFreq<T> sf = sin_freq(s, bld<T>(row_rsrc(d.logl2overz, o, ldi), vj), cp0,
-                                 bld<T>(row_rsrc(d.fkovg, o, ldi), vj), sin_diss1, fs2);
+  T logl2, fkovg;
+  if constexpr (sizeof(T) == 4) {
+    // FP32 forms both from cp0 = 2 pi f / k (src/umwm_init.F90:1512-1521):
+    // f k / g = 2 pi f^2 / (g cp0), log(l2/z) with l2 = cp0 / (2 f) <= 20 m
+    const float fo = cld(d.f, o), icp = __builtin_amdgcn_rcpf(cp0);
+    fkovg = float(kTwoPiD / d.g) * fo * fo * icp;
+    const float l2 = 0.5f * cp0 * __builtin_amdgcn_rcpf(fo);
+    logl2 = l2 > 20.f ? float(log(20.0 / d.z))
+                      : 0.6931471805599453f * __builtin_amdgcn_logf(l2) - float(log(d.z));
+  } else {
+    logl2 = bld<T>(row_rsrc(d.logl2overz, o, ldi), vj);
+    fkovg = bld<T>(row_rsrc(d.fkovg, o, ldi), vj);
+  }
+  const SinFreq<T> sf = sin_freq(s, logl2, cp0, fkovg, sin_diss1, fs2);
   const rsrc_t pe = freq_rsrc(d.e, o, pm, ldi);    // E^n in, E^{n+1} out
   const rsrc_t ps = freq_rsrc(d.es, o, pm, ldi);   // E*
   // the stress partials' cp0(o, i-1) (src/umwm_stress.F90:219,316) and kdk,
