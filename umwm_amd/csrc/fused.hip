@@ -898,8 +898,9 @@ __global__ __launch_bounds__(256) void k_advect(const DomainDev<T>* __restrict__
   const T cp0 = bld<T>(cpr, vj);
   T logl2, fkovg;
   if constexpr (sizeof(T) == 4) {
-    // FP32 forms both from cp0 = 2 pi f / k (src/umwm_init.F90:1512-1521):
-    // f k / g = 2 pi f^2 / (g cp0), log(l2/z) with l2 = cp0 / (2 f) <= 20 m
+    // FP32 forms both from cp0 = 2 pi f / k: f k / g = 2 pi f^2 / (g cp0),
+    // log(l2/z) with l2 = cp0 / (2 f) capped at 20 m
+    // (src/umwm_init.F90:1502,1507,1529-1533)
     const float fo = cld(d.f, o), icp = __builtin_amdgcn_rcpf(cp0);
     fkovg = float(kTwoPiD / d.g) * fo * fo * icp;
     const float l2 = 0.5f * cp0 * __builtin_amdgcn_rcpf(fo);
