@@ -31,7 +31,8 @@ def make(prec):
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", [4, 8])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_equals_members_alone(prec, use_graph):
+@pytest.mark.parametrize("D", [9, 6])  # 6: pm = 24 and 36 take the mask-free, paired sweeps
+def test_batch_equals_members_alone(prec, use_graph, D):
     batch_doms = make(prec)
     alone = make(prec)
     b = hip.Batch(batch_doms)
@@ -39,10 +40,10 @@ def test_batch_equals_members_alone(prec, use_graph):
         # members are grouped by pm; each group runs the batch's D with
         # W = ceil(pm / D) waves, so a member alone gets that same shape
         pmax = max(d.pm for d in batch_doms)
-        b.set_tile(9, -(-pmax // 9), 9, -(-pmax // 9), 2)
+        b.set_tile(D, -(-pmax // D), D, -(-pmax // D), 2)
         for d in alone:
-            w = -(-d.pm // 9)
-            d.set_tile(9, w, 9, w, 2)
+            w = -(-d.pm // D)
+            d.set_tile(D, w, D, w, 2)
         for hour in range(2):
             nsub = b.run_dtg(use_graph=use_graph)
             n_alone = [d.run_dtg() for d in alone]
