@@ -59,3 +59,23 @@ def tiny_case(name, seed=0, **kw):
     args = dict(TINY_CASES[name])
     args.update(kw)
     return configs.tiny(seed=seed, **args)
+
+
+def spectrum_pointwise(a, b, floor=1e-6):
+    """Largest relative error over the bins that hold at least `floor` of
+    their point's largest bin: unlike the normwise relerr it also judges
+    the spectral tail and the swell far below each point's peak."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    peak = np.max(np.abs(b), axis=tuple(range(1, b.ndim)), keepdims=True)
+    m = np.abs(b) >= floor * peak
+    m &= peak > 0
+    if not m.any():
+        return 0.0
+    return float(np.max(np.abs(a[m] - b[m]) / np.abs(b[m])))
+
+
+def angle_err(a, b):
+    """Largest difference of two angle fields, wrapped to [0, pi]."""
+    d = np.asarray(a, dtype=np.float64) - np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(np.angle(np.exp(1j * d))))) if d.size else 0.0

@@ -1,0 +1,310 @@
+"""GPU parity on the benchmark workloads and on the branches the tiny cases
+leave idle, against the FP64 oracle on identical inputs (all calls through
+the C-ABI):
+
+  * C2: the 0.5 deg global lat-lon workload built at 360x180 (GRF land,
+    log-normal depth, GRF winds), FP64 substeps and FP32 over one hour;
+  * C3: batched members drawn by the C3 generator at a tenth of the grid
+    size, covering om 25-49 and pm 24-72, each against its own oracle run;
+  * C5: the hurricane generator on a reduced grid, 100x80 cells of 9 km
+    (om = 49, pm = 72, shelf from 5 m over 200 km, seeded coastline,
+    translating Holland vortex delivered as hourly snapshots, refraction);
+    at 2 km a 150x120 crop puts 23 m/s winds over the 5 m shelf, where the
+    cold start limits dts to about 0.01 s, so the reduced grid keeps the
+    full domain's layout (storm over deep water, coast 500 km away) at a
+    size whose oracle hour takes seconds.  It runs under every tile the
+    autotuner can pick at pm = 72, over an hour in FP64 and FP32, and as a
+    batched ensemble;
+  * sea ice between and above the two thresholds (Kohout attenuation,
+    src/umwm_source_functions.F90:108-153, and the update switched off,
+    src/umwm_advection.F90:93,173) with ice, air- and water-density forcing
+    interpolated in time (src/umwm_forcing.F90:40-99);
+  * the direction and dominant-wave outputs of diag (mwd, dwd, dcp0, dcg0,
+    dcp, dcg; src/umwm_physics.F90:157-203).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests.harness import (angle_err, hs_rmse, make_pair, pointwise_relerr, relerr,
+                           spectrum_pointwise)
+from umwm_amd import configs, hip
+
+pytestmark = pytest.mark.gpu
+
+TOL_RUN = 1e-10
+THREADS = min(os.cpu_count() or 1, 16)
+# bins down to 1e-6 of their point's peak, judged one by one: round-off of
+# the regrouped sums grows where a bin is the small difference of upwind
+# fluxes, so these are held to 1e-8 rather than the normwise 1e-10
+TOL_BIN = 1e-8
+
+DIAG_SCALARS = ["ht", "mwp", "mss", "mwl", "dwl", "dwp", "dcp0", "dcg0", "dcp", "dcg"]
+
+
+def check_state(o, d, tag, tol=TOL_RUN, tol_bin=TOL_BIN):
+    e, e_ref = d.get("e"), o.get("e")
+    err = relerr(e, e_ref)
+    assert err <= tol, f"{tag}: e normwise rel err {err:.3e}"
+    err = spectrum_pointwise(e, e_ref)
+    assert err <= tol_bin, f"{tag}: e bin rel err {err:.3e}"
+    for name in ("ustar", "cd"):
+        err = pointwise_relerr(d.get(name), o.get(name), floor=1e-12)
+        assert err <= tol, f"{tag}: {name} rel err {err:.3e}"
+
+
+def check_diag(o, d, tag, tol=TOL_RUN):
+    for name in DIAG_SCALARS:
+        err = pointwise_relerr(d.get(name), o.get(name), floor=1e-12)
+        assert err <= tol, f"{tag}: {name} rel err {err:.3e}"
+    for name in ("mwd", "dwd"):
+        err = angle_err(d.get(name), o.get(name))
+        assert err <= tol * np.pi, f"{tag}: {name} angle err {err:.3e}"
+    # the dominant bin is found by the same maxloc: dwd is a table value
+    assert np.array_equal(d.get("dwd"), o.get("dwd")), f"{tag}: dwd"
+
+
+# ---------------------------------------------------------------------------
+# C2
+# ---------------------------------------------------------------------------
+def test_c2_fp64_substeps():
+    """The C2 workload at 360x180: cold substep and three more, FP64."""
+    wl = configs.c2(seed=2, mm=360, nm=180, hours=1)
+    o, d = make_pair(wl, threads=THREADS)
+    try:
+        assert d.im > 30000
+        for step in range(4):
+            o.substep()
+            sumt, dts = d.substep()
+            assert abs(dts - o.scalar("dts")) <= TOL_RUN * max(o.scalar("dts"), 1.0), step
+            assert abs(sumt - o.scalar("sumt")) <= TOL_RUN * max(o.scalar("sumt"), 1.0), step
+            check_state(o, d, f"c2 substep {step}")
+        o.diag()
+        d.diag()
+        check_diag(o, d, "c2 diag")
+    finally:
+        d.close()
+        o.close()
+
+
+def test_c2_fp32_one_hour():
+    """The C2 workload at 360x180 over one hour in FP32 against the FP64
+    oracle: Hs within 0.5 %, ustar within 3 %."""
+    wl = configs.c2(seed=2, mm=360, nm=180, hours=1)
+    o, d = make_pair(wl, precision=4, threads=THREADS)
+    try:
+        n_o, n_d = o.run_dtg(), d.run_dtg()
+        assert abs(n_o - n_d) <= 1
+        hs, hs_ref = d.get("ht"), o.get("ht")
+        assert hs_rmse(hs, hs_ref) <= 0.005
+        assert pointwise_relerr(hs, hs_ref, floor=1e-3) <= 0.005
+        assert pointwise_relerr(d.get("ustar"), o.get("ustar")) <= 0.03
+    finally:
+        d.close()
+        o.close()
+
+
+# ---------------------------------------------------------------------------
+# C3
+# ---------------------------------------------------------------------------
+def c3_members(n_pick=7):
+    """Members of the C3 draw at a tenth of the grid size, picked to span
+    its spectral sizes: the smallest and largest pm and om, then distinct
+    pm classes."""
+    wls = configs.c3(seed=3, n=64, hours=2, scale=0.1)
+    pm = np.array([w.params.pm for w in wls])
+    om = np.array([w.params.om for w in wls])
+    pick = [int(np.argmin(pm)), int(np.argmax(pm)), int(np.argmin(om)), int(np.argmax(om))]
+    for k in np.argsort(-pm * 100 - om):
+        if len(set(pick)) >= n_pick:
+            break
+        if pm[k] not in pm[pick]:
+            pick.append(int(k))
+    pick = list(dict.fromkeys(pick))
+    return [wls[k] for k in pick]
+
+
+def test_c3_batched_members_fp64():
+    """C3 members advanced as one batch (graph replay) for two hours, each
+    against its own oracle run."""
+    wls = c3_members()
+    assert len(wls) >= 6
+    assert min(w.params.pm for w in wls) == 24 and max(w.params.pm for w in wls) == 72
+    assert min(w.params.om for w in wls) <= 26 and max(w.params.om for w in wls) >= 48
+    pairs = [make_pair(wl, threads=THREADS) for wl in wls]
+    b = hip.Batch([d for _, d in pairs])
+    try:
+        for hour in range(2):
+            nsub = b.run_dtg(use_graph=True)
+            for k, ((o, d), wl) in enumerate(zip(pairs, wls)):
+                n_o = o.run_dtg()
+                tag = f"c3 member {k} ({d.im} pts, om {d.om}, pm {d.pm}) hour {hour}"
+                assert nsub[k] == n_o, tag
+                check_state(o, d, tag)
+                for name in ("ht", "mwp"):
+                    err = pointwise_relerr(d.get(name), o.get(name), floor=1e-12)
+                    assert err <= TOL_RUN, f"{tag}: {name} {err:.3e}"
+    finally:
+        b.close()
+        for o, d in pairs:
+            d.close()
+            o.close()
+
+
+# ---------------------------------------------------------------------------
+# C5
+# ---------------------------------------------------------------------------
+def c5_crop(member=0):
+    return configs.c5(member=member, mm=100, nm=80, dx=9000.0, hours=1)
+
+
+# every (rates, integrate, advect) choice the autotuner has at pm = 72:
+# rates D in {9, 18, 36} with W = 8, 4, 2; integrate D in {9, 12, 18} with
+# W = 8, 6, 4; NB in {2, 4} (all mask-free; rates pairs directions for
+# even D)
+C5_TILES = [(9, 8, 9, 8, 2), (18, 4, 12, 6, 4), (36, 2, 18, 4, 2), (36, 2, 9, 8, 4),
+            (18, 4, 18, 4, 2), (9, 8, 12, 6, 4)]
+
+
+@pytest.mark.parametrize("tile", C5_TILES)
+def test_c5_crop_tiles_fp64(tile):
+    wl = c5_crop()
+    o, d = make_pair(wl, threads=THREADS)
+    try:
+        assert (d.om, d.pm) == (49, 72)
+        d.set_tile(*tile)
+        for step in range(4):
+            o.substep()
+            d.substep()
+            check_state(o, d, f"c5 crop tile {tile} substep {step}")
+    finally:
+        d.close()
+        o.close()
+
+
+def test_c5_crop_hour_fp64_fp32():
+    """One hour of the crop with the hour-1 snapshot interpolated: FP64 to
+    1e-10 and FP32 to Hs 0.5 % / ustar 3 % against the oracle."""
+    wl = c5_crop()
+    o, d = make_pair(wl, threads=THREADS)
+    _, d32 = make_pair(wl, precision=4, threads=1)
+    try:
+        # the storm and the shelf are both in the crop
+        assert np.max(d.get("wspd")) > 40 and np.min(d.get("d")[: d.im]) < 10
+        n_o = o.run_dtg()
+        assert d.run_dtg() == n_o
+        d32.run_dtg()
+        check_state(o, d, "c5 crop hour")
+        check_diag(o, d, "c5 crop hour")
+        hs, hs_ref = d32.get("ht"), o.get("ht")
+        assert hs_rmse(hs, hs_ref) <= 0.005
+        assert pointwise_relerr(d32.get("ustar"), o.get("ustar")) <= 0.03
+    finally:
+        d.close()
+        d32.close()
+        o.close()
+
+
+def test_c5_crop_ensemble_batched_fp64():
+    """Two hurricane members (heading and Vmax perturbed) advanced as a
+    batch for two hours with the next hourly snapshot loaded in between,
+    each against its own oracle."""
+    wls = [c5_crop(m) for m in (0, 1)]
+    pairs = [make_pair(wl, threads=THREADS) for wl in wls]
+    b = hip.Batch([d for _, d in pairs])
+    try:
+        for hour in range(2):
+            if hour:
+                for (o, d), wl in zip(pairs, wls):
+                    o.forcinginput(wl.forcing_input(hour + 1))
+                    d.forcinginput(wl.forcing_input(hour + 1))
+            nsub = b.run_dtg(use_graph=True)
+            for k, (o, d) in enumerate(pairs):
+                assert nsub[k] == o.run_dtg()
+                check_state(o, d, f"c5 member {k} hour {hour}")
+    finally:
+        b.close()
+        for o, d in pairs:
+            d.close()
+            o.close()
+
+
+# ---------------------------------------------------------------------------
+# sea ice and density forcing
+# ---------------------------------------------------------------------------
+def ice_case(density):
+    return configs.tiny(seed=31, mm=16, nm=12, winds=True, ice=True, density=density)
+
+
+@pytest.mark.parametrize("density", [False, True])
+def test_ice_operators_fp64(density):
+    """s_ice, the ice factor of sin_d12 and the switched-off update of
+    propagation and refraction, operator by operator."""
+    wl = ice_case(density)
+    o, d = make_pair(wl, threads=THREADS)
+    try:
+        fice = o.get("fice")[: o.im]
+        assert (fice > 0.75).any() and ((fice > 0.3) & (fice <= 0.75)).any() and (fice < 0.3).any()
+        for step in range(3):
+            tag = f"ice density={density} substep {step}"
+            for m in (o, d):
+                m.forcinginterpolate()
+            for name in ("fice", "rhoa", "rhow", "rhorat", "wspd"):
+                assert relerr(d.get(name), o.get(name)) <= 1e-15, (tag, name)
+            for m in (o, d):
+                m.sin_d12()
+                m.sds_d12()
+                m.snl_d12()
+                m.s_ice()
+            sice = o.get("sice")
+            assert (sice < 0).any(), tag
+            for name in ("ssin", "sds", "snl", "sice"):
+                assert relerr(d.get(name), o.get(name)) <= 1e-11, (tag, name)
+            for m in (o, d):
+                m.source()
+                m.propagation()
+                m.refraction()
+                m.update()
+            o.stress_atm()
+            d.stress("atm")
+            e = d.get("e")
+            assert relerr(e, o.get("e")) <= 1e-11, tag
+            # above fice_uth the prognostic bins are emptied
+            iced = o.get("fice")[: o.im] > 0.75
+            oc = o.get("oc").astype(int)
+            for i in np.nonzero(iced)[0][:5]:
+                assert np.all(e[i, :, : oc[i]] == 0), tag
+            if step == 0:
+                o.diag()
+                d.diag()
+                check_diag(o, d, tag, tol=1e-11)
+                o.set_scalar("first", 0)
+                d.end_first()
+    finally:
+        d.close()
+        o.close()
+
+
+@pytest.mark.parametrize("precision", [8, 4])
+def test_ice_density_fused_hours(precision):
+    """Two hours of the fused path with hourly ice, wind and density
+    snapshots: FP64 to 1e-10, FP32 to Hs 0.5 %."""
+    wl = ice_case(True)
+    o, d = make_pair(wl, precision=precision, threads=THREADS)
+    try:
+        for hour in range(2):
+            if hour:
+                o.forcinginput(wl.forcing_input(hour + 1))
+                d.forcinginput(wl.forcing_input(hour + 1))
+            n_o = o.run_dtg()
+            n_d = d.run_dtg()
+            if precision == 8:
+                assert n_o == n_d
+                check_state(o, d, f"ice fused hour {hour}")
+                check_diag(o, d, f"ice fused hour {hour}")
+            else:
+                assert hs_rmse(d.get("ht"), o.get("ht")) <= 0.005
+    finally:
+        d.close()
+        o.close()

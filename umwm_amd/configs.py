@@ -221,12 +221,16 @@ def c5(member=0, seed=5, mm=1500, nm=1200, om=49, pm=72, hours=24, dx=2000.0):
 
 
 def tiny(seed=0, mm=14, nm=11, om=25, pm=24, land=True, shelf=True, isglobal=False,
-         winds=False, currents=False, wspd=12.0, wdir=0.3):
+         winds=False, currents=False, wspd=12.0, wdir=0.3, ice=False, density=False,
+         dmin=10.0):
     """Small domains for parity tests: optional land, sloping depth,
-    periodic lat-lon grid, time-varying wind and currents."""
+    periodic lat-lon grid, time-varying wind and currents, sea ice (a field
+    spread over [0, 1], so that points lie below fice_lth, between the two
+    thresholds where Kohout attenuation acts, and above fice_uth where
+    advection is switched off) and air/water density fields."""
     rng = np.random.default_rng(seed)
     kw = dict(mm=mm, nm=nm, om=om, pm=pm, fmin=0.0313, fmax=2.0, fprog=2.0, dtg=3600.0,
-              delx=2e4, dely=1.5e4, wspd0=wspd, wdir0=wdir, stokes=1, dmin=10.0)
+              delx=2e4, dely=1.5e4, wspd0=wspd, wdir0=wdir, stokes=1, dmin=dmin)
     grid = {}
     forcing = None
     if isglobal:
@@ -246,7 +250,7 @@ def tiny(seed=0, mm=14, nm=11, om=25, pm=24, land=True, shelf=True, isglobal=Fal
             z = np.where(ml, 1.0, z)
         grid["z"] = z
         kw.update(topofromfile=1)
-    if winds or currents:
+    if winds or currents or ice or density:
         forcing = {}
     if winds:
         kw.update(winds=1)
@@ -257,6 +261,13 @@ def tiny(seed=0, mm=14, nm=11, om=25, pm=24, land=True, shelf=True, isglobal=Fal
         kw.update(currents=1)
         forcing.update(uc=0.3 * rng.uniform(-1, 1, size=(mm, nm)),
                        vc=0.3 * rng.uniform(-1, 1, size=(mm, nm)))
+    if ice:
+        kw.update(seaice=1)
+        forcing.update(fice=rng.uniform(0.0, 1.0, size=(mm, nm)))
+    if density:
+        kw.update(air_density=1, water_density=1)
+        forcing.update(rhoa=1.2 * (1 + 0.05 * rng.uniform(-1, 1, size=(mm, nm))),
+                       rhow=1025.0 * (1 + 0.01 * rng.uniform(-1, 1, size=(mm, nm))))
     p = abi.make_params(**kw)
     js = abi.make_jonswap(alpha=0.01, gamma=3.3, fp=0.25, fp_from_wind=False,
                           dir_from_wind=True)
@@ -269,6 +280,8 @@ def tiny(seed=0, mm=14, nm=11, om=25, pm=24, land=True, shelf=True, isglobal=Fal
             out = {}
             for key, val in f0.items():
                 out[key] = val * (1 + 0.2 * r2.uniform(-1, 1, size=val.shape))
+            if "fice" in out:
+                out["fice"] = np.clip(out["fice"], 0.0, 1.0)
             return out
     return Workload(f"tiny_s{seed}", p, grid=grid or None, forcing=forcing, jonswap=js,
                     hours=1, snapshots=snaps)

@@ -483,6 +483,14 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
     F.ficeb = D->ficeb; F.ficef = D->ficef;
     F.rhoab = D->rhoab; F.rhoaf = D->rhoaf; F.rhowb = D->rhowb; F.rhowf = D->rhowf;
   }
+  D->fd = A.template alloc<ForcingDev<T>>(1);
+  D->list0 = A.template alloc<int32_t>(1);
+  if (!D->fd || !D->list0) {
+    delete D;
+    return fail(UMWM_ERR_NOMEM, "device allocation failed");
+  }
+  UMWM_HIP_TRY(hipMemcpyAsync(D->fd, &F, sizeof(ForcingDev<T>), hipMemcpyHostToDevice, s));
+  UMWM_HIP_TRY(hipMemsetAsync(D->list0, 0, sizeof(int32_t), s));
   // device copy of the domain struct and the block task list
   D->d = A.template alloc<DomainDev<T>>(1);
   const int nb = h.nblocks;
@@ -869,7 +877,16 @@ struct BatchImpl {
   ScalarsView* h_view = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  bool rot_every = false;
+  // forcing records of the members (refreshed from each member at the
+  // start of every run, so that forcinginput's level swaps reach the
+  // kernels and the captured graph) and the list of forced members
+  ForcingDev<T>* d_fds = nullptr;
+  int32_t* d_flist = nullptr;
+  int nforced = 0, maximm = 0;
+  bool forced_currents = false;
+  // k_keep runs for the batch when any member keeps its sources; the
+  // captured graph is rebuilt when that changes
+  bool keep = false, keep_captured = false;
   int omax = 0;
   ~BatchImpl() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -1041,6 +1058,10 @@ int umwm_hip_forcinginput(umwm_domain* dom, const umwm_forcing_input* nx) {
     F.ucb = D.ucb; F.ucf = D.ucf; F.vcb = D.vcb; F.vcf = D.vcf;
     F.ficeb = D.ficeb; F.ficef = D.ficef;
     F.rhoab = D.rhoab; F.rhoaf = D.rhoaf; F.rhowb = D.rhowb; F.rhowf = D.rhowf;
+    // the kernels (and any graph that captured them) read the levels
+    // through the device record
+    UMWM_HIP_TRY(hipMemcpyAsync(D.fd, &F, sizeof(ForcingDev<T>), hipMemcpyHostToDevice, D.stream));
+    UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
     return 0;
   });
 }
@@ -1529,7 +1550,9 @@ int batch_create_impl(umwm_domain* const* doms, int n, BatchImpl<T>*& out) {
   B->d_doms = B->arena.template alloc<DomainDev<T>>(n);
   B->d_scs = B->arena.template alloc<Scalars*>(n);
   B->d_view = B->arena.template alloc<ScalarsView>(n);
-  if (!B->d_view) {
+  B->d_fds = B->arena.template alloc<ForcingDev<T>>(n);
+  B->d_flist = B->arena.template alloc<int32_t>(n);
+  if (!B->d_view || !B->d_fds || !B->d_flist) {
     delete B;
     return fail(UMWM_ERR_NOMEM, "batch allocation failed");
   }
@@ -1557,6 +1580,16 @@ int batch_create_impl(umwm_domain* const* doms, int n, BatchImpl<T>*& out) {
   }
   std::vector<Scalars*> scs(n);
   for (int k = 0; k < n; ++k) scs[k] = hd[k].sc;
+  std::vector<int32_t> fl;
+  for (int k = 0; k < n; ++k)
+    if (any_forcing(*B->doms[k])) {
+      fl.push_back(k);
+      B->maximm = std::max(B->maximm, B->doms[k]->S.imm);
+      B->forced_currents |= B->doms[k]->S.P.currents != 0;
+    }
+  B->nforced = (int)fl.size();
+  if (!fl.empty())
+    UMWM_HIP_TRY(hipMemcpy(B->d_flist, fl.data(), fl.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   UMWM_HIP_TRY(hipMemcpy(B->d_doms, hd.data(), n * sizeof(DomainDev<T>), hipMemcpyHostToDevice));
   UMWM_HIP_TRY(hipMemcpy(B->d_scs, scs.data(), n * sizeof(Scalars*), hipMemcpyHostToDevice));
   UMWM_HIP_TRY(hipHostMalloc((void**)&B->h_view, n * sizeof(ScalarsView), hipHostMallocDefault));
@@ -1575,19 +1608,45 @@ int on_stream(DomainImpl<T>& D, hipStream_t s, int (*fn)(DomainImpl<T>&)) {
   return rc;
 }
 
+// Uploads every member's current domain and forcing records (a member's
+// record changes with keep_sources, its operator arrays and forcinginput)
+// and drops the captured graph when the batch's keep flag changed.
 template <typename T>
-int launch_forcing_gated(DomainImpl<T>& D) {
-  return launch_forcinginterpolate(D, true);
+int batch_refresh(BatchImpl<T>& B) {
+  const int n = (int)B.doms.size();
+  std::vector<DomainDev<T>> hd(n);
+  std::vector<ForcingDev<T>> hf(n);
+  bool keep = false;
+  for (int k = 0; k < n; ++k) {
+    hd[k] = B.doms[k]->h;
+    hf[k] = B.doms[k]->fh;
+    keep |= B.doms[k]->h.e_keep != nullptr;
+  }
+  UMWM_HIP_TRY(hipMemcpyAsync(B.d_doms, hd.data(), n * sizeof(DomainDev<T>), hipMemcpyHostToDevice, B.stream));
+  UMWM_HIP_TRY(hipMemcpyAsync(B.d_fds, hf.data(), n * sizeof(ForcingDev<T>), hipMemcpyHostToDevice, B.stream));
+  UMWM_HIP_TRY(hipStreamSynchronize(B.stream));
+  B.keep = keep;
+  if (B.exec && B.keep_captured != keep) {
+    (void)hipGraphExecDestroy(B.exec);
+    (void)hipGraphDestroy(B.graph);
+    B.exec = nullptr;
+    B.graph = nullptr;
+  }
+  return 0;
 }
 
+// One substep of every member: the forcing of all forced members in one
+// launch, then the fused kernels once per pm class.  `first` = 1 takes the
+// cold first substep with the rotation tendency; afterwards the kernels
+// refresh the rotation only for members with currents.
 template <typename T>
-int batch_launch_substep(BatchImpl<T>& B, int first, int rotation) {
-  for (auto* D : B.doms)
-    if (any_forcing(*D))
-      if (int rc = on_stream(*D, B.stream, launch_forcing_gated<T>)) return rc;
+int batch_launch_substep(BatchImpl<T>& B, int first) {
+  if (int rc = launch_forcing_list<T>(B.d_doms, B.d_fds, B.d_flist, B.nforced, B.maximm,
+                                      B.forced_currents, true, B.stream))
+    return rc;
   for (const auto& g : B.groups)
-    if (int rc = launch_fused<T>(B.d_doms, g.tasks, g.ntasks, g.omax, first, rotation, g.tile,
-                                 false, B.stream, nullptr))
+    if (int rc = launch_fused<T>(B.d_doms, g.tasks, g.ntasks, g.omax, first, first, g.tile,
+                                 B.keep, B.stream, nullptr))
       return rc;
   for (auto* D : B.doms) D->ef_mirrors_e = true;
   const int n = (int)B.doms.size();
@@ -1600,12 +1659,12 @@ int batch_launch_substep(BatchImpl<T>& B, int first, int rotation) {
 template <typename T>
 int batch_run_dtg_impl(BatchImpl<T>& B, int use_graph, int32_t* nsub) {
   const int n = (int)B.doms.size();
-  for (auto* D : B.doms) {
-    UMWM_HIP_TRY(hipStreamSynchronize(D->stream));
-    hipLaunchKernelGGL(k_begin_dtg, dim3(1), dim3(64), 0, B.stream, D->h.sc);
-  }
-  UMWM_HIP_TRY(hipGetLastError());
+  for (auto* D : B.doms) UMWM_HIP_TRY(hipStreamSynchronize(D->stream));
   int rc;
+  if ((rc = batch_refresh(B))) return rc;
+  for (auto* D : B.doms)
+    hipLaunchKernelGGL(k_begin_dtg, dim3(1), dim3(64), 0, B.stream, D->h.sc);
+  UMWM_HIP_TRY(hipGetLastError());
   // members still in their first substep take it eagerly (cold dts = 0,
   // rotation tendency) before the captured steady-state substep is replayed
   bool any_first = false;
@@ -1613,9 +1672,7 @@ int batch_run_dtg_impl(BatchImpl<T>& B, int use_graph, int32_t* nsub) {
   if (any_first) {
     for (auto* D : B.doms)
       if (!D->first) return fail(UMWM_ERR_STATE, "batch members must start together");
-    if ((rc = batch_launch_substep(B, 1, 1))) return rc;
-    B.rot_every = false;
-    for (auto* D : B.doms) B.rot_every |= D->S.P.currents != 0;
+    if ((rc = batch_launch_substep(B, 1))) return rc;
     for (auto* D : B.doms) {
       D->first = false;
       D->rot_valid = true;
@@ -1624,7 +1681,8 @@ int batch_run_dtg_impl(BatchImpl<T>& B, int use_graph, int32_t* nsub) {
   }
   if (use_graph && !B.exec) {
     UMWM_HIP_TRY(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
-    rc = batch_launch_substep(B, 0, B.rot_every ? 1 : 0);
+    rc = batch_launch_substep(B, 0);
+    B.keep_captured = B.keep;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(B.stream, &g);
     if (rc) return rc;
@@ -1638,7 +1696,7 @@ int batch_run_dtg_impl(BatchImpl<T>& B, int use_graph, int32_t* nsub) {
     for (int c = 0; c < chunk; ++c) {
       if (use_graph) {
         UMWM_HIP_TRY(hipGraphLaunch(B.exec, B.stream));
-      } else if ((rc = batch_launch_substep(B, 0, B.rot_every ? 1 : 0))) {
+      } else if ((rc = batch_launch_substep(B, 0))) {
         return rc;
       }
     }

@@ -71,6 +71,10 @@ struct DomainImpl : DomainBase {
   DomainDev<T> h{};           // host mirror of the device struct
   DomainDev<T>* d = nullptr;  // device copy (array of one)
   ForcingDev<T> fh{};
+  // device copy of fh (array of one), re-uploaded when forcinginput swaps
+  // the levels, and the list {0} that k_forcing's one-domain launch reads
+  ForcingDev<T>* fd = nullptr;
+  int32_t* list0 = nullptr;
   DiagDev diag{};
   OcnDev ocn{};  // allocated with the operator arrays
   StokesDev stokes{};
@@ -107,6 +111,13 @@ template <typename T> int launch_diag(DomainImpl<T>& D);
 template <typename T> int launch_stokes(DomainImpl<T>& D);
 template <typename T> int launch_hs_ice(DomainImpl<T>& D);
 template <typename T> int launch_jonswap(DomainImpl<T>& D, const umwm_jonswap& js);
+
+// Forcing interpolation of the n domains list[0..n) (indices into doms and
+// fds) in one launch, plus the reset of their anycur flags when any has
+// forced currents; maximm is the largest imm among them.
+template <typename T>
+int launch_forcing_list(const DomainDev<T>* doms, const ForcingDev<T>* fds, const int32_t* list,
+                        int n, int maximm, bool currents, bool gate, hipStream_t s);
 
 // Fused substep kernels over a task list that may span many domains.
 template <typename T>

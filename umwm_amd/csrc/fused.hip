@@ -266,7 +266,9 @@ struct TabGen {
     ldi = d.ldi;
     vj = vj_;
     om = d.om;
-    dep = fmaxf(pload(d.depth, ldi, j), 1.f);  // lanes past im read 0
+    // lanes past im read a depth of 0; they take 1 m so that their tables
+    // stay finite (the reference requires only dmin > 0 of sea points)
+    dep = j < d.im ? pload(d.depth, ldi, j) : 1.f;
     sgr = float(d.sfct / (d.rhow0 * d.g));
     sbf = float(d.sbf_fac);
     sbp = float(d.sbp_fac);
@@ -361,10 +363,15 @@ __device__ __forceinline__ T sin_m(const SinFreq<T>& q, T y, T m) {
 template <typename T, int DMAX, bool FULL>
 __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ doms,
                                                const BlockTask* __restrict__ tasks, int first,
-                                               int rotation) {
+                                               int rot_all) {
   extern __shared__ double smem[];
   const BlockTask tk = tasks[blockIdx.x];
   const DomainDev<T>& d = doms[tk.dom];
+  // rotation tendency on the first substep of the launch (rot_all) and on
+  // every substep of a domain with currents (src/umwm_advection.F90:117,
+  // currents .or. first): the choice is per domain, so that batch members
+  // with and without currents each follow the reference
+  const bool rotation = rot_all || d.currents;
   Scalars* sc = d.sc;
   const int W = blockDim.x >> 6, lane = threadIdx.x & 63, w = wave_id();
   const int j = tk.i0 + lane;

@@ -20,45 +20,58 @@ inline dim3 grid1(int n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 // ---------------------------------------------------------------------
 // src/umwm_forcing.F90:40-99 forcinginterpolate (gustiness = 0)
 // ---------------------------------------------------------------------
-// With `gate` (the fused substep) it does nothing once sumt >= dtg: a
-// substep launched past the end of the dtg leaves the forcing as the last
-// active substep interpolated it, as the reference's loop test would.
+// One launch interpolates the forcing of every listed domain (one domain, or
+// all forced members of a batch): block row y serves domain list[y], whose
+// forcing levels are read through its device-resident ForcingDev record, so
+// a captured graph follows forcinginput's level swaps.  With `gate` (the
+// fused substep) the fields are left alone once sumt >= dtg: a substep
+// launched past the end of the dtg leaves the forcing as the last active
+// substep interpolated it, as the reference's loop test would.  With forced
+// currents the kernel also raises the domain's anycur flag where a current
+// is nonzero (zerocurrents, src/umwm_advection.F90:259-265), on the fields
+// as they stand after the launch; k_anycur_reset clears the flags first.
 template <typename T>
-__global__ void k_forcing(DomainDev<T> d, ForcingDev<T> F, int gate) {
+__global__ void k_forcing(const DomainDev<T>* __restrict__ doms,
+                          const ForcingDev<T>* __restrict__ fds,
+                          const int32_t* __restrict__ list, int gate) {
+  const int k = list[blockIdx.y];
+  const DomainDev<T>& d = doms[k];
+  const ForcingDev<T>& F = fds[k];
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= d.imm) return;
   const double s = d.sc->sumt, dtg = d.dtg;
-  if (gate && !(s < dtg)) return;
-  const size_t q2 = (size_t)(F.ni[q] - 1) * d.mm + (F.mi[q] - 1);
-  const T wb = T(1 - s / dtg), wf = T(s), tg = T(dtg);
-  if (F.winds) {
-    T uw = F.uwb[q2] * wb + F.uwf[q2] * wf / tg;
-    T vw = F.vwb[q2] * wb + F.vwf[q2] * wf / tg;
-    d.wspd[q] = sqrt(uw * uw + vw * vw);
-    d.wdir[q] = atan2(vw, uw);
+  if (!gate || s < dtg) {
+    const size_t q2 = (size_t)(F.ni[q] - 1) * d.mm + (F.mi[q] - 1);
+    const T wb = T(1 - s / dtg), wf = T(s), tg = T(dtg);
+    if (F.winds) {
+      T uw = F.uwb[q2] * wb + F.uwf[q2] * wf / tg;
+      T vw = F.vwb[q2] * wb + F.vwf[q2] * wf / tg;
+      d.wspd[q] = sqrt(uw * uw + vw * vw);
+      d.wdir[q] = atan2(vw, uw);
+    }
+    if (F.seaice) d.fice[q] = F.ficeb[q2] * wb + F.ficef[q2] * wf / tg;
+    if (F.currents) {
+      d.uc[q] = F.ucb[q2] * wb + F.ucf[q2] * wf / tg;
+      d.vc[q] = F.vcb[q2] * wb + F.vcf[q2] * wf / tg;
+    }
+    if (F.air_density) d.rhoa[q] = F.rhoab[q] * wb + F.rhoaf[q] * wf / tg;
+    if (F.water_density) d.rhow[q] = F.rhowb[q] * wb + F.rhowf[q] * wf / tg;
+    d.rhorat[q] = d.rhoa[q] / d.rhow[q];
   }
-  if (F.seaice) d.fice[q] = F.ficeb[q2] * wb + F.ficef[q2] * wf / tg;
   if (F.currents) {
-    d.uc[q] = F.ucb[q2] * wb + F.ucf[q2] * wf / tg;
-    d.vc[q] = F.vcb[q2] * wb + F.vcf[q2] * wf / tg;
+    const int n = d.isglobal ? d.imm : d.im;
+    if (q < n && (d.uc[q] != T(0) || d.vc[q] != T(0))) atomicOr(&d.sc->anycur, 1);
   }
-  if (F.air_density) d.rhoa[q] = F.rhoab[q] * wb + F.rhoaf[q] * wf / tg;
-  if (F.water_density) d.rhow[q] = F.rhowb[q] * wb + F.rhowf[q] * wf / tg;
-  d.rhorat[q] = d.rhoa[q] / d.rhow[q];
 }
 
-// any(uc /= 0 .or. vc /= 0) for the zerocurrents test
-// (src/umwm_advection.F90:259-265); the caller zeroes sc->anycur first.
+// Clears anycur of the listed domains whose currents are forced, ahead of
+// k_forcing.
 template <typename T>
-__global__ void k_anycur(DomainDev<T> d) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = d.isglobal ? d.imm : d.im;
-  if (q < n && (d.uc[q] != T(0) || d.vc[q] != T(0))) atomicOr(&d.sc->anycur, 1);
-}
-
-template <typename T>
-__global__ void k_set_anycur(DomainDev<T> d, int v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) d.sc->anycur = v;
+__global__ void k_anycur_reset(const DomainDev<T>* __restrict__ doms,
+                               const ForcingDev<T>* __restrict__ fds,
+                               const int32_t* __restrict__ list, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n && fds[list[t]].currents) doms[list[t]].sc->anycur = 0;
 }
 
 // ---------------------------------------------------------------------
@@ -731,13 +744,22 @@ __global__ void k_jonswap(DomainDev<T> d, umwm_jonswap js, double dlnf) {
   } while (0)
 
 template <typename T>
-int launch_forcinginterpolate(DomainImpl<T>& D, bool gate) {
-  LAUNCH(k_forcing<T>, grid1(D.h.imm), dim3(kBlock), D.h, D.fh, gate ? 1 : 0);
-  if (D.S.P.currents) {
-    LAUNCH(k_set_anycur<T>, dim3(1), dim3(64), D.h, 0);
-    LAUNCH(k_anycur<T>, grid1(D.h.imm), dim3(kBlock), D.h);
-  }
+int launch_forcing_list(const DomainDev<T>* doms, const ForcingDev<T>* fds, const int32_t* list,
+                        int n, int maximm, bool currents, bool gate, hipStream_t s) {
+  if (n <= 0) return 0;
+  if (currents)
+    hipLaunchKernelGGL(k_anycur_reset<T>, dim3((n + 63) / 64), dim3(64), 0, s, doms, fds, list, n);
+  hipLaunchKernelGGL(k_forcing<T>, dim3(grid1(maximm).x, n), dim3(kBlock), 0, s, doms, fds, list,
+                     gate ? 1 : 0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(UMWM_ERR_HIP, std::string("k_forcing: ") + hipGetErrorString(e));
   return 0;
+}
+
+template <typename T>
+int launch_forcinginterpolate(DomainImpl<T>& D, bool gate) {
+  return launch_forcing_list<T>(D.d, D.fd, D.list0, 1, D.h.imm, D.S.P.currents != 0, gate,
+                                D.stream);
 }
 
 template <typename T>
@@ -841,6 +863,8 @@ int launch_jonswap(DomainImpl<T>& D, const umwm_jonswap& js) {
 
 #define INST(T)                                                          \
   template int launch_forcinginterpolate<T>(DomainImpl<T>&, bool);       \
+  template int launch_forcing_list<T>(const DomainDev<T>*, const ForcingDev<T>*, \
+                                      const int32_t*, int, int, bool, bool, hipStream_t); \
   template int launch_op_sin<T>(DomainImpl<T>&);                         \
   template int launch_op_sds<T>(DomainImpl<T>&);                         \
   template int launch_op_snl<T>(DomainImpl<T>&);                         \
