@@ -37,21 +37,22 @@
  *            of the frozen-rate ODE plus an Euler Snl increment) and the
  *            diagnostic tail (breaking/input equilibrium); cold first
  *            substep; propagation (conservation, exact first moment, steady
- *            uniform field); refraction (per-frequency conservation, dtr
+ *            uniform field, single-edge fluxes over a shelf with the group
+ *            speeds of a dispersion root solve); refraction (per-frequency conservation, dtr
  *            bound, the rotation rate as the phase-ray turning rate over a
  *            slope and as the vorticity of a sheared current); stress('atm')
  *            (single-bin wave-supported stress with the cp0(o,i-1) divisor,
  *            the tail integral, skin stress on the relative wind, ustar, cd);
  *            stress('ocn') (E = c M between energy and momentum fluxes, all
  *            dissipation into the top, bottom-friction share, Snl momentum
- *            loss and hand-down of a donor bin, form-drag partitions);
+ *            loss and hand-down of a donor bin, the Snl corrections of the
+ *            receiving bins at the third frequency and above, form-drag
+ *            partitions);
  *            stokes_drift us/vs (textbook Stokes drift of one bin, deep and
  *            finite depth) and the e-folding depth (exponential decay of one
  *            deep-water component).
- *   partial  stress('ocn'): the Snl corrections of bins at the third
- *            frequency and above that receive energy (only donor bins are
- *            pinned); the substep driver order (src/umwm_top.F90:89-141) is
- *            a sequence of calls and has no pin of its own beyond the pieces.
+ *   partial  the substep driver order (src/umwm_top.F90:89-141) is a
+ *            sequence of calls and has no pin of its own beyond the pieces.
  */
 #include "umwm_oracle.h"
 

@@ -342,6 +342,66 @@ def test_propagation_conserves_and_moves_at_cg(O):
     assert np.all(de[:, :, oc:] == 0)
 
 
+def test_propagation_single_edge_fluxes_on_a_shelf(O):
+    """src/umwm_advection.F90:30-47 over a sloping shelf, where the group
+    speed differs between neighbours: one bin of energy E at one interior
+    point moves in one step through its east and north edges (direction
+    between the two) with edge speeds the sum of the two points' group
+    speeds, cos and sin of the direction, times the edge lengths:
+        gain of the east neighbour = dta/4 * 2 (cg(i) + cg(ie)) cos th dy E / ar,
+        gain of the north neighbour = dta/4 * 2 (cg(i) + cg(in)) sin th dx E / ar,
+    and the point loses what they gain (sum(e ar) is conserved).  The group
+    speeds come from a root solve of the dispersion relation at each
+    point's depth, not from the oracle's tables; taking twice the point's
+    own speed at an edge, or the edges' lengths swapped, fails."""
+    mm, nm, dx, dy = 12, 9, 5000.0, 4000.0
+    depth = 8.0 + 6.0 * np.arange(mm)[:, None] * np.ones((1, nm))  # shelf rising west
+    o = regional(O, mm=mm, nm=nm, om=25, pm=24, depth=depth, delx=dx, dely=dy, wspd0=10.0,
+                 dmin=5.0)
+    o.sin_d12()
+    mi, ni = o.get("mi")[: o.im], o.get("ni")[: o.im]
+    i0 = int(np.nonzero((mi == 6) & (ni == 5))[0][0])
+    ie = int(np.nonzero((mi == 7) & (ni == 5))[0][0])
+    inn = int(np.nonzero((mi == 6) & (ni == 6))[0][0])
+    th = o.get("th")
+    p = int(np.argmin(np.abs(th - 0.6)))  # between east and north
+    q = 3
+    assert int(o.get("oc")[i0]) > q
+    e, ef = o.get("e"), o.get("ef")
+    e[:] = 0
+    E = 2e-2
+    e[i0, p, q] = E
+    ef[:] = e
+    dta = 30.0
+    o.set_scalar("dta", dta)
+    o.propagation()
+    out = o.get("ef")
+    f = o.get("f")[q]
+    w = 2 * math.pi * f
+
+    def cg(d):
+        fn = lambda x: G * x * math.tanh(x * d) * (1 + 0.07 * x * x / (1030.0 * G)) - w * w
+        k = optimize.brentq(fn, 1e-8, 1e5, xtol=1e-15, rtol=1e-15)
+        cp = w / k
+        return cp * (0.5 + k * d / math.sinh(2 * k * d) + 0.07 * k * k / (1030.0 * G + 0.07 * k * k))
+
+    d = o.get("d")[: o.im]
+    cgi, cge, cgn = cg(d[i0]), cg(d[ie]), cg(d[inn])
+    assert abs(cge / cgi - 1) > 0.02  # the slope matters at this frequency
+    ar = dx * dy
+    gain_e = dta / 4 * 2 * (cgi + cge) * math.cos(th[p]) * dy * E / ar
+    gain_n = dta / 4 * 2 * (cgi + cgn) * math.sin(th[p]) * dx * E / ar
+    # the oracle's k stops at the reference's Newton tolerance (1e-4)
+    assert out[ie, p, q] == pytest.approx(gain_e, rel=3e-4)
+    assert out[inn, p, q] == pytest.approx(gain_n, rel=3e-4)
+    assert out[i0, p, q] == pytest.approx(E - gain_e - gain_n, rel=1e-4)
+    tot0, tot1 = E * ar, float(np.sum(out[:, p, q])) * ar
+    assert abs(tot1 - tot0) <= 1e-13 * tot0
+    others = np.ones(o.im, bool)
+    others[[i0, ie, inn]] = False
+    assert np.all(out[others, p, q] == 0)
+
+
 def test_propagation_uniform_field_is_steady(O):
     """A uniform spectrum with uniform cg has zero flux divergence away from the
     coast."""

@@ -511,6 +511,61 @@ def test_stress_ocn_snl_momentum_loss_of_a_donor(O):
     assert np.allclose(o.get("epsy_ocn"), eps * math.sin(th), rtol=1e-11)
 
 
+def test_stress_ocn_snl_corrections_of_receiving_bins(O):
+    """src/umwm_stress.F90:51-64,105-133 with one nonzero bin at the sixth
+    frequency (deep water): Snl takes snl_fac Sds e from it and hands the
+    energy to the fifth and fourth frequencies with weights bf1 and bf2
+    (src/umwm_source_functions.F90:156-171), so all three bins lie at the
+    third frequency or above and carry the corrections: the ocean-top
+    stress loses snl(o) (bf1/c(o-1) + bf2/c(o-2)) per unit of each bin's
+    direction, the Snl momentum loss gains snl(o) (bf1 (1 - c(o)/c(o-1)) +
+    bf2 (1 - c(o)/c(o-2))) / c(o), and the energy flux loses c(o) times
+    the first.  The sums are written out per bin from the hand-down of the
+    energy, not from the reference's loops; a swapped o-1/o-2 divisor or a
+    loop starting at the wrong frequency fails."""
+    q, p, fac = 5, 4, 5.0
+    o = _ocn_case(O, 4000.0, q, p, snl_fac=fac)
+    f = o.get("f")
+    dlnf = math.log(f[1] / f[0])
+    b1, b2 = math.exp(-16 * dlnf * dlnf), math.exp(-64 * dlnf * dlnf)
+    bf1, bf2 = b1 / (b1 + b2), b2 / (b1 + b2)
+    cp, kdk = o.get("cp0"), o.get("kdk")
+    c = [cp[:, k] for k in range(q + 1)]
+    sds0 = o.get("sds")[:, p, q] / o.get("cothkd")[:, q]  # Sds before its coth factor
+    e = o.get("e")[:, p, q]
+    X = fac * sds0 * e  # energy Snl moves out of the bin per unit time
+    # snl of the donor and of the two receivers, per unit of their own kdk
+    snl = {q: -X, q - 1: X * bf1 * kdk[:, q] / kdk[:, q - 1],
+           q - 2: X * bf2 * kdk[:, q] / kdk[:, q - 2]}
+    assert np.allclose(o.get("snl")[:, p, q - 1], snl[q - 1], rtol=1e-12)
+    assert np.allclose(o.get("snl")[:, p, q - 2], snl[q - 2], rtol=1e-12)
+    # energy conservation of the hand-down
+    assert np.allclose(sum(snl[k] * kdk[:, k] for k in snl), 0, atol=1e-14 * np.max(X * kdk[:, q]))
+    rhow = o.get("rhow")[: o.im]
+    dth = 2 * math.pi / 24
+    th = o.get("th")[p]
+    oc = o.get("oc").astype(int)
+    assert np.all(oc >= q + 1)  # every bin is prognostic: the Snl loss sees all three
+    corr = sum(kdk[:, k] * snl[k] * (bf1 / c[k - 1] + bf2 / c[k - 2]) for k in snl)
+    loss = sum(kdk[:, k] * snl[k] * (bf1 * (1 - c[k] / c[k - 1]) + bf2 * (1 - c[k] / c[k - 2]))
+               / c[k] for k in snl)
+    ecorr = sum(kdk[:, k] * snl[k] * (bf1 * c[k] / c[k - 1] + bf2 * c[k] / c[k - 2]) for k in snl)
+    sds_c = o.get("sds")[:, p, q]
+    dis = sds_c + o.get("sdt")[:, q] + o.get("sdv")[:, q]
+    scale = rhow * G * dth
+    top = scale * (kdk[:, q] * e * dis / c[q] - corr)
+    assert np.allclose(o.get("taux_ocntop") - o.get("taux_skin"), top * math.cos(th), rtol=1e-11)
+    assert np.allclose(o.get("tauy_ocntop") - o.get("tauy_skin"), top * math.sin(th), rtol=1e-11)
+    assert np.allclose(o.get("taux_snl"), scale * loss * math.cos(th), rtol=1e-11)
+    assert np.allclose(o.get("tauy_snl"), scale * loss * math.sin(th), rtol=1e-11)
+    eps = scale * (kdk[:, q] * e * sds_c - ecorr)
+    assert np.allclose(o.get("epsx_ocn"), eps * math.cos(th), rtol=1e-11)
+    assert np.allclose(o.get("epsy_ocn"), eps * math.sin(th), rtol=1e-11)
+    # the receivers' corrections are not negligible next to the donor's
+    assert np.all(np.abs(corr - kdk[:, q] * snl[q] * (bf1 / c[q - 1] + bf2 / c[q - 2]))
+                  > 0.1 * np.abs(corr))
+
+
 # ------------------------------------------------------------------ Stokes
 def test_stokes_efolding_depth_deep_water(O):
     """src/umwm_stokes.F90:102-116: for one deep-water component the drift
