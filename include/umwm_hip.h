@@ -210,6 +210,23 @@ int umwm_hip_substep(umwm_domain* dom, double* sumt, double* dts);
  */
 int umwm_hip_substeps(umwm_domain* dom, int32_t n, double* sumt, double* dts, int32_t* nsub);
 /*
+ * umwm_hip_substeps without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same results as umwm_hip_substeps, bit for bit.  The fused
+ * kernels' scratch (the frequency prefix of the cos^2 moments and the
+ * stress partial sums, 2 x 4 x om x ldi values of the working precision,
+ * ldi = im + 1 rounded up to 64) lives in one file-scope device array of
+ * 32 MiB per GPU, shared by every call on that GPU and kept until the
+ * process ends; a domain needing more returns UMWM_ERR_SIZE (FP64 with
+ * om = 37 fits up to about 14,000 sea points, FP32 twice that).  The array
+ * keeps the last call's contents and is cleared at the start of each call:
+ * calls that overlap on one GPU (two streams, two threads) race on it.
+ * The domain itself must already exist (umwm_hip_create allocates it).
+ */
+int umwm_hip_substeps_noalloc(umwm_domain* dom, int32_t n, double* sumt, double* dts,
+                              int32_t* nsub);
+/*
  * The inner loop to sumt >= dtg (src/umwm_top.F90:88-150) followed by the
  * per-dtg stokes_drift (when stokes) and diag (:152-155).  `nsub` receives
  * the number of substeps taken.  The first call of a run includes the

@@ -7,6 +7,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,16 +15,47 @@
 #include "physics.hpp"
 #include "rawio.hpp"
 
+// Scratch of umwm_hip_substeps_noalloc: the fused kernels' prefix moments
+// and stress partials ([om][ldi][4] each in the working precision), which
+// the other entry points keep in buffers allocated with the domain.  Every
+// GPU the library runs on holds one copy until the process ends; in host
+// code the name is only a stand-in, and the device address comes from
+// hipGetSymbolAddress (umwm::noalloc_scratch).
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace umwm {
 
 namespace {
 thread_local std::string g_last_error;
+
+// device address of `scratch` per device, looked up once per device with
+// that device current
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
 }
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
+}
+
+// The calling thread's current device's copy of `scratch`.
+int noalloc_scratch(unsigned char** out) {
+  int dev = 0;
+  UMWM_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(UMWM_ERR_HIP, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    UMWM_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(UMWM_ERR_HIP, "hipGetSymbolAddress returned null for the scratch array");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
 }
 
 namespace {
@@ -485,7 +517,12 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   }
   D->fd = A.template alloc<ForcingDev<T>>(1);
   D->list0 = A.template alloc<int32_t>(1);
-  if (!D->fd || !D->list0) {
+  D->d_scr = A.template alloc<DomainDev<T>>(1);
+  if ((rc = noalloc_scratch(&D->scr))) {
+    delete D;
+    return rc;
+  }
+  if (!D->fd || !D->list0 || !D->d_scr) {
     delete D;
     return fail(UMWM_ERR_NOMEM, "device allocation failed");
   }
@@ -570,13 +607,16 @@ int enable_keep(DomainImpl<T>& D) {
   return 0;
 }
 
+// One fused substep; `dd` is the device domain record the fused kernels
+// read (the domain's own by default, or the one whose scratch lives in the
+// shared array).
 template <typename T>
-int fused_substep(DomainImpl<T>& D) {
+int fused_substep(DomainImpl<T>& D, const DomainDev<T>* dd = nullptr) {
   int rc;
   if (any_forcing(D) && (rc = launch_forcinginterpolate(D, true))) return rc;
   const int rotation = (D.S.P.currents || D.first || !D.rot_valid) ? 1 : 0;
-  rc = launch_fused<T>(D.d, D.tasks, D.ntasks, D.h.om, D.first ? 1 : 0, rotation, D.tile,
-                       D.h.e_keep != nullptr, D.stream, D.timing ? D.ev : nullptr);
+  rc = launch_fused<T>(dd ? dd : D.d, D.tasks, D.ntasks, D.h.om, D.first ? 1 : 0, rotation,
+                       D.tile, D.h.e_keep != nullptr, D.stream, D.timing ? D.ev : nullptr);
   if (rc) return rc;
   D.ef_mirrors_e = true;
   D.rot_valid = true;
@@ -601,6 +641,35 @@ template <typename T>
 int substeps_impl(DomainImpl<T>& D, int n) {
   for (int k = 0; k < n; ++k)
     if (int rc = fused_substep(D)) return rc;
+  return read_scalars(D);
+}
+
+// umwm_hip_substeps_noalloc: the same substeps with the prefix moments and
+// stress partials in the device's shared scratch array.  Calls no
+// allocating routine: the record copy and the scratch address were set up
+// when the domain was created.
+template <typename T>
+int substeps_noalloc_impl(DomainImpl<T>& D, int n) {
+  const size_t part = (size_t)4 * D.h.om * D.h.ldi * sizeof(T);
+  const size_t off = (part + 255) / 256 * 256;
+  if (off + part > kNoAllocScratchBytes)
+    return fail(UMWM_ERR_SIZE, "substeps_noalloc: the domain needs " +
+                                   std::to_string(off + part) + " bytes of scratch (4 om ldi "
+                                   "values twice), more than the " +
+                                   std::to_string(kNoAllocScratchBytes) +
+                                   " bytes of the shared array; use umwm_hip_substeps");
+  if (!D.scr || !D.d_scr) return fail(UMWM_ERR_STATE, "substeps_noalloc: no scratch set up");
+  D.h_scr = D.h;
+  D.h_scr.pfx = reinterpret_cast<T*>(D.scr);
+  D.h_scr.spart = reinterpret_cast<T*>(D.scr + off);
+  UMWM_HIP_TRY(hipMemcpyAsync(D.d_scr, &D.h_scr, sizeof(DomainDev<T>), hipMemcpyHostToDevice,
+                              D.stream));
+  // the array keeps the previous call's contents: start from zeros, as the
+  // domain's own buffers did (lanes past im are read, never used)
+  if (int rc = dfill<T>(reinterpret_cast<T*>(D.scr), (off + part) / sizeof(T), T(0), D.stream))
+    return rc;
+  for (int k = 0; k < n; ++k)
+    if (int rc = fused_substep(D, (const DomainDev<T>*)D.d_scr)) return rc;
   return read_scalars(D);
 }
 
@@ -1161,6 +1230,18 @@ int umwm_hip_substeps(umwm_domain* dom, int32_t n, double* sumt, double* dts, in
   if (n < 1) return fail(UMWM_ERR_ARGUMENT, "substeps: n must be >= 1");
   return dispatch(dom, [&](auto& D) {
     if (int rc = substeps_impl(D, n)) return rc;
+    if (sumt) *sumt = D.sc_host->sumt;
+    if (dts) *dts = D.sc_host->dts;
+    if (nsub) *nsub = D.sc_host->nsub;
+    return 0;
+  });
+}
+
+int umwm_hip_substeps_noalloc(umwm_domain* dom, int32_t n, double* sumt, double* dts,
+                              int32_t* nsub) {
+  if (n < 1) return fail(UMWM_ERR_ARGUMENT, "substeps_noalloc: n must be >= 1");
+  return dispatch(dom, [&](auto& D) {
+    if (int rc = substeps_noalloc_impl(D, n)) return rc;
     if (sumt) *sumt = D.sc_host->sumt;
     if (dts) *dts = D.sc_host->dts;
     if (nsub) *nsub = D.sc_host->nsub;

@@ -75,6 +75,12 @@ struct DomainImpl : DomainBase {
   // the levels, and the list {0} that k_forcing's one-domain launch reads
   ForcingDev<T>* fd = nullptr;
   int32_t* list0 = nullptr;
+  // umwm_hip_substeps_noalloc: the device's shared scratch array (from
+  // hipGetSymbolAddress at creation) and a second device copy of the domain
+  // record whose pfx and spart point into it, with its host image
+  unsigned char* scr = nullptr;
+  DomainDev<T>* d_scr = nullptr;
+  DomainDev<T> h_scr{};
   DiagDev diag{};
   OcnDev ocn{};  // allocated with the operator arrays
   StokesDev stokes{};

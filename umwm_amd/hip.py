@@ -22,6 +22,7 @@ EXPORTS = [
     "umwm_hip_sin_d12", "umwm_hip_sds_d12", "umwm_hip_snl_d12", "umwm_hip_s_ice",
     "umwm_hip_source", "umwm_hip_propagation", "umwm_hip_refraction", "umwm_hip_update",
     "umwm_hip_stress", "umwm_hip_diag", "umwm_hip_end_first", "umwm_hip_substep", "umwm_hip_substeps",
+    "umwm_hip_substeps_noalloc",
     "umwm_hip_run_dtg", "umwm_hip_get_field", "umwm_hip_set_field", "umwm_hip_get_scalar",
     "umwm_hip_set_scalar", "umwm_hip_field_size", "umwm_hip_set_timing",
     "umwm_hip_get_timing", "umwm_hip_synchronize", "umwm_hip_batch_create",
@@ -68,6 +69,8 @@ def lib():
         getattr(L, "umwm_hip_" + fn).argtypes = [vp, C.c_char_p]
     L.umwm_hip_substep.argtypes = [vp, P(C.c_double), P(C.c_double)]
     L.umwm_hip_substeps.argtypes = [vp, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_int32)]
+    L.umwm_hip_substeps_noalloc.argtypes = [vp, C.c_int32, P(C.c_double), P(C.c_double),
+                                            P(C.c_int32)]
     L.umwm_hip_run_dtg.argtypes = [vp, P(C.c_int32)]
     L.umwm_hip_get_field.argtypes = [vp, C.c_char_p, P(C.c_double), C.c_int64]
     L.umwm_hip_set_field.argtypes = [vp, C.c_char_p, P(C.c_double), C.c_int64]
@@ -255,6 +258,14 @@ class Domain:
         substeps taken this dtg)."""
         s, d, k = C.c_double(), C.c_double(), C.c_int32()
         check(lib().umwm_hip_substeps(self.h, int(n), C.byref(s), C.byref(d), C.byref(k)))
+        return s.value, d.value, k.value
+
+    def substeps_noalloc(self, n):
+        """substeps(n) with the fused kernels' scratch in the library's
+        shared device array: no device allocation (see umwm_hip.h)."""
+        s, d, k = C.c_double(), C.c_double(), C.c_int32()
+        check(lib().umwm_hip_substeps_noalloc(self.h, int(n), C.byref(s), C.byref(d),
+                                              C.byref(k)))
         return s.value, d.value, k.value
 
     def run_dtg(self):
