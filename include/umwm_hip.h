@@ -295,9 +295,10 @@ int64_t umwm_hip_field_size(const umwm_domain* dom, const char* name);
 /* Enables per-kernel event timing of the fused path (off by default). */
 int umwm_hip_set_timing(umwm_domain* dom, int enable);
 /* Accumulated milliseconds and launch counts of the four fused kernels
- * since the last reset: ms[0..3] = rates, integrate, advect, stress;
- * counts[0..3] likewise.  Resets the accumulators when reset != 0. */
-int umwm_hip_get_timing(umwm_domain* dom, double ms[4], int64_t counts[4],
+ * and of the surrogate's inference since the last reset: ms[0..4] =
+ * rates, integrate, advect, stress, surrogate (network and its inputs);
+ * counts[0..4] likewise.  Resets the accumulators when reset != 0. */
+int umwm_hip_get_timing(umwm_domain* dom, double ms[5], int64_t counts[5],
                         int reset);
 /* Waits for all work queued on the domain's stream. */
 int umwm_hip_synchronize(umwm_domain* dom);
@@ -323,12 +324,97 @@ int umwm_hip_batch_destroy(umwm_batch* b);
  * class of members).  nsub[k] receives member k's substep count (array of
  * n, may be NULL). */
 int umwm_hip_batch_run_dtg(umwm_batch* b, int use_graph, int32_t* nsub);
+/* n substeps of every member, launched back to back (hipGraph replays
+ * when use_graph != 0) with one host wait; a member whose sumt has reached
+ * dtg does nothing.  The caller starts each dtg (as umwm_hip_batch_run_dtg
+ * does) by setting sumt = 0 after loading the next forcing.  nsub[k]
+ * receives member k's substeps taken this dtg (array of n members, may be
+ * NULL). */
+int umwm_hip_batch_substeps(umwm_batch* b, int32_t n, int use_graph, int32_t* nsub);
 /* Tile shape of the batch's fused launches (layout of umwm_hip_set_tile). */
 int umwm_hip_batch_tile(const umwm_batch* b, int32_t out[5]);
 /* Sets the batch's tile shape (same rules as umwm_hip_set_tile over the
  * largest pm of the batch).  Members are launched in groups of equal pm;
  * each group uses the tile's D1, D2 and NB with W = ceil(pm / D). */
 int umwm_hip_batch_set_tile(umwm_batch* b, const int32_t tile[5]);
+
+/* ---------------------------------------------------------------------- */
+/* Learned source-term surrogate.  No counterpart in the reference, whose */
+/* source terms are closed forms (src/umwm_source_functions.F90:16-105,   */
+/* combined in src/umwm_physics.F90:59-68).  A multilayer perceptron maps */
+/* one sea point's spectrum (ln e of the om x pm bins), wind (u10, v10)   */
+/* and ln depth to the combined rate R = ssin - sds - sbf of every bin    */
+/* (sds after its cothkd factor).  Weights and activations are bfloat16,  */
+/* products accumulate in FP32 on the MFMA units; the network is trained  */
+/* on the device from samples the plain source kernels produce.           */
+/* ---------------------------------------------------------------------- */
+typedef struct umwm_surrogate umwm_surrogate;   /* opaque */
+
+typedef struct umwm_surrogate_config {
+  int32_t om, pm;       /* spectral shape served: om*pm + 3 inputs, om*pm outputs */
+  int32_t hidden;       /* width of each hidden layer (512) */
+  int32_t layers;       /* hidden layers, GELU (3) */
+  int32_t batch;        /* training rows per Adam step, a multiple of 8 */
+  int32_t chunk;        /* sea points per inference launch (>= 64) */
+  int64_t capacity;     /* training samples held on the device */
+  uint64_t seed;        /* weight initialisation and minibatch draws */
+  double fmin, fmax;    /* frequency grid of the served domains */
+} umwm_surrogate_config;
+
+/* How an attached domain uses the network.  Every audit_stride-th group of
+ * 64 sea points (rotating with the substep; 0: none) is also evaluated by
+ * the plain kernel.  A point falls back to the plain source terms where its
+ * wind speed or depth lies outside the training range widened by
+ * range_margin (relative), where a normalised spectral input exceeds zmax
+ * standard deviations, where the dissipation the network implies
+ * (ssin - sbf - R) is negative in more than resid_tol of the point's
+ * energy-weighted rates, or where an audit finds the energy-weighted rate
+ * error above audit_tol. */
+typedef struct umwm_surrogate_policy {
+  int32_t audit_stride;
+  int32_t reserved0;
+  double resid_tol, audit_tol, range_margin, zmax;
+} umwm_surrogate_policy;
+
+int umwm_hip_surrogate_create(const umwm_surrogate_config* cfg, umwm_surrogate** out);
+int umwm_hip_surrogate_destroy(umwm_surrogate* s);
+/* out[0] = inputs, out[1] = outputs, out[2] = hidden width, out[3] =
+ * hidden layers, out[4] = samples collected, out[5] = 1 once trained. */
+int umwm_hip_surrogate_dims(const umwm_surrogate* s, int32_t out[6]);
+/* Layer `layer` (0 .. layers): weights [N][K] row-major (K inputs, N
+ * outputs of the layer) and bias [N], FP32 master copies. */
+int umwm_hip_surrogate_get_layer(umwm_surrogate* s, int32_t layer, float* w, float* b);
+int umwm_hip_surrogate_set_layer(umwm_surrogate* s, int32_t layer, const float* w,
+                                 const float* b);
+/* Network outputs y (n x outputs) of n raw input rows x (n x inputs),
+ * without normalisation. */
+int umwm_hip_surrogate_forward(umwm_surrogate* s, const float* x, int32_t n, float* y);
+/* One forward and backward pass over n (a multiple of 8) raw rows: the
+ * mean squared error against t (n x outputs) and the gradients of every
+ * layer, gw[l] [N][K] and gb[l] [N] (either may be NULL). */
+int umwm_hip_surrogate_gradients(umwm_surrogate* s, const float* x, const float* t, int32_t n,
+                                 double* loss, float* const* gw, float* const* gb);
+/* Appends one training sample per sea point of the domain at its current
+ * state: the plain source kernels (sin_d12, sds_d12, snl_d12) evaluate
+ * ssin, sds and sbf on its spectrum.  `total` receives the samples held. */
+int umwm_hip_surrogate_collect(umwm_surrogate* s, umwm_domain* dom, int64_t* total);
+/* Normalisation over the first ntrain samples (all when <= 0), then
+ * `steps` Adam steps on minibatches drawn from them; the mean loss of the
+ * first and last few steps (normalised units). */
+int umwm_hip_surrogate_train(umwm_surrogate* s, int32_t steps, double lr, int64_t ntrain,
+                             double* loss_first, double* loss_last);
+/* Mean loss over collected samples [first, first + n) (held-out rows). */
+int umwm_hip_surrogate_eval(umwm_surrogate* s, int64_t first, int64_t n, double* loss);
+/* Makes an FP32 domain's fused substeps take the network's rates where the
+ * policy allows (NULL surrogate detaches).  The surrogate's buffers serve
+ * one domain at a time: domains sharing it run one after another. */
+int umwm_hip_surrogate_attach(umwm_domain* dom, umwm_surrogate* s,
+                              const umwm_surrogate_policy* policy);
+/* Statistics since attach or the last reset: out[0] point-substeps,
+ * out[1] of which took the network's rates, out[2..4] fell back on range,
+ * residual and audit, out[5] audited, out[6] energy-weighted audit error
+ * (sum |R - R_plain| e dk / sum (|ssin| + sds + sbf) e dk). */
+int umwm_hip_surrogate_stats(umwm_domain* dom, double out[7], int reset);
 
 #ifdef __cplusplus
 }

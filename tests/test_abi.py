@@ -43,7 +43,9 @@ def _c_layout():
     """Offsets and sizes of the header's structs as the C compiler sees them."""
     src = ["#include <stddef.h>", "#include <stdio.h>", '#include "umwm_hip.h"', "int main(void){"]
     for cname, py in (("umwm_params", abi.Params), ("umwm_grid_input", abi.GridInput),
-                      ("umwm_forcing_input", abi.ForcingInput), ("umwm_jonswap", abi.Jonswap)):
+                      ("umwm_forcing_input", abi.ForcingInput), ("umwm_jonswap", abi.Jonswap),
+                      ("umwm_surrogate_config", abi.SurrogateConfig),
+                      ("umwm_surrogate_policy", abi.SurrogatePolicy)):
         src.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
         for f, _ in py._fields_:
             src.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
@@ -64,7 +66,9 @@ def test_struct_layout_matches_ctypes(tmp_path):
             s, f, v = ln.split()
             table[(s, f)] = int(v)
     for cname, py in (("umwm_params", abi.Params), ("umwm_grid_input", abi.GridInput),
-                      ("umwm_forcing_input", abi.ForcingInput), ("umwm_jonswap", abi.Jonswap)):
+                      ("umwm_forcing_input", abi.ForcingInput), ("umwm_jonswap", abi.Jonswap),
+                      ("umwm_surrogate_config", abi.SurrogateConfig),
+                      ("umwm_surrogate_policy", abi.SurrogatePolicy)):
         assert table[(cname, "size")] == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert table[(cname, f)] == getattr(py, f).offset, (cname, f)

@@ -1,0 +1,228 @@
+"""GPU: the learned source-term surrogate.
+
+  * the BF16 MFMA kernels (forward, loss, backward with split-K weight
+    gradients) against the FP64 numpy restatement oracle/mlp_ref.py, which
+    finite differences pin (tests/test_surrogate_ref.py), on a small network
+    and on the full 1335 -> 512 -> 512 -> 512 -> 1332 one;
+  * training on the device from samples of the plain FP32 source kernels on
+    seeded config-1 grids, judged on samples of a held-out seed;
+  * the surrogate path of the fused substep on a fresh config-1 seed against
+    the FP64 oracle (Hs within 2 %, ustar within 3 %), its fallback rate, and
+    what it costs against the plain FP32 path;
+  * the fallbacks: an untrained network under a full audit falls back
+    everywhere (and the run equals the plain FP32 one), and inputs outside
+    the training range fall back on the range check.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from oracle import mlp_ref
+from tests.harness import hs_rmse, make_pair, pointwise_relerr
+from umwm_amd import abi, configs, hip
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(os.cpu_count() or 1, 16)
+
+
+def make_net(om, pm, hidden, layers, batch, chunk=None, capacity=0, seed=1):
+    cfg = abi.make_surrogate_config(om=om, pm=pm, hidden=hidden, layers=layers, batch=batch,
+                                    chunk=chunk or max(batch, 64), capacity=capacity, seed=seed)
+    return hip.Surrogate(cfg)
+
+
+def ref_arrays(net):
+    """The network's weights as the device multiplies them (bf16)."""
+    ws, bs = [], []
+    for l in range(net.layers + 1):
+        w, b = net.get_layer(l)
+        ws.append(mlp_ref.bf16_round(w))
+        bs.append(b.astype(np.float64))
+    return ws, bs
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def cosine(a, b):
+    a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+    return float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+
+
+@pytest.mark.parametrize("shape", ["small", "full"])
+def test_kernels_match_fp64_restatement(shape):
+    """Forward outputs and every layer's gradients.  The device rounds
+    inputs, weights and activations to bfloat16 (8 significant bits) and
+    accumulates in FP32; the reference gets the same bf16 inputs and
+    weights and computes in FP64, so the bound is a few bf16 roundings
+    carried through the layers: 2 % normwise on outputs and gradients,
+    cosine above 0.999."""
+    if shape == "small":
+        om, pm, hidden, layers, n = 4, 4, 64, 2, 64
+    else:
+        om, pm, hidden, layers, n = 37, 36, 512, 3, 256
+    net = make_net(om, pm, hidden, layers, batch=n)
+    try:
+        rng = np.random.default_rng(7)
+        if shape == "small":  # weights of our own, set through the ABI
+            for l in range(layers + 1):
+                K, N = net.widths[l], net.widths[l + 1]
+                net.set_layer(l, rng.normal(0, 1 / np.sqrt(K), (N, K)), rng.normal(0, 0.1, N))
+        x = rng.normal(size=(n, net.nin)).astype(np.float32)
+        t = rng.normal(size=(n, net.nout)).astype(np.float32)
+        ws, bs = ref_arrays(net)
+        xb = mlp_ref.bf16_round(x)
+        _, _, y_ref = mlp_ref.forward(ws, bs, xb)
+        y = net.forward(x)
+        assert rel(y, y_ref) <= 0.02, rel(y, y_ref)
+        L, gws, gbs = net.gradients(x, t)
+        L_ref, gw_ref, gb_ref = mlp_ref.gradients(ws, bs, xb, t)
+        assert abs(L - L_ref) <= 0.02 * L_ref, (L, L_ref)
+        for l in range(layers + 1):
+            assert rel(gws[l], gw_ref[l]) <= 0.02, (l, rel(gws[l], gw_ref[l]))
+            assert cosine(gws[l], gw_ref[l]) >= 0.999, l
+            assert rel(gbs[l], gb_ref[l]) <= 0.02, (l, rel(gbs[l], gb_ref[l]))
+    finally:
+        net.close()
+
+
+# ---------------------------------------------------------------------------
+# training on the plain kernel's samples and the surrogate path
+# ---------------------------------------------------------------------------
+TRAIN_SEEDS = list(range(1, 31))
+HELDOUT_SEED = 31
+EVAL_SEED = 1000
+
+
+def fp32_domain(wl):
+    d = hip.Domain(wl.params, wl.grid_input(), wl.forcing_input(0), precision=4)
+    d.init_jonswap(wl.jonswap)
+    d.stokes_drift(True)
+    if wl.snapshots is not None:
+        d.forcinginput(wl.forcing_input(1))
+    return d
+
+
+def harvest(net, seeds, per_seed=6):
+    """`per_seed` snapshots of each seed's first hour, at the cold start and
+    then as the model time passes each further 1/per_seed of the hour."""
+    for s in seeds:
+        d = fp32_domain(configs.c1_seeded(s))
+        try:
+            net.collect(d)
+            taken = 1
+            while d.scalar("sumt") < 3600.0 or taken == 1:
+                d.substep()
+                if taken < per_seed and d.scalar("sumt") >= taken * 3600.0 / per_seed:
+                    net.collect(d)
+                    taken += 1
+        finally:
+            d.close()
+
+
+@pytest.fixture(scope="module")
+def trained():
+    net = make_net(37, 36, 512, 3, batch=1024, chunk=4096, capacity=600_000, seed=3)
+    harvest(net, TRAIN_SEEDS)
+    ntrain = net.info()["samples"]
+    harvest(net, [HELDOUT_SEED])
+    total = net.info()["samples"]
+    assert total - ntrain >= 2852  # the held-out seed found room
+    t0 = time.perf_counter()
+    first, last = net.train(3000, lr=1e-3, ntrain=ntrain)
+    last2 = net.train(1000, lr=2e-4, ntrain=ntrain)[1]
+    t_train = time.perf_counter() - t0
+    held = net.eval(ntrain, (total - ntrain) // 8 * 8)
+    print(f"surrogate: {ntrain} training samples, {total - ntrain} held out; loss "
+          f"{first:.4f} -> {last:.4f} -> {last2:.4f}, held-out {held:.4f}, {t_train:.1f} s")
+    yield net, dict(ntrain=ntrain, first=first, last=last2, held=held)
+    net.close()
+
+
+def test_training_reduces_the_loss(trained):
+    net, info = trained
+    assert info["ntrain"] == 30 * 6 * 2852
+    assert info["last"] <= 0.1 * info["first"]
+    # the held-out seed is fitted about as well as the training seeds
+    assert info["held"] <= 3.0 * info["last"] + 0.01
+
+
+def test_surrogate_path_on_a_fresh_seed(trained):
+    """One hour on a config-1 seed the network never saw: the FP32 path with
+    the surrogate against the FP64 oracle (Hs 2 %, ustar 3 %), beside the
+    plain FP32 path; the fallback rate and the time of both paths."""
+    net, _ = trained
+    wl = configs.c1_seeded(EVAL_SEED)
+    o, plain = make_pair(wl, precision=4, threads=THREADS)
+    _, sur = make_pair(wl, precision=4, threads=1)
+    try:
+        net.attach(sur, abi.make_surrogate_policy())
+        o.run_dtg()
+        t0 = time.perf_counter()
+        plain.run_dtg()
+        t_plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sur.run_dtg()
+        t_sur = time.perf_counter() - t0
+        st = hip.Surrogate.stats(sur)
+        hs_ref = o.get("ht")
+        e_sur = hs_rmse(sur.get("ht"), hs_ref)
+        e_plain = hs_rmse(plain.get("ht"), hs_ref)
+        u_sur = pointwise_relerr(sur.get("ustar"), o.get("ustar"))
+        used = st["used"] / max(st["points"], 1)
+        print(f"surrogate path: Hs RMSE {e_sur:.4f} (plain FP32 {e_plain:.2e}), ustar "
+              f"{u_sur:.4f}, surrogate used on {used:.3f} of point-substeps, audit error "
+              f"{st['audit_error']:.4f}, time {t_sur:.3f} s vs plain {t_plain:.3f} s")
+        assert st["points"] > 0 and used > 0.5
+        assert e_sur <= 0.02
+        assert u_sur <= 0.03
+    finally:
+        for m in (o, plain, sur):
+            m.close()
+
+
+def test_untrained_network_falls_back_under_full_audit():
+    """Random weights (statistics from a few samples): with every workgroup
+    audited each substep, the audit rejects the network at every point and
+    the run is the plain FP32 run."""
+    net = make_net(37, 36, 64, 1, batch=64, chunk=4096, capacity=20_000, seed=5)
+    wl = configs.c1_seeded(2)
+    a, b = fp32_domain(wl), fp32_domain(wl)
+    try:
+        net.collect(a)
+        net.train(1, lr=1e-6)
+        net.attach(b, abi.make_surrogate_policy(audit_stride=1, audit_tol=0.02, resid_tol=1e30))
+        for _ in range(4):
+            a.substep()
+            b.substep()
+        st = hip.Surrogate.stats(b)
+        assert st["audited"] == st["points"] > 0
+        assert st["used"] == 0
+        assert st["fallback_audit"] + st["fallback_range"] == st["points"]
+        assert pointwise_relerr(b.get("ht"), a.get("ht")) <= 1e-5
+        assert np.max(np.abs(b.get("e") - a.get("e"))) <= 1e-5 * np.max(np.abs(a.get("e")))
+    finally:
+        for x in (a, b):
+            x.close()
+        net.close()
+
+
+def test_out_of_range_inputs_fall_back(trained):
+    """Winds of 40 m/s lie outside the training range (10-20 m/s): every
+    point falls back on the range check."""
+    net, _ = trained
+    wl = configs.c1_seeded(EVAL_SEED, vary=False)
+    wl.params.wspd0 = 40.0
+    d = fp32_domain(wl)
+    try:
+        net.attach(d, abi.make_surrogate_policy())
+        for _ in range(3):
+            d.substep()
+        st = hip.Surrogate.stats(d)
+        assert st["points"] > 0 and st["fallback_range"] == st["points"]
+    finally:
+        d.close()

@@ -62,6 +62,33 @@ class Jonswap(C.Structure):
     ]
 
 
+class SurrogateConfig(C.Structure):
+    _fields_ = [
+        ("om", I32), ("pm", I32), ("hidden", I32), ("layers", I32), ("batch", I32),
+        ("chunk", I32), ("capacity", C.c_int64), ("seed", C.c_uint64),
+        ("fmin", F64), ("fmax", F64),
+    ]
+
+
+class SurrogatePolicy(C.Structure):
+    _fields_ = [
+        ("audit_stride", I32), ("reserved0", I32), ("resid_tol", F64), ("audit_tol", F64),
+        ("range_margin", F64), ("zmax", F64),
+    ]
+
+
+def make_surrogate_config(om=37, pm=36, hidden=512, layers=3, batch=4096, chunk=65536,
+                          capacity=1 << 20, seed=1, fmin=0.0313, fmax=2.0):
+    return SurrogateConfig(om=om, pm=pm, hidden=hidden, layers=layers, batch=batch, chunk=chunk,
+                           capacity=capacity, seed=seed, fmin=fmin, fmax=fmax)
+
+
+def make_surrogate_policy(audit_stride=16, resid_tol=0.05, audit_tol=0.25, range_margin=0.1,
+                          zmax=8.0):
+    return SurrogatePolicy(audit_stride=audit_stride, resid_tol=resid_tol, audit_tol=audit_tol,
+                           range_margin=range_margin, zmax=zmax)
+
+
 # Stokes depth levels of the reference namelist (namelists/main.nml:72).
 DEFAULT_DEPTHS = [0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.2, 1.4,
                   1.6, 1.8, 2.0, 2.4, 2.8, 3.2, 3.6, 4.0, 4.5, 5.0, 6, 7, 8, 9,

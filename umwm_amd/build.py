@@ -22,7 +22,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 FLANG = os.path.join(ROCM, "bin", "amdflang")
 
-HIP_SOURCES = ["ops.hip", "fused.hip", "capi.hip"]
+HIP_SOURCES = ["ops.hip", "fused.hip", "capi.hip", "mlp.hip", "surrogate.hip"]
 CXX_SOURCES = ["setup.cpp", "rawio.cpp"]
 
 HIP_FLAGS = [

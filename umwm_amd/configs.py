@@ -101,6 +101,38 @@ def c1(mm=64, nm=48):
                     note="64x48 regional, 10 km, 4000 m, 15 m/s toward 0 rad, JONSWAP fp=0.3 Hz")
 
 
+def c1_seeded(seed, mm=64, nm=48, hours=1, vary=True):
+    """The config-1 grid (64x48, 10 km, 4000 m, om = 37, pm = 36) with a
+    seeded wind (mean speed 10-20 m/s, any direction) and JONSWAP start
+    (fp 0.2-0.4 Hz, alpha 0.005-0.02): the sample source of the learned
+    surrogate, whose evaluation takes a seed the training never saw.  With
+    `vary` the wind is a forcing field whose speed varies by +-25 % and
+    direction by +-20 deg over the grid (Gaussian random fields of 150 km
+    correlation), so that the points of one run differ; else it is uniform
+    as in config 1."""
+    rng = np.random.default_rng(10_000 + seed)
+    w = float(rng.uniform(10.0, 20.0))
+    wd = float(rng.uniform(-np.pi, np.pi))
+    alpha = float(rng.uniform(0.005, 0.02))
+    fp = float(rng.uniform(0.2, 0.4))
+    kw = dict(mm=mm, nm=nm, om=37, pm=36, fmin=0.0313, fmax=2.0, fprog=2.0, dtg=3600.0,
+              delx=1e4, dely=1e4, dpt=4000.0, wspd0=w, wdir0=wd, uc0=0.0, vc0=0.0, rhoa0=1.2,
+              rhow0=1030.0, fice0=0.0, stokes=1)
+    forcing = None
+    if vary:
+        xy = planar_xyz(mm, nm, 1e4, 1e4)
+        sp = w * (1 + 0.25 * np.tanh(grf(xy, 1.5e5, rng, nfeat=128)))
+        dr = wd + np.radians(20.0) * np.tanh(grf(xy, 1.5e5, rng, nfeat=128))
+        forcing = dict(uw=sp * np.cos(dr), vw=sp * np.sin(dr))
+        kw.update(winds=1)
+    p = abi.make_params(**kw)
+    js = abi.make_jonswap(alpha=alpha, gamma=3.3, sigma_a=0.07, sigma_b=0.09, fp=fp,
+                          fp_from_wind=False, dir_from_wind=True)
+    return Workload(f"c1_s{seed}", p, forcing=forcing, jonswap=js, hours=hours,
+                    snapshots=(lambda h: forcing) if forcing else None,
+                    note=f"config-1 grid, seed {seed}: {w:.1f} m/s toward {wd:.2f} rad")
+
+
 def c2(seed=2, mm=720, nm=360, om=37, pm=36, hours=6, land_frac=0.30):
     rng = np.random.default_rng(seed)
     dlon, dlat = 360.0 / mm, 180.0 / nm

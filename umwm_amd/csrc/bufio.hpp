@@ -163,6 +163,12 @@ __device__ __forceinline__ void cld4(const T* p, int idx, T& a, T& b, T& c, T& d
   d = v[3];
 }
 
+// A bfloat16 value (raw 16-bit buffer load) widened to float.
+__device__ __forceinline__ float bld_bf16(rsrc_t r, uint32_t voff, uint32_t soff = 0) {
+  const unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+  return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
 // A lane offset past every descriptor's range: the hardware range check
 // drops a store made with it (descriptors cover less than 2^31 bytes), so
 // stores can be predicated per lane without a divergent branch.

@@ -14,6 +14,7 @@
 #include "domain.hpp"
 #include "physics.hpp"
 #include "rawio.hpp"
+#include "surrogate.hpp"
 
 // Scratch of umwm_hip_substeps_noalloc: the fused kernels' prefix moments
 // and stress partials ([om][ldi][4] each in the working precision), which
@@ -197,6 +198,10 @@ int ensure_ops(DomainImpl<T>& D) {
   if (D.ops_ready) return 0;
   const size_t ns = (size_t)D.h.om * D.h.pm * D.h.ldi, nt = (size_t)D.h.om * D.h.ldi;
   DomainDev<T>& h = D.h;
+  h.ef = D.arena.template alloc<T>(ns);
+  if (!h.ef) return fail(UMWM_ERR_NOMEM, "device allocation of operator arrays failed");
+  // ef as the fused path leaves it: E^{n+1}, equal to e
+  UMWM_HIP_TRY(hipMemcpyAsync(h.ef, h.e, ns * sizeof(T), hipMemcpyDeviceToDevice, D.stream));
   h.ssin = D.arena.template alloc<T>(ns);
   h.sds = D.arena.template alloc<T>(ns);
   h.snl = D.arena.template alloc<T>(ns);
@@ -268,10 +273,12 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
     delete D;
     return fail(UMWM_ERR_SIZE, "domain too large for 32-bit buffer descriptors");
   }
+  // ef is an operator-path array (allocated with the others on first use):
+  // the fused substep keeps E_s in place and never stores it
   h.e = A.template alloc<T>(ns);
-  h.ef = A.template alloc<T>(ns);
+  h.ef = nullptr;
   h.es = A.template alloc<T>(ns);
-  if (!h.e || !h.ef || !h.es) {
+  if (!h.e || !h.es) {
     delete D;
     return fail(UMWM_ERR_NOMEM, "device allocation of the spectrum failed");
   }
@@ -422,8 +429,7 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   }
   // spectrum: e = tiny(e) everywhere, ghost included (src/umwm_init.F90:1182)
   const T tiny = sizeof(T) == 8 ? T(DBL_MIN) : T(FLT_MIN);
-  if ((rc = dfill<T>(h.e, ns, tiny, s)) || (rc = dfill<T>(h.ef, ns, T(0), s)) ||
-      (rc = dfill<T>(h.es, ns, tiny, s))) {
+  if ((rc = dfill<T>(h.e, ns, tiny, s)) || (rc = dfill<T>(h.es, ns, tiny, s))) {
     delete D;
     return rc;
   }
@@ -464,8 +470,7 @@ int create_impl(const umwm_params& P, const umwm_grid_input* g, const umwm_forci
   for (int l = 0; l < S.lm; ++l) dlh[l] = S.depth_l[l];
   UMWM_HIP_TRY(hipMemcpyAsync(dl, dlh.data(), lm * sizeof(double), hipMemcpyHostToDevice, s));
   D->stokes.depth = dl;
-  // us/vs carry om extra rows used as scratch for directional sums
-  const size_t nus = (size_t)(lm + std::max(om, 1)) * ldi;
+  const size_t nus = (size_t)lm * ldi;
   D->stokes.us = A.template alloc<double>(nus);
   D->stokes.vs = A.template alloc<double>(nus);
   D->stokes.ds = A.template alloc<double>(ldi);
@@ -610,13 +615,30 @@ int enable_keep(DomainImpl<T>& D) {
 // One fused substep; `dd` is the device domain record the fused kernels
 // read (the domain's own by default, or the one whose scratch lives in the
 // shared array).
+// The attached surrogate's rates for the current spectrum (FP32 domains).
+template <typename T>
+int surrogate_rates(DomainImpl<T>& D, hipStream_t s, bool& on) {
+  on = false;
+  if constexpr (std::is_same_v<T, float>) {
+    if (D.sur && D.h.sur_r) {
+      on = true;
+      return launch_surrogate_inference(D, s);
+    }
+  }
+  return 0;
+}
+
 template <typename T>
 int fused_substep(DomainImpl<T>& D, const DomainDev<T>* dd = nullptr) {
   int rc;
   if (any_forcing(D) && (rc = launch_forcinginterpolate(D, true))) return rc;
+  bool sur = false;
+  if (D.timing) UMWM_HIP_TRY(hipEventRecord(D.ev[5], D.stream));
+  if ((rc = surrogate_rates(D, D.stream, sur))) return rc;
+  if (D.timing) UMWM_HIP_TRY(hipEventRecord(D.ev[6], D.stream));
   const int rotation = (D.S.P.currents || D.first || !D.rot_valid) ? 1 : 0;
   rc = launch_fused<T>(dd ? dd : D.d, D.tasks, D.ntasks, D.h.om, D.first ? 1 : 0, rotation,
-                       D.tile, D.h.e_keep != nullptr, D.stream, D.timing ? D.ev : nullptr);
+                       D.tile, D.h.e_keep != nullptr, D.stream, D.timing ? D.ev : nullptr, sur);
   if (rc) return rc;
   D.ef_mirrors_e = true;
   D.rot_valid = true;
@@ -631,6 +653,12 @@ int fused_substep(DomainImpl<T>& D, const DomainDev<T>* dd = nullptr) {
       UMWM_HIP_TRY(hipEventElapsedTime(&ms, D.ev[q], D.ev[q + 1]));
       D.ms[q] += ms;
       D.launches[q] += 1;
+    }
+    if (sur) {
+      float ms = 0;
+      UMWM_HIP_TRY(hipEventElapsedTime(&ms, D.ev[5], D.ev[6]));
+      D.ms[4] += ms;
+      D.launches[4] += 1;
     }
   }
   return 0;
@@ -720,7 +748,11 @@ bool find_field(DomainImpl<T>& D, const std::string& n, FieldRef& r) {
   auto set = [&](FieldKind k, void* p) { r.kind = k; r.ptr = p; return true; };
   // spectra
   if (n == "e") return set(F_SPEC, h.e);
-  if (n == "ef") return set(F_SPEC, D.ef_mirrors_e ? h.e : h.ef);
+  if (n == "ef") {
+    if (D.ef_mirrors_e) return set(F_SPEC, h.e);
+    if (ensure_ops(D)) return false;
+    return set(F_SPEC, h.ef);
+  }
   if (n == "es") return set(F_SPEC, h.es);
   if (n == "ssin" || n == "sds" || n == "snl" || n == "dummy" || n == "rotl" || n == "rotr" ||
       n == "sdt" || n == "sice") {
@@ -1553,9 +1585,9 @@ int umwm_hip_set_timing(umwm_domain* dom, int enable) {
   });
 }
 
-int umwm_hip_get_timing(umwm_domain* dom, double ms[4], int64_t counts[4], int reset) {
+int umwm_hip_get_timing(umwm_domain* dom, double ms[5], int64_t counts[5], int reset) {
   return dispatch(dom, [&](auto& D) {
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 5; ++q) {
       if (ms) ms[q] = D.ms[q];
       if (counts) counts[q] = D.launches[q];
       if (reset) {
@@ -1725,9 +1757,15 @@ int batch_launch_substep(BatchImpl<T>& B, int first) {
   if (int rc = launch_forcing_list<T>(B.d_doms, B.d_fds, B.d_flist, B.nforced, B.maximm,
                                       B.forced_currents, true, B.stream))
     return rc;
+  bool any_sur = false;
+  for (auto* D : B.doms) {
+    bool on = false;
+    if (int rc = surrogate_rates(*D, B.stream, on)) return rc;
+    any_sur |= on;
+  }
   for (const auto& g : B.groups)
     if (int rc = launch_fused<T>(B.d_doms, g.tasks, g.ntasks, g.omax, first, first, g.tile,
-                                 B.keep, B.stream, nullptr))
+                                 B.keep, B.stream, nullptr, any_sur))
       return rc;
   for (auto* D : B.doms) D->ef_mirrors_e = true;
   const int n = (int)B.doms.size();
@@ -1737,17 +1775,11 @@ int batch_launch_substep(BatchImpl<T>& B, int first) {
   return 0;
 }
 
+// Members still in their first substep take it eagerly (cold dts = 0,
+// rotation tendency); then the steady-state substep is captured once.
 template <typename T>
-int batch_run_dtg_impl(BatchImpl<T>& B, int use_graph, int32_t* nsub) {
-  const int n = (int)B.doms.size();
-  for (auto* D : B.doms) UMWM_HIP_TRY(hipStreamSynchronize(D->stream));
+int batch_prepare(BatchImpl<T>& B, int use_graph) {
   int rc;
-  if ((rc = batch_refresh(B))) return rc;
-  for (auto* D : B.doms)
-    hipLaunchKernelGGL(k_begin_dtg, dim3(1), dim3(64), 0, B.stream, D->h.sc);
-  UMWM_HIP_TRY(hipGetLastError());
-  // members still in their first substep take it eagerly (cold dts = 0,
-  // rotation tendency) before the captured steady-state substep is replayed
   bool any_first = false;
   for (auto* D : B.doms) any_first |= D->first;
   if (any_first) {
@@ -1771,6 +1803,43 @@ int batch_run_dtg_impl(BatchImpl<T>& B, int use_graph, int32_t* nsub) {
     B.graph = g;
     UMWM_HIP_TRY(hipGraphInstantiate(&B.exec, g, nullptr, nullptr, 0));
   }
+  return 0;
+}
+
+// n substeps of every member, launched back to back (graph replays) with
+// one host wait at the end; a member whose sumt reached dtg does nothing.
+// nsub[k] receives member k's substeps taken this dtg.
+template <typename T>
+int batch_substeps_impl(BatchImpl<T>& B, int n, int use_graph, int32_t* nsub) {
+  const int nd = (int)B.doms.size();
+  for (auto* D : B.doms) UMWM_HIP_TRY(hipStreamSynchronize(D->stream));
+  int rc;
+  if ((rc = batch_refresh(B))) return rc;
+  if ((rc = batch_prepare(B, use_graph))) return rc;
+  for (int c = 0; c < n; ++c) {
+    if (use_graph) {
+      UMWM_HIP_TRY(hipGraphLaunch(B.exec, B.stream));
+    } else if ((rc = batch_launch_substep(B, 0))) {
+      return rc;
+    }
+  }
+  UMWM_HIP_TRY(hipMemcpyAsync(B.h_view, B.d_view, nd * sizeof(ScalarsView), hipMemcpyDeviceToHost, B.stream));
+  UMWM_HIP_TRY(hipStreamSynchronize(B.stream));
+  if (nsub)
+    for (int k = 0; k < nd; ++k) nsub[k] = B.h_view[k].nsub;
+  return 0;
+}
+
+template <typename T>
+int batch_run_dtg_impl(BatchImpl<T>& B, int use_graph, int32_t* nsub) {
+  const int n = (int)B.doms.size();
+  for (auto* D : B.doms) UMWM_HIP_TRY(hipStreamSynchronize(D->stream));
+  int rc;
+  if ((rc = batch_refresh(B))) return rc;
+  for (auto* D : B.doms)
+    hipLaunchKernelGGL(k_begin_dtg, dim3(1), dim3(64), 0, B.stream, D->h.sc);
+  UMWM_HIP_TRY(hipGetLastError());
+  if ((rc = batch_prepare(B, use_graph))) return rc;
   // replay in chunks; finished members exit their blocks at once
   int chunk = 4, total = 0;
   for (;;) {
@@ -1845,6 +1914,14 @@ int umwm_hip_batch_run_dtg(umwm_batch* b, int use_graph, int32_t* nsub) {
   return batch_run_dtg_impl(*static_cast<BatchImpl<double>*>(b->impl), use_graph, nsub);
 }
 
+int umwm_hip_batch_substeps(umwm_batch* b, int32_t n, int use_graph, int32_t* nsub) {
+  if (!b) return fail(UMWM_ERR_ARGUMENT, "null batch");
+  if (n < 1) return fail(UMWM_ERR_ARGUMENT, "batch_substeps: n must be >= 1");
+  if (b->precision == 4)
+    return batch_substeps_impl(*static_cast<BatchImpl<float>*>(b->impl), n, use_graph, nsub);
+  return batch_substeps_impl(*static_cast<BatchImpl<double>*>(b->impl), n, use_graph, nsub);
+}
+
 int umwm_hip_batch_tile(const umwm_batch* b, int32_t out[5]) {
   if (!b || !out) return fail(UMWM_ERR_ARGUMENT, "null argument");
   const Tile t = b->precision == 4 ? static_cast<BatchImpl<float>*>(b->impl)->tile
@@ -1878,6 +1955,103 @@ int umwm_hip_batch_set_tile(umwm_batch* b, const int32_t tile[5]) {
   };
   if (b->precision == 4) return apply(static_cast<BatchImpl<float>*>(b->impl));
   return apply(static_cast<BatchImpl<double>*>(b->impl));
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// surrogate entry points that take a domain
+// ---------------------------------------------------------------------
+namespace {
+
+__global__ void k_zero_f64(double* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+int attach_impl(DomainImpl<float>& D, SurrogateImpl* S, const umwm_surrogate_policy* pol) {
+  UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  if (!S) {
+    D.sur = nullptr;
+    D.h.sur_r = nullptr;
+    UMWM_HIP_TRY(hipMemcpy(D.d, &D.h, sizeof(DomainDev<float>), hipMemcpyHostToDevice));
+    return 0;
+  }
+  if (S->om != D.h.om || S->pm != D.h.pm)
+    return fail(UMWM_ERR_ARGUMENT, "surrogate_attach: the domain's om x pm differs from the network's");
+  if (!S->stats_ready) return fail(UMWM_ERR_STATE, "surrogate_attach: train the network first");
+  if (!D.h.sur_r) {
+    const size_t nb = (size_t)D.h.om * D.h.pm;
+    uint16_t* r = D.arena.template alloc<uint16_t>(nb * D.h.ldi);
+    int32_t* ok = D.arena.template alloc<int32_t>(D.h.ldi);
+    int32_t* use = D.arena.template alloc<int32_t>(D.h.ldi);
+    double* st = D.arena.template alloc<double>(SS_N);
+    if (!r || !ok || !use || !st) return fail(UMWM_ERR_NOMEM, "surrogate_attach: allocation failed");
+    D.h.sur_r = r;
+    D.h.sur_ok = ok;
+    D.h.sur_use = use;
+    D.h.sur_stats = st;
+    if (int rc = dfill<int32_t>(ok, D.h.ldi, 0, D.stream)) return rc;
+    if (int rc = dfill<int32_t>(use, D.h.ldi, 0, D.stream)) return rc;
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, D.stream, st, (int)SS_N);
+    UMWM_HIP_TRY(hipGetLastError());
+  }
+  umwm_surrogate_policy p{};
+  p.audit_stride = 16;
+  p.resid_tol = 0.05;
+  p.audit_tol = 0.25;
+  p.range_margin = 0.1;
+  p.zmax = 8.0;
+  if (pol) p = *pol;
+  if (p.audit_stride < 0 || !(p.resid_tol >= 0) || !(p.audit_tol >= 0) || !(p.range_margin >= 0) ||
+      !(p.zmax > 0))
+    return fail(UMWM_ERR_ARGUMENT, "surrogate policy: audit_stride >= 0, tolerances >= 0, zmax > 0");
+  D.h.sur_audit_stride = p.audit_stride;
+  D.h.sur_resid_tol = (float)p.resid_tol;
+  D.h.sur_audit_tol = (float)p.audit_tol;
+  D.sur_margin = p.range_margin;
+  D.sur_zmax = p.zmax;
+  D.sur = S;
+  UMWM_HIP_TRY(hipMemcpyAsync(D.d, &D.h, sizeof(DomainDev<float>), hipMemcpyHostToDevice, D.stream));
+  UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int umwm_hip_surrogate_collect(umwm_surrogate* s, umwm_domain* dom, int64_t* total) {
+  if (!s || !s->impl) return fail(UMWM_ERR_ARGUMENT, "null surrogate");
+  return dispatch(dom, [&](auto& D) {
+    if (int rc = ensure_ops(D)) return rc;
+    return surrogate_collect(*s->impl, D, total);
+  });
+}
+
+int umwm_hip_surrogate_attach(umwm_domain* dom, umwm_surrogate* s, const umwm_surrogate_policy* pol) {
+  if (!dom || !dom->base) return fail(UMWM_ERR_ARGUMENT, "null domain");
+  if (dom->base->precision != 4)
+    return fail(UMWM_ERR_ARGUMENT, "surrogate_attach: the surrogate serves FP32 domains");
+  return attach_impl(*static_cast<DomainImpl<float>*>(dom->base), s ? s->impl : nullptr, pol);
+}
+
+int umwm_hip_surrogate_stats(umwm_domain* dom, double out[7], int reset) {
+  if (!dom || !dom->base || !out) return fail(UMWM_ERR_ARGUMENT, "null argument");
+  if (dom->base->precision != 4) return fail(UMWM_ERR_STATE, "surrogate_stats: no surrogate");
+  auto& D = *static_cast<DomainImpl<float>*>(dom->base);
+  if (!D.h.sur_stats) return fail(UMWM_ERR_STATE, "surrogate_stats: no surrogate attached");
+  double st[SS_N];
+  UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  UMWM_HIP_TRY(hipMemcpy(st, D.h.sur_stats, sizeof(st), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 6; ++k) out[k] = st[k];
+  out[6] = st[SS_AUDIT_DEN] > 0 ? st[SS_AUDIT_NUM] / st[SS_AUDIT_DEN] : 0.0;
+  if (reset) {
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, D.stream, D.h.sur_stats, (int)SS_N);
+    UMWM_HIP_TRY(hipGetLastError());
+    UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  }
+  return 0;
 }
 
 }  // extern "C"

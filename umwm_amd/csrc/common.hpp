@@ -112,6 +112,20 @@ struct DomainDev {
   T* pfx;
   T* spart;
   Scalars* sc;
+  // learned surrogate of the FP32 path (null sur_r: none attached): the
+  // network's combined rate R = ssin - sds - sbf per bin as bfloat16 bits,
+  // [o*pm+p][ldi]; per point the input-range verdict (sur_ok) and the
+  // choice k_rates made for the current substep (sur_use, read by
+  // k_integrate); statistics (SurStat, surrogate.hpp); the policy: every
+  // sur_audit_stride-th workgroup of points (rotating with the substep) is
+  // also evaluated by the plain kernel, and a point falls back where its
+  // negative implied dissipation or its audited error exceeds the
+  // tolerances (energy-weighted fractions)
+  const uint16_t* sur_r;
+  int32_t *sur_ok, *sur_use;
+  double* sur_stats;
+  int32_t sur_audit_stride, sur_pad;
+  float sur_resid_tol, sur_audit_tol;
 };
 
 // Slots of one (frequency, point) record of DomainDev::ftab: the tables the

@@ -26,6 +26,8 @@ struct Tile {
 
 bool tile_supported(int D);
 
+struct SurrogateImpl;  // surrogate.hpp
+
 class DeviceArena {
  public:
   ~DeviceArena() { release(); }
@@ -61,8 +63,9 @@ struct DomainBase {
   // the fused substep does not store E_s: after it, ef equals e (as after
   // the reference's e = ef) and reads of "ef" are served from e
   bool ef_mirrors_e = false;
-  double ms[4] = {0, 0, 0, 0};          // rates, integrate, advect, stress
-  int64_t launches[4] = {0, 0, 0, 0};
+  // rates, integrate, advect, stress, surrogate inference
+  double ms[5] = {0, 0, 0, 0, 0};
+  int64_t launches[5] = {0, 0, 0, 0, 0};
 };
 
 template <typename T>
@@ -81,6 +84,9 @@ struct DomainImpl : DomainBase {
   unsigned char* scr = nullptr;
   DomainDev<T>* d_scr = nullptr;
   DomainDev<T> h_scr{};
+  // attached learned surrogate (borrowed; FP32 domains only)
+  SurrogateImpl* sur = nullptr;
+  double sur_margin = 0.1, sur_zmax = 8.0;
   DiagDev diag{};
   OcnDev ocn{};  // allocated with the operator arrays
   StokesDev stokes{};
@@ -90,7 +96,8 @@ struct DomainImpl : DomainBase {
   Scalars* sc_host = nullptr;  // pinned mirror
   bool ops_ready = false;      // operator-path arrays allocated
   bool rot_valid = false;      // rotation computed at least once
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // ev[0..4]: around the fused kernels; ev[5], ev[6]: around the inference
+  hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   // forcing device buffers (level n and n+1)
   T *uwb = nullptr, *uwf = nullptr, *vwb = nullptr, *vwf = nullptr;
   T *ucb = nullptr, *ucf = nullptr, *vcb = nullptr, *vcf = nullptr;
@@ -125,9 +132,16 @@ template <typename T>
 int launch_forcing_list(const DomainDev<T>* doms, const ForcingDev<T>* fds, const int32_t* list,
                         int n, int maximm, bool currents, bool gate, hipStream_t s);
 
-// Fused substep kernels over a task list that may span many domains.
+// Fused substep kernels over a task list that may span many domains; `sur`
+// selects the variants that take the rates of an attached surrogate where
+// a domain has one.
 template <typename T>
 int launch_fused(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int omax,
-                 int first, int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev);
+                 int first, int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev,
+                 bool sur = false);
+
+// The attached surrogate's rates for the domain's current spectrum and
+// forcing (features, network, R into DomainDev::sur_r), on stream s.
+int launch_surrogate_inference(DomainImpl<float>& D, hipStream_t s);
 
 }  // namespace umwm

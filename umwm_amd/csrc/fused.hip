@@ -60,6 +60,7 @@
 #include "bufio.hpp"
 #include "domain.hpp"
 #include "physics.hpp"
+#include "surrogate.hpp"
 
 namespace umwm {
 namespace {
@@ -208,6 +209,11 @@ struct FreqTab {
 template <typename T>
 __device__ __forceinline__ rsrc_t freq_rsrc(const T* base, int o, int rows, size_t ldi) {
   return row_rsrc(base, (size_t)o, (size_t)rows * ldi);
+}
+
+// Frequency o's slice of the surrogate's bf16 rate rows, [om][pm][ldi].
+__device__ __forceinline__ rsrc_t sur_rsrc(const uint16_t* base, int o, int pm, size_t ldi) {
+  return row_rsrc(base, (size_t)o, (size_t)pm * ldi);
 }
 
 // First N slots of the record of (o, j); vr = j * FT_N * sizeof(T).
@@ -360,7 +366,7 @@ __device__ __forceinline__ T sin_m(const SinFreq<T>& q, T y, T m) {
 // ---------------------------------------------------------------------
 // k_rates
 // ---------------------------------------------------------------------
-template <typename T, int DMAX, bool FULL>
+template <typename T, int DMAX, bool FULL, bool SUR>
 __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ doms,
                                                const BlockTask* __restrict__ tasks, int first,
                                                int rot_all) {
@@ -377,6 +383,9 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
   const int j = tk.i0 + lane;
   const bool active = sc->sumt < d.dtg;
   const bool valid = active && j < d.im;
+  // substeps taken so far: the rotating audit group (read before the last
+  // block counts this substep)
+  [[maybe_unused]] const int nsub0 = sc->nsub;
   const int pm = d.pm, om = d.om;
   const size_t ldi = d.ldi, ldq = d.ldq;
   const int D = FULL ? DMAX : (pm + W - 1) / W, p0 = w * D;
@@ -385,6 +394,8 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
   const uint32_t vj = (uint32_t)j * (uint32_t)sizeof(T);
   const uint32_t vr = (uint32_t)j * (uint32_t)(FT_N * sizeof(T));
   const uint32_t rb = (uint32_t)ldi * (uint32_t)sizeof(T);  // bytes of one direction row
+  // the surrogate's bf16 rate rows
+  [[maybe_unused]] const uint32_t vj2 = (uint32_t)j * 2u, rb2 = (uint32_t)ldi * 2u;
   // Arrays the frequency loop uses, held in registers: buffer stores are
   // opaque to the compiler, which would otherwise reload these pointers
   // from the domain record after every store.
@@ -445,14 +456,50 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       mq[q] = over ? fs2 : T(1);
       if (over) mb |= MB(1) << q;
     }
-    double A0 = 0, AC = 0, AS = 0;
+    // Surrogate (FP32): lanes whose inputs passed the range check take the
+    // network's combined rate R = ssin - sds - sbf, from which the implied
+    // dissipation sds = ssin - sbf - R (clamped at 0) enters the rate as
+    // the plain one would.  Audited workgroups and waves holding lanes that
+    // failed the check also evaluate the plain sds; a lane whose implied
+    // dissipation is negative in more than the residual tolerance of its
+    // energy-weighted rates, or whose audited error exceeds the audit
+    // tolerance, falls back to the plain rates (a second, plain-only sweep
+    // when its wave did not evaluate them).  All waves of a workgroup hold
+    // the same 64 points, so these wave-level choices agree across waves.
+    [[maybe_unused]] int sok = 0;
+    [[maybe_unused]] bool aud = false;
+    bool do_sur = false;
+    bool do_plain = true;
+    [[maybe_unused]] const uint16_t* const rsur = d.sur_r;
+    if constexpr (SUR) {
+      if (rsur) {
+        sok = valid ? pload(d.sur_ok, ldi, j) : 0;
+        aud = d.sur_audit_stride > 0 && ((tk.local + nsub0) % d.sur_audit_stride) == 0;
+        do_sur = __any(sok != 0);
+        do_plain = aud || __any(valid && sok == 0);
+      }
+    }
     T rmax = 0;
+    [[maybe_unused]] T rmax_s = 0, neg = 0, tot = 0, anum = 0, aden = 0;
+    // One sweep over frequencies: `dp` evaluates the plain dissipation,
+    // `ds` the network's; the first pass (`pass0`) also stores the prefix
+    // moments and the rotation maximum.
+    auto sweep = [&](bool dp, bool ds, bool pass0) {
+    double A0 = 0, AC = 0, AS = 0;
     T ec[DMAX];
+    [[maybe_unused]] T rc[DMAX];
     FreqTab<T> tc;
     {
       const rsrc_t ep = freq_rsrc(eb, 0, pm, ldi);
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) ec[q] = bld<T>(FULL || q < nd ? ep : rz, vj, dirp(q) * rb);
+      if constexpr (SUR) {
+        if (ds) {
+          const rsrc_t rp = sur_rsrc(rsur, 0, pm, ldi);
+#pragma unroll
+          for (int q = 0; q < DMAX; ++q) rc[q] = bld_bf16(FULL || q < nd ? rp : rz, vj2, dirp(q) * rb2);
+        }
+      }
     }
     // FP32 derives the tables from k (TabGen); FP64 loads the record
     constexpr bool kDerive = sizeof(T) == 4;
@@ -462,9 +509,17 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       // issue the next frequency's loads before computing this one
       const int on = o + 1 < om ? o + 1 : o;
       T en[DMAX];
+      [[maybe_unused]] T rn[DMAX];
       const rsrc_t epn = freq_rsrc(eb, on, pm, ldi);
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(FULL || q < nd ? epn : rz, vj, dirp(q) * rb);
+      if constexpr (SUR) {
+        if (ds) {
+          const rsrc_t rp = sur_rsrc(rsur, on, pm, ldi);
+#pragma unroll
+          for (int q = 0; q < DMAX; ++q) rn[q] = bld_bf16(FULL || q < nd ? rp : rz, vj2, dirp(q) * rb2);
+        }
+      }
       float knext = 0.f;
       if constexpr (kDerive) {
         knext = tg.load_k(o + 4);
@@ -479,7 +534,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       const T crest = tc[FT_SBF] + sdt + nu4 * k2 - sice;  // sbf + sdt + sdv - sice
       const SinFreq<T> sf = sin_freq(s, tc[FT_LOGL2], tc[FT_CP0], fo * k * ig, sin_diss1, fs2);
       const T pa0 = T(hdth * A0), pac = T(hdth * AC), pas = T(hdth * AS);
-      if (valid && w == 0) {
+      if (pass0 && valid && w == 0) {
         if constexpr (sizeof(T) == 4) {
           bst16(freq_rsrc(pfxb, o, 4, ldi), (uint32_t)j * 16u,
                 typename Vec16<T>::type{pa0, pac, pas, T(0)});
@@ -494,11 +549,27 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       const T Lk = T(4) * spow * Tr<T>::lg(k) + Tr<T>::lg(cfo);  // lg(k^4) = 4 lg k
       const T cs = T(1) - (tc[FT_B1] + tc[FT_B2]);                // snl_arg
       const T q0 = T(1) + mssf * pa0, qc = mssf * pac, qs = mssf * pas;
+      [[maybe_unused]] const T sbfv = tc[FT_SBF], kdkv = tc[FT_KDK];
       T m0 = 0, mc = 0, ms = 0, rq = 0;
+      [[maybe_unused]] T rqs = 0;
       // (e k4)^p of |e|: a bin emptied by round-off to about -1e-30 gives a
       // vanishing sds, as the 0 the reference's power of a non-positive
       // base is taken to be here, without a clamp instruction
       auto sds_of = [&](T dm, T e) { return dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk)); };
+      // the network's implied dissipation of one bin: its rate, residual
+      // and audit sums
+      auto sur_bin = [&](T ss, T R, T e, T sdp, bool use_slot) {
+        const T sdss = fmax(ss - sbfv - R, T(0));
+        const T r2 = ss - fma(sdss, cs, crest);
+        rqs = fmax(rqs, use_slot ? fabs(r2) : T(0));
+        const T wv = e * kdkv;
+        neg = fma(fmax(R - (ss - sbfv), T(0)), wv, neg);
+        tot = fma(fabs(ss) + sbfv, wv, tot);
+        if (aud) {
+          anum = fma(fabs(sdss - sdp), wv, anum);
+          aden = fma(sdp + fabs(ss) + sbfv, wv, aden);
+        }
+      };
       // an opaque copy per frequency (an empty asm statement): without it the
       // compiler hoists the DMAX bit tests out of the frequency loop into
       // DMAX registers, which costs a wave per SIMD at DMAX = 36
@@ -518,9 +589,20 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
           const T ssa = ya * (ya > T(0) ? sf.pos : na), ssb = yb * (yb > T(0) ? sf.pos : nb);
           T dm = fma(qc, c2, fma(qs, s2, q0));
           dm = dm * dm;
-          const T ra = ssa - fma(sds_of(dm, ea), cs, crest);
-          const T rb2 = ssb - fma(sds_of(dm, eb2), cs, crest);
-          rq = fmax(rq, fmax(fabs(ra), fabs(rb2)));
+          T spa = 0, spb = 0;
+          if (dp) {
+            spa = sds_of(dm, ea);
+            spb = sds_of(dm, eb2);
+            const T ra = ssa - fma(spa, cs, crest);
+            const T rb2 = ssb - fma(spb, cs, crest);
+            rq = fmax(rq, fmax(fabs(ra), fabs(rb2)));
+          }
+          if constexpr (SUR) {
+            if (ds) {
+              sur_bin(ssa, rc[h], ea, spa, true);
+              sur_bin(ssb, rc[h + H], eb2, spb, true);
+            }
+          }
           const T ep2 = ea + eb2;
           m0 += ep2;
           mc = fma(ep2, c2, mc);
@@ -535,15 +617,25 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
           const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
           T dm = fma(qc, c2, fma(qs, s2, q0));
           dm = dm * dm;
-          const T r = ss - fma(sds_of(dm, e), cs, crest);
-          rq = fmax(rq, FULL || q < nd ? fabs(r) : T(0));
+          T sp = 0;
+          if (dp) {
+            sp = sds_of(dm, e);
+            const T r = ss - fma(sp, cs, crest);
+            rq = fmax(rq, FULL || q < nd ? fabs(r) : T(0));
+          }
+          if constexpr (SUR) {
+            if (ds) sur_bin(ss, rc[q], e, sp, FULL || q < nd);
+          }
           m0 += e;
           mc = fma(e, c2, mc);
           ms = fma(e, s2, ms);
         }
       }
-      if (o <= oc) rmax = fmax(rmax, rq);  // rates of prognostic bins only
-      if (rotation) {
+      if (o <= oc) {  // rates of prognostic bins only
+        rmax = fmax(rmax, rq);
+        if constexpr (SUR) rmax_s = fmax(rmax_s, rqs);
+      }
+      if (rotation && pass0) {
         // rotation tendency of the first substep (src/umwm_advection.F90:329-349)
         const rsrc_t cpr = row_rsrc(d.cp0, o, ldi);
         const T dcx = bld<T>(cpr, vie) - bld<T>(cpr, viw);
@@ -579,11 +671,67 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       AS += k3 * s2s;
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) ec[q] = en[q];
+      if constexpr (SUR) {
+        if (ds) {
+#pragma unroll
+          for (int q = 0; q < DMAX; ++q) rc[q] = rn[q];
+        }
+      }
       if constexpr (kDerive) tg.advance(o, knext);
+    }
+    };
+    sweep(do_plain, do_sur, true);
+    // the surrogate's verdict per lane (the same in every wave)
+    [[maybe_unused]] bool use = false;
+    if constexpr (SUR) {
+      if (rsur) {
+        double sneg = neg, stot = tot, snum = anum, sden = aden;
+        if (W > 1) {
+          __syncthreads();
+          smem[(0 * W + w) * 64 + lane] = sneg;
+          smem[(1 * W + w) * 64 + lane] = stot;
+          smem[(2 * W + w) * 64 + lane] = snum;
+          smem[(3 * W + w) * 64 + lane] = sden;
+          __syncthreads();
+          sneg = stot = snum = sden = 0;
+          for (int q = 0; q < W; ++q) {
+            sneg += smem[(0 * W + q) * 64 + lane];
+            stot += smem[(1 * W + q) * 64 + lane];
+            snum += smem[(2 * W + q) * 64 + lane];
+            sden += smem[(3 * W + q) * 64 + lane];
+          }
+        }
+        const bool rfail = sneg > (double)d.sur_resid_tol * stot;
+        const bool afail = aud && snum > (double)d.sur_audit_tol * sden;
+        use = valid && sok && !rfail && !afail;
+        if (!do_plain && __any(valid && !use)) {
+          sweep(true, false, false);  // plain rates for the lanes that fell back
+          do_plain = true;
+        }
+        if (w == 0) {
+          if (valid) pstore(d.sur_use, ldi, j, (int32_t)(use ? 1 : 0));
+          double st[SS_N] = {};
+          st[SS_POINTS] = valid ? 1 : 0;
+          st[SS_USED] = use ? 1 : 0;
+          st[SS_FALL_RANGE] = valid && !sok ? 1 : 0;
+          st[SS_FALL_RESID] = valid && sok && rfail ? 1 : 0;
+          st[SS_FALL_AUDIT] = valid && sok && !rfail && afail ? 1 : 0;
+          st[SS_AUDITED] = valid && aud ? 1 : 0;
+          st[SS_AUDIT_NUM] = valid && aud ? snum : 0;
+          st[SS_AUDIT_DEN] = valid && aud ? sden : 0;
+#pragma unroll
+          for (int q = 0; q < SS_N; ++q) {
+            const double v = wave_sum(st[q]);
+            if (lane == 0 && v != 0) atomicAdd(d.sur_stats + q, v);
+          }
+        }
+      }
     }
     // max |rate| of the point over all waves
     __syncthreads();
-    smem[w * 64 + lane] = (double)rmax;
+    T rsel = rmax;
+    if constexpr (SUR) rsel = use ? rmax_s : rmax;
+    smem[w * 64 + lane] = (double)rsel;
     __syncthreads();
     if (w == 0) {
       double mx = 0;
@@ -679,7 +827,7 @@ __device__ __forceinline__ Prefix<T> load_prefix(const T* pfx, size_t ldi, int o
   return r;
 }
 
-template <typename T, int DMAX, bool FULL>
+template <typename T, int DMAX, bool FULL, bool SUR>
 __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restrict__ doms,
                                                    const BlockTask* __restrict__ tasks) {
   const BlockTask tk = tasks[blockIdx.x];
@@ -732,11 +880,36 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
 #pragma unroll
   for (int q = 0; q < DMAX; ++q) H1[q] = H2[q] = 0;
   T b1m1 = 0, b2m2 = 0, b2m1 = 0;  // bf1r(o-1), bf2r(o-2), bf2r(o-1)
+  // Surrogate (FP32): the lanes k_rates let take the network's rates use
+  // the implied dissipation sds = max(ssin - sbf - R, 0) (after cothkd) in
+  // the integration, in snl and in the diagnostic tail, whose equilibrium
+  // k^-4 ((ssin - sdt - sdv + sice) / (cf coth dm))^(1/p) equals
+  // e ((ssin - sdt - sdv + sice) / sds)^(1/p) for the plain sds.
+  [[maybe_unused]] int use = 0;
+  bool do_plain = true;
+  [[maybe_unused]] bool do_sur = false;
+  [[maybe_unused]] const uint16_t* const rsur = d.sur_r;
+  [[maybe_unused]] const uint32_t vj2 = (uint32_t)j * 2u, rb2 = (uint32_t)ldi * 2u;
+  if constexpr (SUR) {
+    if (rsur) {
+      use = valid ? pload(d.sur_use, ldi, j) : 0;
+      do_sur = __any(use != 0);
+      do_plain = __any(valid && use == 0);
+    }
+  }
   T ec[DMAX];
+  [[maybe_unused]] T rc[DMAX];
   {
     const rsrc_t ep = freq_rsrc(eb, 0, pm, ldi);
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) ec[q] = bld<T>(FULL || q < nd ? ep : rz, vj, (p0 + q) * rb);
+    if constexpr (SUR) {
+      if (do_sur) {
+        const rsrc_t rp = sur_rsrc(rsur, 0, pm, ldi);
+#pragma unroll
+        for (int q = 0; q < DMAX; ++q) rc[q] = bld_bf16(FULL || q < nd ? rp : rz, vj2, (p0 + q) * rb2);
+      }
+    }
   }
   FreqTab<T> tc;
   Prefix<T> pc{};
@@ -751,9 +924,17 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   for (int o = 0; o < om; ++o) {
     const int on = o + 1 < om ? o + 1 : o;
     T en[DMAX];
+    [[maybe_unused]] T rn[DMAX];
     const rsrc_t epn = freq_rsrc(eb, on, pm, ldi);
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(FULL || q < nd ? epn : rz, vj, (p0 + q) * rb);
+    if constexpr (SUR) {
+      if (do_sur) {
+        const rsrc_t rp = sur_rsrc(rsur, on, pm, ldi);
+#pragma unroll
+        for (int q = 0; q < DMAX; ++q) rn[q] = bld_bf16(FULL || q < nd ? rp : rz, vj2, (p0 + q) * rb2);
+      }
+    }
     Prefix<T> pn{};
     float knext = 0.f;
     if constexpr (kDerive) {
@@ -777,6 +958,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     const T Lk = spow * lk4 + Tr<T>::lg(cf);  // sds before cothkd
     const T c0 = -dtx * crest, dtc = dtx * coth;
     const T icfc = Tr<T>::rcp(cf * coth);
+    [[maybe_unused]] const T icoth = Tr<T>::rcp(coth), sbfv = tc[FT_SBF];
     // dm = q0 + qc cos 2th + qs sin 2th = (q0 - qc) + ct (2 qc ct + 2 qs st),
     // with cos 2th = 2 ct^2 - 1 and sin 2th = 2 ct st: the sweep keeps
     // only ct and st per direction
@@ -806,7 +988,15 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
         const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
         T dm = fma(ct, fma(qc2, ct, qs2 * st), q0m);
         dm = dm * dm;
-        const T sdsp = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk));  // |e|: see k_rates
+        // sds before cothkd; |e|: see k_rates
+        T sdsp = do_plain ? dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk)) : T(0);
+        [[maybe_unused]] T sdss = 0;
+        if constexpr (SUR) {
+          if (do_sur) {
+            sdss = fmax(ss - sbfv - rc[q], T(0));
+            sdsp = use ? sdss * icoth : sdsp;
+          }
+        }
         const T X = sdsp * e;
         T G = 0;
         if constexpr (decltype(PROG)::value) {
@@ -818,7 +1008,13 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
           // diagnostic tail, src/umwm_physics.F90:70-80
           const T num = fmax(ss, T(0)) - ctail;
           const T t = fmax(num, T(0)) * (icfc * Tr<T>::rcp(dm));
-          const T es_ = Tr<T>::ex(fma(ipow, Tr<T>::lg(t), -lk4));  // k^-4 t^(1/p)
+          T es_ = Tr<T>::ex(fma(ipow, Tr<T>::lg(t), -lk4));  // k^-4 t^(1/p)
+          if constexpr (SUR) {
+            if (do_sur && use) {
+              const T lq = Tr<T>::lg(fmax(num, T(0))) - Tr<T>::lg(sdss);
+              es_ = sdss > T(0) ? e * Tr<T>::ex(ipow * lq) : T(0);
+            }
+          }
           bst(FULL || q < nd ? rs0 : rz, vtail, T(0.5) * (e + es_), so);
         }
         const T h1 = fma(hdb1, X, H1[q]);  // bf1r(o-1) X(o) into bin o-1
@@ -849,6 +1045,12 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     b1m1 = tc[FT_B1];
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) ec[q] = en[q];
+    if constexpr (SUR) {
+      if (do_sur) {
+#pragma unroll
+        for (int q = 0; q < DMAX; ++q) rc[q] = rn[q];
+      }
+    }
     if constexpr (kDerive) {
       tg.advance(o, knext);
       pc = pn;
@@ -1138,26 +1340,41 @@ __global__ __launch_bounds__(64) void k_stress(const DomainDev<T>* __restrict__ 
   pstore(d.hs_ice, ldi, j, 4 * sqrt(hs * d.dth));
 }
 
-template <typename T, int D1, int D2>
-int launch_all(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int omax, int first,
-               int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev) {
-  const dim3 grid(ntasks);
-  const size_t sh1 = tile.W1 > 1 ? (size_t)2 * 3 * tile.W1 * 64 * sizeof(double)
-                                 : (size_t)64 * sizeof(double);
+template <typename T, int D1, int D2, bool SUR>
+void launch_sweeps(const DomainDev<T>* doms, const BlockTask* tasks, dim3 grid, size_t sh1,
+                   int first, int rotation, Tile tile, hipStream_t s, hipEvent_t* ev) {
   // a launch whose waves all hold DMAX directions (W D = pm) takes the
   // variant with no per-slot masks
-  if (ev) (void)hipEventRecord(ev[0], s);
   if (tile.W1 * D1 == tile.pmax)
-    hipLaunchKernelGGL((k_rates<T, D1, true>), grid, dim3(64 * tile.W1), sh1, s, doms, tasks,
+    hipLaunchKernelGGL((k_rates<T, D1, true, SUR>), grid, dim3(64 * tile.W1), sh1, s, doms, tasks,
                        first, rotation);
   else
-    hipLaunchKernelGGL((k_rates<T, D1, false>), grid, dim3(64 * tile.W1), sh1, s, doms, tasks,
+    hipLaunchKernelGGL((k_rates<T, D1, false, SUR>), grid, dim3(64 * tile.W1), sh1, s, doms, tasks,
                        first, rotation);
   if (ev) (void)hipEventRecord(ev[1], s);
   if (tile.W2 * D2 == tile.pmax)
-    hipLaunchKernelGGL((k_integrate<T, D2, true>), grid, dim3(64 * tile.W2), 0, s, doms, tasks);
+    hipLaunchKernelGGL((k_integrate<T, D2, true, SUR>), grid, dim3(64 * tile.W2), 0, s, doms, tasks);
   else
-    hipLaunchKernelGGL((k_integrate<T, D2, false>), grid, dim3(64 * tile.W2), 0, s, doms, tasks);
+    hipLaunchKernelGGL((k_integrate<T, D2, false, SUR>), grid, dim3(64 * tile.W2), 0, s, doms,
+                       tasks);
+}
+
+template <typename T, int D1, int D2>
+int launch_all(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int omax, int first,
+               int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev, bool sur) {
+  const dim3 grid(ntasks);
+  // the moments' double buffer, and the surrogate's four cross-wave sums
+  const size_t sh1 = (size_t)std::max(2 * 3 * tile.W1, 4 * tile.W1) * 64 * sizeof(double);
+  if (ev) (void)hipEventRecord(ev[0], s);
+  if constexpr (std::is_same_v<T, float>) {
+    if (sur)
+      launch_sweeps<T, D1, D2, true>(doms, tasks, grid, sh1, first, rotation, tile, s, ev);
+    else
+      launch_sweeps<T, D1, D2, false>(doms, tasks, grid, sh1, first, rotation, tile, s, ev);
+  } else {
+    if (sur) return fail(UMWM_ERR_ARGUMENT, "the surrogate serves the FP32 path only");
+    launch_sweeps<T, D1, D2, false>(doms, tasks, grid, sh1, first, rotation, tile, s, ev);
+  }
   if (keep) hipLaunchKernelGGL((k_keep<T>), grid, dim3(64), 0, s, doms, tasks);
   if (ev) (void)hipEventRecord(ev[2], s);
   // advect: one wave per (64 points, frequency), FW frequencies per
@@ -1178,10 +1395,10 @@ int launch_all(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int
 
 template <typename T, int D1>
 int launch_d2(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int omax, int first,
-              int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev) {
+              int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev, bool sur) {
 #define UMWM_D2(D2_)                                                                         \
   case D2_:                                                                                  \
-    return launch_all<T, D1, D2_>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev);
+    return launch_all<T, D1, D2_>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev, sur);
   switch (tile.D2) {
     UMWM_D2(3)
     UMWM_D2(6)
@@ -1199,22 +1416,23 @@ bool tile_supported(int D) { return D == 3 || D == 6 || D == 9 || D == 12 || D =
 
 template <typename T>
 int launch_fused(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, int omax,
-                 int first, int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev) {
+                 int first, int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev,
+                 bool sur) {
   if (ntasks <= 0) return 0;
   if (tile.W1 < 1 || tile.W1 > 8 || tile.W2 < 1 || tile.W2 > 8)
     return fail(UMWM_ERR_ARGUMENT, "tile W must be 1..8");
   switch (tile.D1) {
-    case 6: return launch_d2<T, 6>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev);
-    case 9: return launch_d2<T, 9>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev);
-    case 18: return launch_d2<T, 18>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev);
-    case 36: return launch_d2<T, 36>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev);
+    case 6: return launch_d2<T, 6>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev, sur);
+    case 9: return launch_d2<T, 9>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev, sur);
+    case 18: return launch_d2<T, 18>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev, sur);
+    case 36: return launch_d2<T, 36>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev, sur);
   }
   return fail(UMWM_ERR_ARGUMENT, "unsupported rates tile D");
 }
 
 template int launch_fused<float>(const DomainDev<float>*, const BlockTask*, int, int, int, int,
-                                 Tile, bool, hipStream_t, hipEvent_t*);
+                                 Tile, bool, hipStream_t, hipEvent_t*, bool);
 template int launch_fused<double>(const DomainDev<double>*, const BlockTask*, int, int, int, int,
-                                  Tile, bool, hipStream_t, hipEvent_t*);
+                                  Tile, bool, hipStream_t, hipEvent_t*, bool);
 
 }  // namespace umwm

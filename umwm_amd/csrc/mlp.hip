@@ -1,0 +1,470 @@
+// BF16 MFMA kernels of the surrogate network (mlp.hpp): one tiled GEMM
+// with operand-layout and epilogue variants, the loss, the gradient
+// reductions and the Adam step.
+//
+// GEMM tiling: a workgroup of 4 waves computes a 128x128 block of C; wave
+// (wm, wn) owns the 64x64 quarter (wm, wn) as 2x2 tiles of the 32x32x16
+// bf16 MFMA (v_mfma_f32_32x32x16_bf16, FP32 accumulators).  The K loop
+// steps by 32: each thread stages two 16-byte chunks of A and of B in
+// registers while the waves multiply the tile held in LDS, then writes
+// them to the other LDS buffer (one barrier per step).  In LDS both tiles
+// are k-contiguous rows ([m][k], [n][k], padded by 8 to spread banks), so
+// every MFMA operand is one 16-byte read; operands stored m- or
+// n-contiguous in memory (the weight-gradient products over the batch)
+// are transposed on their way into LDS.
+//
+// MFMA operand layout (32x32x16): lane l supplies A(m = l%32, k = 8(l/32)
+// + j) and B(k = 8(l/32) + j, n = l%32), j = 0..7; accumulator register i
+// of lane l holds C(m = 8(i/4) + 4(l/32) + i%4, n = l%32).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <random>
+#include <string>
+
+#include "common.hpp"
+#include "mlp.hpp"
+
+namespace umwm {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;
+
+template <int AL, int BL, int EP>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16 As[2][BM][LDK];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[2][BN][LDK];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int kb = blockIdx.z * g.kchunk;
+  const int ke = min(g.K, kb + g.kchunk);
+  const bf16x8 zero = {};
+  bf16x8 ra[2], rb[2];
+  // Every index below stays inside the operand: rows (m, n) are checked
+  // against M and N, k against the chunk end; a 16-byte chunk never
+  // straddles the end because K, and M or N on their contiguous axis, are
+  // multiples of 8 (checked by gemm()).
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int q = tid + c * 256;
+      if constexpr (AL == A_MK) {
+        const int r = q >> 2, kc = (q & 3) * 8, m = m0 + r, k = k0 + kc;
+        ra[c] = (m < g.M && k < ke) ? *(const bf16x8*)(g.A + (int64_t)m * g.lda + k) : zero;
+      } else {
+        const int kr = q >> 4, mc = (q & 15) * 8, m = m0 + mc, k = k0 + kr;
+        ra[c] = (k < ke && m < g.M) ? *(const bf16x8*)(g.A + (int64_t)k * g.lda + m) : zero;
+      }
+      if constexpr (BL == B_NK) {
+        const int r = q >> 2, kc = (q & 3) * 8, n = n0 + r, k = k0 + kc;
+        rb[c] = (n < g.N && k < ke) ? *(const bf16x8*)(g.B + (int64_t)n * g.ldb + k) : zero;
+      } else {
+        const int kr = q >> 4, nc = (q & 15) * 8, n = n0 + nc, k = k0 + kr;
+        rb[c] = (k < ke && n < g.N) ? *(const bf16x8*)(g.B + (int64_t)k * g.ldb + n) : zero;
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int q = tid + c * 256;
+      if constexpr (AL == A_MK) {
+        *(bf16x8*)&As[buf][q >> 2][(q & 3) * 8] = ra[c];
+      } else {
+        const int kr = q >> 4, mc = (q & 15) * 8;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) As[buf][mc + i][kr] = ra[c][i];
+      }
+      if constexpr (BL == B_NK) {
+        *(bf16x8*)&Bs[buf][q >> 2][(q & 3) * 8] = rb[c];
+      } else {
+        const int kr = q >> 4, nc = (q & 15) * 8;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Bs[buf][nc + i][kr] = rb[c][i];
+      }
+    }
+  };
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.f;
+  int buf = 0;
+  if (kb < ke) {
+    gload(kb);
+    sstore(0);
+  }
+  __syncthreads();
+  for (int k0 = kb; k0 < ke; k0 += BK) {
+    const bool more = k0 + BK < ke;
+    if (more) gload(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int kk = ks * 16 + (lane >> 5) * 8;
+      bf16x8 fa[2], fb[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        fa[t] = *(const bf16x8*)&As[buf][wm * 64 + t * 32 + (lane & 31)][kk];
+        fb[t] = *(const bf16x8*)&Bs[buf][wn * 64 + t * 32 + (lane & 31)][kk];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    if (more) sstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  // epilogue
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int m = m0 + wm * 64 + a * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        const int n = n0 + wn * 64 + b * 32 + (lane & 31);
+        if (m >= g.M || n >= g.N) continue;
+        const float v = acc[a][b][i];
+        const int64_t o = (int64_t)m * g.ldc + n;
+        if constexpr (EP == EP_GELU) {
+          const float z = v + g.bias[n];
+          if (g.Zout) g.Zout[o] = (bf16)z;
+          ((bf16*)g.C)[o] = (bf16)gelu_f(z);
+        } else if constexpr (EP == EP_F32) {
+          ((float*)g.C)[o] = v + (g.bias ? g.bias[n] : 0.f);
+        } else if constexpr (EP == EP_BF16_BM) {
+          ((bf16*)g.C)[o] = (bf16)(v + g.bias[m]);
+        } else if constexpr (EP == EP_DGELU) {
+          ((bf16*)g.C)[o] = (bf16)(v * gelu_grad_f((float)g.Zin[(int64_t)m * g.ldz + n]));
+        } else if constexpr (EP == EP_SUR) {
+          ((bf16*)g.C)[o] = (bf16)(g.oc[m] * sinhf(g.osd[m] * (v + g.bias[m]) + g.omu[m]));
+        } else {
+          ((float*)g.C)[(int64_t)blockIdx.z * g.M * g.ldc + o] = v;
+        }
+      }
+}
+
+// gw[i] = sum over s of part[s][i] (deterministic split-K reduction).
+__global__ void k_reduce_parts(const float* __restrict__ part, int splits, int64_t n,
+                               float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int k = 0; k < splits; ++k) s += part[(int64_t)k * n + i];
+  out[i] = s;
+}
+
+// gb[n] = sum over rows of d[r][n] (bf16 [rows][ld]).
+__global__ void k_colsum(const bf16* __restrict__ d, int rows, int cols, int64_t ld,
+                         float* __restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; ++r) s += (float)d[(int64_t)r * ld + n];
+  out[n] = s;
+}
+
+// Mean squared error over the real outputs with per-output weights.
+__global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t,
+                       const float* __restrict__ wout, int rows, int nout, int ld,
+                       bf16* __restrict__ dy, double* __restrict__ loss) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)rows * ld;
+  double l = 0;
+  if (i < n) {
+    const int c = (int)(i % ld);
+    float g = 0.f;
+    if (c < nout) {
+      const float wo = wout ? wout[c] : 1.f;
+      const float d = y[i] - t[i];
+      const float inv = 1.f / ((float)rows * (float)nout);
+      g = 2.f * wo * d * inv;
+      l = (double)wo * d * d * inv;
+    }
+    dy[i] = (bf16)g;
+  }
+  // wave sum, one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
+  if ((threadIdx.x & 63) == 0 && l != 0) atomicAdd(loss, l);
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ gr, float* __restrict__ m,
+                       float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                       float wd, float c1, float c2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = gr[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] = p[i] * (1.f - lr * wd) - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ a, bf16* __restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (bf16)a[i];
+}
+
+inline dim3 grid_for(int64_t n, int t = 256) { return dim3((unsigned)((n + t - 1) / t)); }
+
+template <int AL, int BL>
+int gemm_ep(int ep, const GemmArgs& g, dim3 grid, hipStream_t s) {
+  switch (ep) {
+    case EP_GELU: hipLaunchKernelGGL((k_gemm<AL, BL, EP_GELU>), grid, dim3(256), 0, s, g); break;
+    case EP_F32: hipLaunchKernelGGL((k_gemm<AL, BL, EP_F32>), grid, dim3(256), 0, s, g); break;
+    case EP_BF16_BM: hipLaunchKernelGGL((k_gemm<AL, BL, EP_BF16_BM>), grid, dim3(256), 0, s, g); break;
+    case EP_DGELU: hipLaunchKernelGGL((k_gemm<AL, BL, EP_DGELU>), grid, dim3(256), 0, s, g); break;
+    case EP_PART: hipLaunchKernelGGL((k_gemm<AL, BL, EP_PART>), grid, dim3(256), 0, s, g); break;
+    case EP_SUR: hipLaunchKernelGGL((k_gemm<AL, BL, EP_SUR>), grid, dim3(256), 0, s, g); break;
+    default: return fail(UMWM_ERR_ARGUMENT, "gemm: unknown epilogue");
+  }
+  return 0;
+}
+
+}  // namespace
+
+int gemm(int alay, int blay, int ep, const GemmArgs& g0, int splits, hipStream_t s) {
+  GemmArgs g = g0;
+  if (g.M <= 0 || g.N <= 0 || g.K <= 0) return 0;
+  // 16-byte chunks: K and the contiguous axis of transposed operands must
+  // be multiples of 8, and so must the leading dimensions
+  if (g.K % 8 || g.lda % 8 || g.ldb % 8 || (alay == A_KM && g.M % 8) || (blay == B_KN && g.N % 8))
+    return fail(UMWM_ERR_ARGUMENT, "gemm: K, lda, ldb (and M or N when transposed) must be "
+                                   "multiples of 8");
+  splits = std::max(1, splits);
+  int kchunk = (g.K + splits - 1) / splits;
+  kchunk = (kchunk + BK - 1) / BK * BK;
+  splits = (g.K + kchunk - 1) / kchunk;
+  g.kchunk = kchunk;
+  if (ep != EP_PART && splits != 1) return fail(UMWM_ERR_ARGUMENT, "gemm: split-K needs EP_PART");
+  const dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, splits);
+  int rc;
+  if (alay == A_MK && blay == B_NK) rc = gemm_ep<A_MK, B_NK>(ep, g, grid, s);
+  else if (alay == A_MK && blay == B_KN) rc = gemm_ep<A_MK, B_KN>(ep, g, grid, s);
+  else if (alay == A_KM && blay == B_KN) rc = gemm_ep<A_KM, B_KN>(ep, g, grid, s);
+  else rc = gemm_ep<A_KM, B_NK>(ep, g, grid, s);
+  if (rc) return rc;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(UMWM_ERR_HIP, std::string("gemm: ") + hipGetErrorString(e));
+  return 0;
+}
+
+Mlp::~Mlp() {
+  for (void* p : allocs) (void)hipFree(p);
+}
+
+namespace {
+template <typename U>
+U* dalloc(Mlp& m, size_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(U)) != hipSuccess) return nullptr;
+  m.allocs.push_back(p);
+  return static_cast<U*>(p);
+}
+inline int pad8(int v) { return (v + 7) / 8 * 8; }
+}  // namespace
+
+int mlp_create(const std::vector<int>& dims, uint64_t seed, int rows, Mlp& m) {
+  const int nl = (int)dims.size() - 1;
+  if (nl < 1 || rows < 1) return fail(UMWM_ERR_ARGUMENT, "mlp: need >= 1 layer and >= 1 row");
+  m.nin = dims.front();
+  m.nout = dims.back();
+  m.cap_rows = rows;
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
+  size_t part = 0;
+  for (int l = 0; l < nl; ++l) {
+    MlpLayerDev L{};
+    L.K = pad8(dims[l]);
+    L.N = pad8(dims[l + 1]);
+    const size_t nw = (size_t)L.N * L.K;
+    L.w = dalloc<float>(m, nw); L.b = dalloc<float>(m, L.N); L.wb = dalloc<bf16>(m, nw);
+    L.gw = dalloc<float>(m, nw); L.gb = dalloc<float>(m, L.N);
+    L.mw = dalloc<float>(m, nw); L.vw = dalloc<float>(m, nw);
+    L.mb = dalloc<float>(m, L.N); L.vb = dalloc<float>(m, L.N);
+    if (!L.vb) return fail(UMWM_ERR_NOMEM, "mlp: device allocation failed");
+    // He-uniform for the GELU layers, Glorot-uniform for the linear output;
+    // padding rows and columns zero
+    const double fan_in = dims[l], fan_out = dims[l + 1];
+    const double a = l + 1 < nl ? std::sqrt(6.0 / fan_in) : std::sqrt(6.0 / (fan_in + fan_out));
+    std::vector<float> hw(nw, 0.f), zero(nw, 0.f), zb(L.N, 0.f);
+    for (int n = 0; n < dims[l + 1]; ++n)
+      for (int k = 0; k < dims[l]; ++k) hw[(size_t)n * L.K + k] = (float)(a * U(rng));
+    UMWM_HIP_TRY(hipMemcpy(L.w, hw.data(), nw * 4, hipMemcpyHostToDevice));
+    for (float* p : {L.gw, L.mw, L.vw}) UMWM_HIP_TRY(hipMemcpy(p, zero.data(), nw * 4, hipMemcpyHostToDevice));
+    for (float* p : {L.b, L.gb, L.mb, L.vb}) UMWM_HIP_TRY(hipMemcpy(p, zb.data(), L.N * 4, hipMemcpyHostToDevice));
+    m.L.push_back(L);
+    part = std::max(part, nw);
+  }
+  // activation buffers
+  m.x = dalloc<bf16>(m, (size_t)rows * m.L[0].K);
+  int wmax = m.L[0].K;
+  for (int l = 0; l + 1 < nl; ++l) {
+    m.z.push_back(dalloc<bf16>(m, (size_t)rows * m.L[l].N));
+    m.h.push_back(dalloc<bf16>(m, (size_t)rows * m.L[l].N));
+    wmax = std::max(wmax, m.L[l].N);
+  }
+  wmax = std::max(wmax, m.L.back().N);
+  m.y = dalloc<float>(m, (size_t)rows * m.L.back().N);
+  m.dy = dalloc<bf16>(m, (size_t)rows * m.L.back().N);
+  m.da = dalloc<bf16>(m, (size_t)rows * wmax);
+  m.db = dalloc<bf16>(m, (size_t)rows * wmax);
+  // split-K partials: at most 32 splits of the largest weight matrix
+  m.part_elems = part * 32;
+  m.part = dalloc<float>(m, m.part_elems);
+  m.loss_acc = dalloc<double>(m, 1);
+  if (!m.loss_acc || !m.part || !m.db) return fail(UMWM_ERR_NOMEM, "mlp: device allocation failed");
+  if (int rc = mlp_sync_bf16(m, nullptr)) return rc;
+  UMWM_HIP_TRY(hipDeviceSynchronize());
+  return 0;
+}
+
+int mlp_sync_bf16(Mlp& m, hipStream_t s) {
+  for (auto& L : m.L) {
+    const int64_t nw = (int64_t)L.N * L.K;
+    hipLaunchKernelGGL(k_to_bf16, grid_for(nw), dim3(256), 0, s, (const float*)L.w, L.wb, nw);
+  }
+  UMWM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int mlp_forward(Mlp& m, int rows, hipStream_t s) {
+  if (rows > m.cap_rows) return fail(UMWM_ERR_SIZE, "mlp: more rows than the buffers hold");
+  const int nl = (int)m.L.size();
+  const bf16* in = m.x;
+  for (int l = 0; l < nl; ++l) {
+    const MlpLayerDev& L = m.L[l];
+    GemmArgs g{};
+    g.A = in; g.lda = L.K;
+    g.B = L.wb; g.ldb = L.K;
+    g.M = rows; g.N = L.N; g.K = L.K;
+    g.bias = L.b;
+    g.ldc = L.N;
+    int rc;
+    if (l + 1 < nl) {
+      g.C = m.h[l]; g.Zout = m.z[l];
+      rc = gemm(A_MK, B_NK, EP_GELU, g, 1, s);
+      in = m.h[l];
+    } else {
+      g.C = m.y;
+      rc = gemm(A_MK, B_NK, EP_F32, g, 1, s);
+    }
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int mlp_loss(Mlp& m, const float* t, const float* wout, int rows, hipStream_t s) {
+  const int ld = m.L.back().N;
+  const int64_t n = (int64_t)rows * ld;
+  hipLaunchKernelGGL(k_loss, grid_for(n), dim3(256), 0, s, (const float*)m.y, t, wout, rows,
+                     m.nout, ld, m.dy, m.loss_acc);
+  UMWM_HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int mlp_backward(Mlp& m, int rows, hipStream_t s) {
+  const int nl = (int)m.L.size();
+  const bf16* dz = m.dy;
+  bf16* bufs[2] = {m.da, m.db};
+  int pb = 0;
+  for (int l = nl - 1; l >= 0; --l) {
+    const MlpLayerDev& L = m.L[l];
+    const bf16* a = l == 0 ? m.x : m.h[l - 1];
+    // weight gradient over the batch, split over rows so that the small
+    // output (N x K) still fills the GPU
+    const int tiles = ((L.N + BM - 1) / BM) * ((L.K + BN - 1) / BN);
+    int splits = std::max(1, std::min(rows / 256, (1024 + tiles - 1) / tiles));
+    splits = std::min(splits, 32);
+    GemmArgs g{};
+    g.A = dz; g.lda = L.N;       // A(m = out, k = row) = dz[row][out]
+    g.B = a; g.ldb = L.K;        // B(k = row, n = in) = a[row][in]
+    g.M = L.N; g.N = L.K; g.K = rows;
+    g.C = m.part; g.ldc = L.K;
+    if ((size_t)splits * L.N * L.K > m.part_elems) splits = (int)(m.part_elems / ((size_t)L.N * L.K));
+    // gemm() may lower the split count to whole chunks of 32 rows
+    int kchunk = (rows + splits - 1) / splits;
+    kchunk = (kchunk + BK - 1) / BK * BK;
+    const int used = (rows + kchunk - 1) / kchunk;
+    if (rows % 8) return fail(UMWM_ERR_ARGUMENT, "mlp: training rows must be a multiple of 8");
+    if (int rc = gemm(A_KM, B_KN, EP_PART, g, splits, s)) return rc;
+    const int64_t nw = (int64_t)L.N * L.K;
+    hipLaunchKernelGGL(k_reduce_parts, grid_for(nw), dim3(256), 0, s, (const float*)m.part, used,
+                       nw, L.gw);
+    hipLaunchKernelGGL(k_colsum, grid_for(L.N), dim3(256), 0, s, dz, rows, L.N, (int64_t)L.N, L.gb);
+    UMWM_HIP_TRY(hipGetLastError());
+    if (l > 0) {
+      // d a_{l-1} = dz W_l, times gelu'(z_{l-1})
+      GemmArgs d{};
+      d.A = dz; d.lda = L.N;      // A(m = row, k = out)
+      d.B = L.wb; d.ldb = L.K;    // B(k = out, n = in) = W[out][in]
+      d.M = rows; d.N = L.K; d.K = L.N;
+      d.C = bufs[pb]; d.ldc = L.K;
+      d.Zin = m.z[l - 1]; d.ldz = L.K;
+      if (int rc = gemm(A_MK, B_KN, EP_DGELU, d, 1, s)) return rc;
+      dz = bufs[pb];
+      pb ^= 1;
+    }
+  }
+  return 0;
+}
+
+int mlp_adam(Mlp& m, float lr, float b1, float b2, float eps, float wd, hipStream_t s) {
+  m.step += 1;
+  const float c1 = 1.f / (1.f - std::pow(b1, (float)m.step));
+  const float c2 = 1.f / (1.f - std::pow(b2, (float)m.step));
+  for (auto& L : m.L) {
+    const int64_t nw = (int64_t)L.N * L.K;
+    hipLaunchKernelGGL(k_adam, grid_for(nw), dim3(256), 0, s, L.w, (const float*)L.gw, L.mw, L.vw,
+                       nw, lr, b1, b2, eps, wd, c1, c2);
+    hipLaunchKernelGGL(k_adam, grid_for(L.N), dim3(256), 0, s, L.b, (const float*)L.gb, L.mb, L.vb,
+                       (int64_t)L.N, lr, b1, b2, eps, 0.f, c1, c2);
+  }
+  UMWM_HIP_TRY(hipGetLastError());
+  return mlp_sync_bf16(m, s);
+}
+
+int mlp_infer_transposed(Mlp& m, int rows, bf16* out, int64_t ldo, hipStream_t s,
+                         const float* omu, const float* osd, const float* oc) {
+  if (rows > m.cap_rows) return fail(UMWM_ERR_SIZE, "mlp: more rows than the buffers hold");
+  const int nl = (int)m.L.size();
+  const bf16* in = m.x;
+  for (int l = 0; l + 1 < nl; ++l) {
+    const MlpLayerDev& L = m.L[l];
+    GemmArgs g{};
+    g.A = in; g.lda = L.K;
+    g.B = L.wb; g.ldb = L.K;
+    g.M = rows; g.N = L.N; g.K = L.K;
+    g.bias = L.b;
+    g.C = m.h[l]; g.ldc = L.N;
+    if (int rc = gemm(A_MK, B_NK, EP_GELU, g, 1, s)) return rc;
+    in = m.h[l];
+  }
+  // out[o][j] = sum_k W[o][k] h[j][k] + b[o]: the network's outputs of point
+  // j along a contiguous row per output (the spectra's [bin][point] layout)
+  const MlpLayerDev& L = m.L.back();
+  GemmArgs g{};
+  g.A = L.wb; g.lda = L.K;
+  g.B = in; g.ldb = L.K;
+  g.M = m.nout; g.N = rows; g.K = L.K;
+  g.bias = L.b;
+  g.C = out; g.ldc = ldo;
+  if (omu && osd && oc) {
+    g.omu = omu; g.osd = osd; g.oc = oc;
+    return gemm(A_MK, B_NK, EP_SUR, g, 1, s);
+  }
+  return gemm(A_MK, B_NK, EP_BF16_BM, g, 1, s);
+}
+
+}  // namespace umwm

@@ -554,47 +554,59 @@ __global__ void k_stress_ocn(DomainDev<T> d, OcnDev out, double bf1, double bf2)
 // ---------------------------------------------------------------------
 // src/umwm_physics.F90:89-205 diag
 // ---------------------------------------------------------------------
+// One pass over the point's spectrum, frequency by frequency: the tables of
+// a frequency are read once and the directions of its spectrum rows (one
+// coalesced row per (o, p) across the lanes) are summed in FP64.  The
+// dominant bin is the reference's maxloc over e kdk (o fastest,
+// src/umwm_physics.F90:157-160): of equal maxima the one first in (p, o)
+// order.
 template <typename T>
 __global__ void k_diag(DomainDev<T> d, DiagDev g) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= d.im) return;
   const size_t ldi = d.ldi;
   const int om = d.om, pm = d.pm, oc = d.oc[j];
-  double m0 = 0, m2 = 0, mk = 0, mx = 0, my = 0, gxx = 0, gxy = 0, gyy = 0;
+  double m0 = 0, m2 = 0, mk = 0, mx = 0, my = 0, gxx = 0, gxy = 0, gyy = 0, xc = 0, yc = 0;
   double best = -INFINITY;
   int opk = 0, ppk = 0;
-  for (int p = 0; p < pm; ++p) {
-    const double c = d.cth[p], s = d.sth[p];
-    for (int o = 0; o < om; ++o) {
-      const size_t t = (size_t)o * ldi + j;
-      const double e = d.e[((size_t)o * pm + p) * ldi + j];
-      const double kdk = d.kdk[t], f = d.f[o];
+  for (int o = 0; o < om; ++o) {
+    const size_t t = (size_t)o * ldi + j;
+    const double kdk = d.kdk[t], f = d.f[o], k3dk = d.k3dk[t];
+    const double icp = d.invcp0[t], cg = d.cg0[t];
+    const bool prog = o <= oc;
+    const T* eo = d.e + (size_t)o * pm * ldi + j;
+    double s0 = 0, sk = 0, sx = 0, sy = 0, sxx = 0, sxy = 0, syy = 0;
+    for (int p = 0; p < pm; ++p) {
+      const double c = d.cth[p], s = d.sth[p];
+      const double e = eo[(size_t)p * ldi];
       const double ek = e * kdk;
-      m0 += ek;
-      m2 += (f * f) * e * kdk;
-      mk += e * (double)d.k3dk[t];
-      if (o <= oc) {
-        const double ekc = ek * (double)d.invcp0[t], cg = d.cg0[t];
-        mx += ekc * c;
-        my += ekc * s;
-        gxx += cg * ekc * (c * c);
-        gxy += cg * ekc * c * s;
-        gyy += cg * ekc * (s * s);
+      s0 += ek;
+      sk += e * k3dk;
+      sx += ek * c;
+      sy += ek * s;
+      if (prog) {
+        sxx += ek * (c * c);
+        sxy += ek * c * s;
+        syy += ek * (s * s);
       }
-      if (ek > best) {  // maxloc: first maximum, o fastest
+      if (ek > best || (ek == best && (p < ppk || (p == ppk && o < opk)))) {
         best = ek;
         opk = o;
         ppk = p;
       }
     }
-  }
-  double xc = 0, yc = 0;
-  for (int p = 0; p < pm; ++p) {
-    double mag = 0;
-    for (int o = 0; o < om; ++o)
-      mag += (double)d.e[((size_t)o * pm + p) * ldi + j] * (double)d.kdk[(size_t)o * ldi + j];
-    xc += mag * (double)d.cth[p];
-    yc += mag * (double)d.sth[p];
+    m0 += s0;
+    m2 += (f * f) * s0;
+    mk += sk;
+    xc += sx;
+    yc += sy;
+    if (prog) {
+      mx += sx * icp;
+      my += sy * icp;
+      gxx += cg * icp * sxx;
+      gxy += cg * icp * sxy;
+      gyy += cg * icp * syy;
+    }
   }
   const double tiny = 2.2250738585072014e-308, rw = d.rhow[j];
   g.mwp[j] = sqrt(m0 / (m2 + tiny));
@@ -633,50 +645,59 @@ __global__ void k_hs_ice(DomainDev<T> d) {
 // ---------------------------------------------------------------------
 // src/umwm_stokes.F90:13-132 stokes_drift
 // ---------------------------------------------------------------------
+// us(l), vs(l) = sum over o of util(o,l) sum over p of e cos th, e sin th
+// (src/umwm_stokes.F90:59-105) in one pass over the spectrum: per
+// frequency the point's directional sums are formed once and added into
+// the lm level accumulators it keeps in LDS (in o order at every level),
+// which are written once at the end.
 template <typename T>
-__global__ void k_stokes_dir(DomainDev<T> d, double* sc, double* ss) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= d.im) return;
+__global__ __launch_bounds__(64) void k_stokes(DomainDev<T> d, StokesDev S) {
+  extern __shared__ double acc[];  // [lm][2][points of the block]
+  const int t = threadIdx.x, np = blockDim.x;
+  const int j = blockIdx.x * np + t;
+  const int lm = d.lm;
+  double* au = acc + t;
+  for (int l = 0; l < lm; ++l) {
+    au[(2 * l) * np] = 0;
+    au[(2 * l + 1) * np] = 0;
+  }
+  if (j >= d.im) return;  // no barrier follows: each lane owns its column
+  const size_t ldi = d.ldi;
+  const double dj = d.depth[j];
   for (int o = 0; o < d.om; ++o) {
     double a = 0, b = 0;
+    const T* eo = d.e + (size_t)o * d.pm * ldi + j;
     for (int p = 0; p < d.pm; ++p) {
-      const double e = d.e[((size_t)o * d.pm + p) * d.ldi + j];
+      const double e = eo[(size_t)p * ldi];
       a += e * (double)d.cth[p];
       b += e * (double)d.sth[p];
     }
-    sc[(size_t)o * d.ldi + j] = a;
-    ss[(size_t)o * d.ldi + j] = b;
-  }
-}
-
-template <typename T>
-__global__ void k_stokes_lev(DomainDev<T> d, StokesDev S, const double* sc, const double* ss) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int l = blockIdx.y;
-  if (j >= d.im) return;
-  const double dep = S.depth[l], dj = d.depth[j];
-  double u = 0, v = 0;
-  if (!(fabs(dep) > dj)) {
-    for (int o = 0; o < d.om; ++o) {
-      const size_t t = (size_t)o * d.ldi + j;
-      const double k = d.k[t], kd = k * dj, arg = 2 * k * (dep + dj);
+    const size_t tq = (size_t)o * ldi + j;
+    const double k = d.k[tq], kd = k * dj, dwn = d.dwn[tq];
+    const double fo = kTwoPiD * (double)d.f[o];
+    const double sh = sinh(kd);
+    for (int l = 0; l < lm; ++l) {
+      const double dep = S.depth[l];
+      if (fabs(dep) > dj) continue;  // level below the sea bed: zero drift
+      const double arg = 2 * k * (dep + dj);
       double ut;
       if (fabs(arg) > 50 || kd > 50) {
-        ut = kTwoPiD * (double)d.f[o] * 2 * (k * k) * exp(2 * k * dep) * (double)d.dwn[t] * d.dth;
+        ut = fo * 2 * (k * k) * exp(2 * k * dep) * dwn * d.dth;
       } else {
-        const double sh = sinh(kd);
-        ut = kTwoPiD * (double)d.f[o] * (k * k) * cosh(2 * k * (dep + dj)) / (sh * sh) *
-             (double)d.dwn[t] * d.dth;
+        ut = fo * (k * k) * cosh(2 * k * (dep + dj)) / (sh * sh) * dwn * d.dth;
       }
-      u += ut * sc[t];
-      v += ut * ss[t];
+      au[(2 * l) * np] += ut * a;
+      au[(2 * l + 1) * np] += ut * b;
     }
   }
-  S.us[(size_t)l * d.ldi + j] = u;
-  S.vs[(size_t)l * d.ldi + j] = v;
-  if (l == 0) {
-    d.us1[j] = u;
-    d.vs1[j] = v;
+  for (int l = 0; l < lm; ++l) {
+    const double u = au[(2 * l) * np], v = au[(2 * l + 1) * np];
+    S.us[(size_t)l * ldi + j] = u;
+    S.vs[(size_t)l * ldi + j] = v;
+    if (l == 0) {
+      d.us1[j] = u;
+      d.vs1[j] = v;
+    }
   }
 }
 
@@ -845,12 +866,14 @@ int launch_hs_ice(DomainImpl<T>& D) {
 template <typename T>
 int launch_stokes(DomainImpl<T>& D) {
   if (D.h.lm < 1) return 0;
-  // directional sums reuse the stokes us/vs buffers' tail as scratch
-  double* sc = D.stokes.us + (size_t)D.h.lm * D.h.ldi;
-  double* ss = D.stokes.vs + (size_t)D.h.lm * D.h.ldi;
-  LAUNCH(k_stokes_dir<T>, grid1(D.h.im), dim3(kBlock), D.h, sc, ss);
-  LAUNCH(k_stokes_lev<T>, dim3(grid1(D.h.im).x, D.h.lm), dim3(kBlock), D.h, D.stokes,
-         (const double*)sc, (const double*)ss);
+  // 64 points per block, fewer when the levels' accumulators would pass
+  // 64 KB of LDS (lm > 64)
+  const int np = D.h.lm <= 64 ? 64 : 32;
+  const size_t sh = (size_t)2 * D.h.lm * np * sizeof(double);
+  hipLaunchKernelGGL(k_stokes<T>, dim3((D.h.im + np - 1) / np), dim3(np), sh, D.stream, D.h,
+                     D.stokes);
+  hipError_t e_ = hipGetLastError();
+  if (e_ != hipSuccess) return fail(UMWM_ERR_HIP, std::string("k_stokes: ") + hipGetErrorString(e_));
   LAUNCH(k_stokes_efold<T>, grid1(D.h.im), dim3(kBlock), D.h, D.stokes);
   return 0;
 }

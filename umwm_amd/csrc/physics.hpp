@@ -214,6 +214,10 @@ __device__ inline double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
 }
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 
 // Order-preserving map of doubles (NaN aside) onto unsigned 64-bit keys, so
 // that an integer atomic max combines doubles; ~okey(x) does the same for a

@@ -27,8 +27,13 @@ EXPORTS = [
     "umwm_hip_set_scalar", "umwm_hip_field_size", "umwm_hip_set_timing",
     "umwm_hip_get_timing", "umwm_hip_synchronize", "umwm_hip_batch_create",
     "umwm_hip_batch_destroy", "umwm_hip_batch_run_dtg", "umwm_hip_batch_tile",
+    "umwm_hip_batch_substeps",
     "umwm_hip_set_tile", "umwm_hip_get_tile", "umwm_hip_batch_set_tile",
     "umwm_hip_write_output", "umwm_hip_write_restart", "umwm_hip_read_restart",
+    "umwm_hip_surrogate_create", "umwm_hip_surrogate_destroy", "umwm_hip_surrogate_dims",
+    "umwm_hip_surrogate_get_layer", "umwm_hip_surrogate_set_layer", "umwm_hip_surrogate_forward",
+    "umwm_hip_surrogate_gradients", "umwm_hip_surrogate_collect", "umwm_hip_surrogate_train",
+    "umwm_hip_surrogate_eval", "umwm_hip_surrogate_attach", "umwm_hip_surrogate_stats",
 ]
 
 
@@ -83,10 +88,26 @@ def lib():
     L.umwm_hip_batch_create.argtypes = [P(vp), C.c_int32, P(vp)]
     L.umwm_hip_batch_destroy.argtypes = [vp]
     L.umwm_hip_batch_run_dtg.argtypes = [vp, C.c_int, P(C.c_int32)]
+    L.umwm_hip_batch_substeps.argtypes = [vp, C.c_int32, C.c_int, P(C.c_int32)]
     L.umwm_hip_batch_tile.argtypes = [vp, P(C.c_int32)]
     L.umwm_hip_set_tile.argtypes = [vp, P(C.c_int32)]
     L.umwm_hip_get_tile.argtypes = [vp, P(C.c_int32)]
     L.umwm_hip_batch_set_tile.argtypes = [vp, P(C.c_int32)]
+    fp = P(C.c_float)
+    L.umwm_hip_surrogate_create.argtypes = [P(abi.SurrogateConfig), P(vp)]
+    L.umwm_hip_surrogate_destroy.argtypes = [vp]
+    L.umwm_hip_surrogate_dims.argtypes = [vp, P(C.c_int32)]
+    L.umwm_hip_surrogate_get_layer.argtypes = [vp, C.c_int32, fp, fp]
+    L.umwm_hip_surrogate_set_layer.argtypes = [vp, C.c_int32, fp, fp]
+    L.umwm_hip_surrogate_forward.argtypes = [vp, fp, C.c_int32, fp]
+    L.umwm_hip_surrogate_gradients.argtypes = [vp, fp, fp, C.c_int32, P(C.c_double), P(fp),
+                                               P(fp)]
+    L.umwm_hip_surrogate_collect.argtypes = [vp, vp, P(C.c_int64)]
+    L.umwm_hip_surrogate_train.argtypes = [vp, C.c_int32, C.c_double, C.c_int64, P(C.c_double),
+                                           P(C.c_double)]
+    L.umwm_hip_surrogate_eval.argtypes = [vp, C.c_int64, C.c_int64, P(C.c_double)]
+    L.umwm_hip_surrogate_attach.argtypes = [vp, vp, P(abi.SurrogatePolicy)]
+    L.umwm_hip_surrogate_stats.argtypes = [vp, P(C.c_double), C.c_int]
     _lib = L
     return L
 
@@ -286,8 +307,10 @@ class Domain:
         check(lib().umwm_hip_set_timing(self.h, int(on)))
 
     def timing(self, reset=False):
-        ms = (C.c_double * 4)()
-        cnt = (C.c_int64 * 4)()
+        """Mean-free accumulators: (ms, counts) of rates, integrate, advect,
+        stress and the surrogate's inference."""
+        ms = (C.c_double * 5)()
+        cnt = (C.c_int64 * 5)()
         check(lib().umwm_hip_get_timing(self.h, ms, cnt, int(reset)))
         return list(ms), list(cnt)
 
@@ -321,6 +344,13 @@ class Batch:
         check(lib().umwm_hip_batch_run_dtg(self.h, int(use_graph), n))
         return list(n)
 
+    def substeps(self, n, use_graph=True):
+        """n substeps of every member back to back; returns each member's
+        substeps taken this dtg."""
+        out = (C.c_int32 * len(self.domains))()
+        check(lib().umwm_hip_batch_substeps(self.h, int(n), int(use_graph), out))
+        return list(out)
+
     def tile(self):
         t = (C.c_int32 * 5)()
         check(lib().umwm_hip_batch_tile(self.h, t))
@@ -329,3 +359,106 @@ class Batch:
     def set_tile(self, D1, W1, D2, W2, NB=2):
         t = (C.c_int32 * 5)(int(D1), int(W1), int(D2), int(W2), int(NB))
         check(lib().umwm_hip_batch_set_tile(self.h, t))
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Surrogate:
+    """The learned source-term surrogate (umwm_surrogate): a BF16 MFMA
+    network trained on the device from samples of the plain source
+    kernels."""
+
+    def __init__(self, config):
+        h = C.c_void_p()
+        check(lib().umwm_hip_surrogate_create(C.byref(config), C.byref(h)))
+        self.h = h
+        d = (C.c_int32 * 6)()
+        check(lib().umwm_hip_surrogate_dims(h, d))
+        self.nin, self.nout, self.hidden, self.layers = list(d)[:4]
+        self.widths = [self.nin] + [self.hidden] * self.layers + [self.nout]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().umwm_hip_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        d = (C.c_int32 * 6)()
+        check(lib().umwm_hip_surrogate_dims(self.h, d))
+        return {"samples": d[4], "trained": bool(d[5])}
+
+    def get_layer(self, l):
+        K, N = self.widths[l], self.widths[l + 1]
+        w = np.empty((N, K), np.float32)
+        b = np.empty(N, np.float32)
+        check(lib().umwm_hip_surrogate_get_layer(self.h, l, _fptr(w), _fptr(b)))
+        return w, b
+
+    def set_layer(self, l, w, b):
+        w, b = _f32(w), _f32(b)
+        assert w.shape == (self.widths[l + 1], self.widths[l]) and b.shape == (w.shape[0],)
+        check(lib().umwm_hip_surrogate_set_layer(self.h, l, _fptr(w), _fptr(b)))
+
+    def forward(self, x):
+        x = _f32(x)
+        y = np.empty((x.shape[0], self.nout), np.float32)
+        check(lib().umwm_hip_surrogate_forward(self.h, _fptr(x), x.shape[0], _fptr(y)))
+        return y
+
+    def gradients(self, x, t):
+        x, t = _f32(x), _f32(t)
+        n = x.shape[0]
+        gws = [np.empty((self.widths[l + 1], self.widths[l]), np.float32)
+               for l in range(self.layers + 1)]
+        gbs = [np.empty(self.widths[l + 1], np.float32) for l in range(self.layers + 1)]
+        pw = (C.POINTER(C.c_float) * len(gws))(*[_fptr(g) for g in gws])
+        pb = (C.POINTER(C.c_float) * len(gbs))(*[_fptr(g) for g in gbs])
+        loss = C.c_double()
+        check(lib().umwm_hip_surrogate_gradients(self.h, _fptr(x), _fptr(t), n, C.byref(loss),
+                                                 pw, pb))
+        return loss.value, gws, gbs
+
+    def collect(self, dom):
+        n = C.c_int64()
+        check(lib().umwm_hip_surrogate_collect(self.h, dom.h, C.byref(n)))
+        return n.value
+
+    def train(self, steps, lr=1e-3, ntrain=0):
+        a, b = C.c_double(), C.c_double()
+        check(lib().umwm_hip_surrogate_train(self.h, int(steps), float(lr), int(ntrain),
+                                             C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def eval(self, first, n):
+        v = C.c_double()
+        check(lib().umwm_hip_surrogate_eval(self.h, int(first), int(n), C.byref(v)))
+        return v.value
+
+    def attach(self, dom, policy=None):
+        check(lib().umwm_hip_surrogate_attach(dom.h, self.h,
+                                              C.byref(policy) if policy is not None else None))
+
+    @staticmethod
+    def detach(dom):
+        check(lib().umwm_hip_surrogate_attach(dom.h, None, None))
+
+    @staticmethod
+    def stats(dom, reset=False):
+        out = (C.c_double * 7)()
+        check(lib().umwm_hip_surrogate_stats(dom.h, out, int(reset)))
+        keys = ("points", "used", "fallback_range", "fallback_residual", "fallback_audit",
+                "audited", "audit_error")
+        return dict(zip(keys, list(out)))
+
