@@ -343,10 +343,14 @@ int umwm_hip_batch_set_tile(umwm_batch* b, const int32_t tile[5]);
 /* source terms are closed forms (src/umwm_source_functions.F90:16-105,   */
 /* combined in src/umwm_physics.F90:59-68).  A multilayer perceptron maps */
 /* one sea point's spectrum (ln e of the om x pm bins), wind (u10, v10)   */
-/* and ln depth to the combined rate R = ssin - sds - sbf of every bin    */
-/* (sds after its cothkd factor).  Weights and activations are bfloat16,  */
-/* products accumulate in FP32 on the MFMA units; the network is trained  */
-/* on the device from samples the plain source kernels produce.           */
+/* and ln depth to the one non-local factor of the combined source rate   */
+/* R = ssin - sds - sbf: the directional spreading factor dm^2 = (1 +     */
+/* mss_fac mss)^2 of sds_d12 in every bin, learned as ln dm^2.  The fused */
+/* kernels form R from it and the closed forms of the bin (ssin, sbf and  */
+/* the saturation form cf coth (e k^4)^p of sds).  Weights and            */
+/* activations are bfloat16, products accumulate in FP32 on the MFMA      */
+/* units; the network is trained on the device from samples the plain     */
+/* source kernels produce.                                                */
 /* ---------------------------------------------------------------------- */
 typedef struct umwm_surrogate umwm_surrogate;   /* opaque */
 
@@ -362,14 +366,16 @@ typedef struct umwm_surrogate_config {
 } umwm_surrogate_config;
 
 /* How an attached domain uses the network.  Every audit_stride-th group of
- * 64 sea points (rotating with the substep; 0: none) is also evaluated by
- * the plain kernel.  A point falls back to the plain source terms where its
- * wind speed or depth lies outside the training range widened by
- * range_margin (relative), where a normalised spectral input exceeds zmax
- * standard deviations, where the dissipation the network implies
- * (ssin - sbf - R) is negative in more than resid_tol of the point's
- * energy-weighted rates, or where an audit finds the energy-weighted rate
- * error above audit_tol. */
+ * 64 sea points (rotating with the substep; 0: none) is audited: the
+ * dissipation the network's dm^2 gives is compared with the plain one.  A
+ * point falls back to the plain source terms where its wind speed or depth
+ * lies outside the training range widened by range_margin (relative),
+ * where a normalised spectral input exceeds zmax standard deviations,
+ * where in more than resid_tol of its energy-weighted bins the network's
+ * dm^2 leaves the bounds the moment prefix sets, [1, (1 + 2 mss_fac m0)^2]
+ * widened by a quarter (m0 the zeroth moment below the bin's frequency),
+ * or where an audit finds its energy-weighted error above audit_tol.
+ * Where the network serves, its dm^2 is clamped to those bounds. */
 typedef struct umwm_surrogate_policy {
   int32_t audit_stride;
   int32_t reserved0;
@@ -389,6 +395,9 @@ int umwm_hip_surrogate_set_layer(umwm_surrogate* s, int32_t layer, const float* 
 /* Network outputs y (n x outputs) of n raw input rows x (n x inputs),
  * without normalisation. */
 int umwm_hip_surrogate_forward(umwm_surrogate* s, const float* x, int32_t n, float* y);
+/* The same outputs through the inference kernels a substep uses (the
+ * last layer computed transposed, outputs per row of points, bfloat16). */
+int umwm_hip_surrogate_infer(umwm_surrogate* s, const float* x, int32_t n, float* y);
 /* One forward and backward pass over n (a multiple of 8) raw rows: the
  * mean squared error against t (n x outputs) and the gradients of every
  * layer, gw[l] [N][K] and gb[l] [N] (either may be NULL). */
@@ -403,6 +412,11 @@ int umwm_hip_surrogate_collect(umwm_surrogate* s, umwm_domain* dom, int64_t* tot
  * first and last few steps (normalised units). */
 int umwm_hip_surrogate_train(umwm_surrogate* s, int32_t steps, double lr, int64_t ntrain,
                              double* loss_first, double* loss_last);
+/* The normalisation the last training fixed: input means and inverse     */
+/* standard deviations (inputs values each), output means and standard    */
+/* deviations (outputs values each) of ln dm^2; any pointer may be NULL.  */
+int umwm_hip_surrogate_get_norm(umwm_surrogate* s, float* in_mean, float* in_inv_sd,
+                                float* out_mean, float* out_sd);
 /* Mean loss over collected samples [first, first + n) (held-out rows). */
 int umwm_hip_surrogate_eval(umwm_surrogate* s, int64_t first, int64_t n, double* loss);
 /* Makes an FP32 domain's fused substeps take the network's rates where the
@@ -410,10 +424,17 @@ int umwm_hip_surrogate_eval(umwm_surrogate* s, int64_t first, int64_t n, double*
  * one domain at a time: domains sharing it run one after another. */
 int umwm_hip_surrogate_attach(umwm_domain* dom, umwm_surrogate* s,
                               const umwm_surrogate_policy* policy);
+/* The network's spreading factor dm^2 of the last substep (before the
+ * kernels clamp it to its bounds), (om, pm, im) doubles in the reference's
+ * shape, and each sea point's choice (1: the network's rates) as `use`
+ * (im). */
+int umwm_hip_surrogate_fields(umwm_domain* dom, double* dm_net, double* use);
 /* Statistics since attach or the last reset: out[0] point-substeps,
  * out[1] of which took the network's rates, out[2..4] fell back on range,
  * residual and audit, out[5] audited, out[6] energy-weighted audit error
- * (sum |R - R_plain| e dk / sum (|ssin| + sds + sbf) e dk). */
+ * of the dissipation (sum |sds_net - sds| e dk / sum sds e dk over the
+ * audited points, sds_net = the clamped network dm^2 times the saturation
+ * form). */
 int umwm_hip_surrogate_stats(umwm_domain* dom, double out[7], int reset);
 
 #ifdef __cplusplus

@@ -79,6 +79,10 @@ def test_kernels_match_fp64_restatement(shape):
         _, _, y_ref = mlp_ref.forward(ws, bs, xb)
         y = net.forward(x)
         assert rel(y, y_ref) <= 0.02, rel(y, y_ref)
+        # the inference kernels of a substep: last layer transposed, output
+        # rounded to bfloat16
+        yi = net.infer(x)
+        assert rel(yi, y_ref) <= 0.025, rel(yi, y_ref)
         L, gws, gbs = net.gradients(x, t)
         L_ref, gw_ref, gb_ref = mlp_ref.gradients(ws, bs, xb, t)
         assert abs(L - L_ref) <= 0.02 * L_ref, (L, L_ref)
@@ -93,8 +97,9 @@ def test_kernels_match_fp64_restatement(shape):
 # ---------------------------------------------------------------------------
 # training on the plain kernel's samples and the surrogate path
 # ---------------------------------------------------------------------------
-TRAIN_SEEDS = list(range(1, 31))
-HELDOUT_SEED = 31
+# 117 seeds x 6 snapshots x 2852 sea points = 2.0 M training samples
+TRAIN_SEEDS = list(range(1, 118))
+HELDOUT_SEED = 118
 EVAL_SEED = 1000
 
 
@@ -126,7 +131,7 @@ def harvest(net, seeds, per_seed=6):
 
 @pytest.fixture(scope="module")
 def trained():
-    net = make_net(37, 36, 512, 3, batch=1024, chunk=4096, capacity=600_000, seed=3)
+    net = make_net(37, 36, 512, 3, batch=2048, chunk=4096, capacity=2_020_000, seed=3)
     harvest(net, TRAIN_SEEDS)
     ntrain = net.info()["samples"]
     harvest(net, [HELDOUT_SEED])
@@ -145,10 +150,38 @@ def trained():
 
 def test_training_reduces_the_loss(trained):
     net, info = trained
-    assert info["ntrain"] == 30 * 6 * 2852
+    assert info["ntrain"] == len(TRAIN_SEEDS) * 6 * 2852 >= 2_000_000
     assert info["last"] <= 0.1 * info["first"]
     # the held-out seed is fitted about as well as the training seeds
     assert info["held"] <= 3.0 * info["last"] + 0.01
+
+
+def test_network_dissipation_at_a_fresh_start(trained):
+    """The dissipation the network gives a fresh seed's initial spectrum
+    against the plain sds_d12 evaluation of the same state (energy-weighted
+    relative error), through the substep's inference path: the network's
+    spreading factor dm^2 times the closed form sds / dm^2 of the bin."""
+    net, _ = trained
+    wl = configs.c1_seeded(EVAL_SEED)
+    a, b = fp32_domain(wl), fp32_domain(wl)
+    try:
+        net.attach(a, abi.make_surrogate_policy(audit_stride=1))
+        a.substep()  # cold substep: inference on the initial spectrum
+        dm_net, use = hip.Surrogate.fields(a)
+        for op in ("forcinginterpolate", "sin_d12", "sds_d12", "snl_d12"):
+            getattr(b, op)()
+        sds, dm = b.get("sds"), b.get("dummy")
+        assert np.all(np.isfinite(dm_net))
+        sds_net = sds / dm * dm_net
+        w = b.get("e") * b.get("kdk")[:, None, :]
+        err = float(np.sum(np.abs(sds_net - sds) * w) / np.sum(sds * w))
+        st = hip.Surrogate.stats(a)
+        print(f"surrogate at a fresh start: energy-weighted sds error {err:.4f}, "
+              f"points using it {np.mean(use):.3f}, stats {st}")
+        assert err <= 0.1
+    finally:
+        a.close()
+        b.close()
 
 
 def test_surrogate_path_on_a_fresh_seed(trained):
@@ -176,13 +209,59 @@ def test_surrogate_path_on_a_fresh_seed(trained):
         used = st["used"] / max(st["points"], 1)
         print(f"surrogate path: Hs RMSE {e_sur:.4f} (plain FP32 {e_plain:.2e}), ustar "
               f"{u_sur:.4f}, surrogate used on {used:.3f} of point-substeps, audit error "
-              f"{st['audit_error']:.4f}, time {t_sur:.3f} s vs plain {t_plain:.3f} s")
+              f"{st['audit_error']:.4f}, time {t_sur:.3f} s vs plain {t_plain:.3f} s; {st}")
         assert st["points"] > 0 and used > 0.5
         assert e_sur <= 0.02
         assert u_sur <= 0.03
     finally:
         for m in (o, plain, sur):
             m.close()
+
+
+def test_spectrum_inference_matches_fp64_restatement(trained):
+    """The substep's inference path, whose first layer forms its inputs from
+    the spectrum itself (ln e, normalised, bf16) and whose last layer writes
+    dm^2 - 1 = exp(sd y + mu) - 1 into the [bin][point] array, against the
+    FP64 numpy forward on the same state: inputs built here from e, wind and
+    depth, rounded to bf16, the network's bf16 weights.  The bound is that
+    of the kernel test plus the bf16 rounding of dm^2 - 1 (0.4 %): 2 %
+    normwise on ln dm^2 and on dm^2."""
+    net, _ = trained
+    wl = configs.c1_seeded(EVAL_SEED)
+    a, b = fp32_domain(wl), fp32_domain(wl)
+    try:
+        net.attach(a, abi.make_surrogate_policy())
+        e_a0 = a.get("e")
+        a.substep()  # cold substep: inference on the initial spectrum
+        dm_net, _ = hip.Surrogate.fields(a)
+        b.forcinginterpolate()
+        # the reference sees the state the substep's inference saw
+        assert np.array_equal(e_a0, b.get("e"))
+        im = b.im
+        e = b.get("e")
+        feats = [np.log(np.maximum(e.transpose(0, 2, 1).reshape(im, -1), 1e-30))]
+        w = np.maximum(b.get("wspd")[:im], 1e-2)
+        wd = b.get("wdir")[:im]
+        feats.append(np.stack([w * np.cos(wd), w * np.sin(wd), np.log(b.get("d")[:im])], 1))
+        x = np.concatenate(feats, 1)
+        fmu, fis, tmu, tsd = (v.astype(np.float64) for v in net.get_norm())
+        xb = mlp_ref.bf16_round((x - fmu) * fis)
+        ws, bs = ref_arrays(net)
+        y = mlp_ref.forward(ws, bs, xb)[2][:, :net.nout]
+        ln_ref = np.minimum(tsd * y + tmu, 30.0)
+        # fields are (im, pm, om); outputs are bins o pm + p
+        ln_net = np.log(dm_net.transpose(0, 2, 1).reshape(im, -1))
+        # the same rows through the bf16-input inference kernels
+        ln_inf = np.minimum(tsd * net.infer(((x - fmu) * fis).astype(np.float32)) + tmu, 30.0)
+        for f in ("wspd", "wdir"):
+            assert np.array_equal(a.get(f)[:im], b.get(f)[:im])
+        print(f"spectrum inference: rel(fields, numpy) {rel(ln_net, ln_ref):.4f}, "
+              f"rel(infer, numpy) {rel(ln_inf, ln_ref):.4f}")
+        assert rel(ln_net, ln_ref) <= 0.02, rel(ln_net, ln_ref)
+        assert rel(np.exp(ln_net), np.exp(ln_ref)) <= 0.02
+    finally:
+        a.close()
+        b.close()
 
 
 def test_untrained_network_falls_back_under_full_audit():

@@ -169,6 +169,17 @@ __device__ __forceinline__ float bld_bf16(rsrc_t r, uint32_t voff, uint32_t soff
   return __builtin_bit_cast(float, (uint32_t)h << 16);
 }
 
+// The raw bits of a bfloat16 value, zero-extended, for loads issued ahead
+// of their use: widening at the load would need the data at once, and with
+// few registers free the compiler then waits for every load in flight
+// after each one.  bf16_bits_to_f widens at the use.
+__device__ __forceinline__ uint32_t bld_u16(rsrc_t r, uint32_t voff, uint32_t soff = 0) {
+  return __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ float bf16_bits_to_f(uint32_t h) {
+  return __builtin_bit_cast(float, h << 16);
+}
+
 // A lane offset past every descriptor's range: the hardware range check
 // drops a store made with it (descriptors cover less than 2^31 bytes), so
 // stores can be predicated per lane without a divergent branch.

@@ -186,6 +186,8 @@ void tile_to(const Tile& t, int32_t* a) {
 template <typename T>
 DomainImpl<T>::~DomainImpl() {
   if (stream) (void)hipStreamSynchronize(stream);
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (sc_host) (void)hipHostFree(sc_host);
@@ -664,11 +666,62 @@ int fused_substep(DomainImpl<T>& D, const DomainDev<T>* dd = nullptr) {
   return 0;
 }
 
+// What a captured steady-state substep depends on beyond stable device
+// pointers: the launch shape, the keep and surrogate switches and the
+// surrogate's range settings (launch arguments by value).
+template <typename T>
+uint64_t graph_signature(const DomainImpl<T>& D) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  const Tile& t = D.tile;
+  for (int v : {t.D1, t.W1, t.D2, t.W2, t.NB, t.pmax}) mix((uint64_t)v);
+  mix(D.h.e_keep != nullptr);
+  mix((uint64_t)(uintptr_t)D.sur);
+  mix((uint64_t)(uintptr_t)D.h.sur_r);
+  uint64_t b;
+  std::memcpy(&b, &D.sur_margin, 8);
+  mix(b);
+  std::memcpy(&b, &D.sur_zmax, 8);
+  mix(b);
+  return h | 1;  // never 0, the value of "nothing captured"
+}
+
+// One steady-state substep (not the first, no timing) replayed from a
+// captured hipGraph: forcing interpolation, the surrogate's inference and
+// the fused kernels, the same launches fused_substep makes.
+template <typename T>
+int graph_substep(DomainImpl<T>& D) {
+  const uint64_t sig = graph_signature(D);
+  if (!D.gexec || D.gsig != sig) {
+    if (D.gexec) (void)hipGraphExecDestroy(D.gexec);
+    if (D.graph) (void)hipGraphDestroy(D.graph);
+    D.gexec = nullptr;
+    D.graph = nullptr;
+    UMWM_HIP_TRY(hipStreamBeginCapture(D.stream, hipStreamCaptureModeThreadLocal));
+    const int rc = fused_substep(D);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(D.stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(UMWM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    D.graph = g;
+    UMWM_HIP_TRY(hipGraphInstantiate(&D.gexec, g, nullptr, nullptr, 0));
+    D.gsig = sig;
+  }
+  UMWM_HIP_TRY(hipGraphLaunch(D.gexec, D.stream));
+  return 0;
+}
+
 // n substeps back to back; the scalars are read once, after the last.
+// Steady-state substeps replay the captured graph.
 template <typename T>
 int substeps_impl(DomainImpl<T>& D, int n) {
-  for (int k = 0; k < n; ++k)
-    if (int rc = fused_substep(D)) return rc;
+  for (int k = 0; k < n; ++k) {
+    const bool steady = !D.first && D.rot_valid && !D.timing && D.use_graph;
+    if (int rc = steady ? graph_substep(D) : fused_substep(D)) return rc;
+  }
   return read_scalars(D);
 }
 
@@ -1977,8 +2030,10 @@ int attach_impl(DomainImpl<float>& D, SurrogateImpl* S, const umwm_surrogate_pol
     UMWM_HIP_TRY(hipMemcpy(D.d, &D.h, sizeof(DomainDev<float>), hipMemcpyHostToDevice));
     return 0;
   }
-  if (S->om != D.h.om || S->pm != D.h.pm)
-    return fail(UMWM_ERR_ARGUMENT, "surrogate_attach: the domain's om x pm differs from the network's");
+  if (S->om != D.h.om || S->pm != D.h.pm || D.S.P.fmin != S->fmin || D.S.P.fmax != S->fmax)
+    return fail(UMWM_ERR_ARGUMENT,
+                "surrogate_attach: the domain's spectral grid (om, pm, fmin, fmax) differs from "
+                "the network's");
   if (!S->stats_ready) return fail(UMWM_ERR_STATE, "surrogate_attach: train the network first");
   if (!D.h.sur_r) {
     const size_t nb = (size_t)D.h.om * D.h.pm;
@@ -2036,6 +2091,33 @@ int umwm_hip_surrogate_attach(umwm_domain* dom, umwm_surrogate* s, const umwm_su
   return attach_impl(*static_cast<DomainImpl<float>*>(dom->base), s ? s->impl : nullptr, pol);
 }
 
+int umwm_hip_surrogate_fields(umwm_domain* dom, double* dm_net, double* use) {
+  if (!dom || !dom->base) return fail(UMWM_ERR_ARGUMENT, "null domain");
+  if (dom->base->precision != 4) return fail(UMWM_ERR_STATE, "surrogate_fields: no surrogate");
+  auto& D = *static_cast<DomainImpl<float>*>(dom->base);
+  if (!D.h.sur_r) return fail(UMWM_ERR_STATE, "surrogate_fields: no surrogate attached");
+  UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+  const size_t ldi = D.h.ldi, im = D.h.im, om = D.h.om, pm = D.h.pm;
+  if (dm_net) {
+    std::vector<uint16_t> h(om * pm * ldi);
+    UMWM_HIP_TRY(hipMemcpy(h.data(), D.h.sur_r, h.size() * 2, hipMemcpyDeviceToHost));
+    for (size_t o = 0; o < om; ++o)
+      for (size_t p = 0; p < pm; ++p)
+        for (size_t j = 0; j < im; ++j) {
+          const uint32_t bits = (uint32_t)h[(o * pm + p) * ldi + j] << 16;
+          float v;
+          std::memcpy(&v, &bits, 4);
+          dm_net[j * om * pm + p * om + o] = 1.0 + (double)v;  // stored as dm^2 - 1
+        }
+  }
+  if (use) {
+    std::vector<int32_t> u(im);
+    UMWM_HIP_TRY(hipMemcpy(u.data(), D.h.sur_use, im * 4, hipMemcpyDeviceToHost));
+    for (size_t j = 0; j < im; ++j) use[j] = u[j];
+  }
+  return 0;
+}
+
 int umwm_hip_surrogate_stats(umwm_domain* dom, double out[7], int reset) {
   if (!dom || !dom->base || !out) return fail(UMWM_ERR_ARGUMENT, "null argument");
   if (dom->base->precision != 4) return fail(UMWM_ERR_STATE, "surrogate_stats: no surrogate");
@@ -2044,7 +2126,12 @@ int umwm_hip_surrogate_stats(umwm_domain* dom, double out[7], int reset) {
   double st[SS_N];
   UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
   UMWM_HIP_TRY(hipMemcpy(st, D.h.sur_stats, sizeof(st), hipMemcpyDeviceToHost));
-  for (int k = 0; k < 6; ++k) out[k] = st[k];
+  // words SS_POINTS .. SS_AUDITED hold unsigned 64-bit counts
+  for (int k = 0; k < SS_AUDIT_NUM; ++k) {
+    unsigned long long c;
+    std::memcpy(&c, st + k, sizeof(c));
+    out[k] = (double)c;
+  }
   out[6] = st[SS_AUDIT_DEN] > 0 ? st[SS_AUDIT_NUM] / st[SS_AUDIT_DEN] : 0.0;
   if (reset) {
     hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, D.stream, D.h.sur_stats, (int)SS_N);

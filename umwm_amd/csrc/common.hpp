@@ -113,14 +113,14 @@ struct DomainDev {
   T* spart;
   Scalars* sc;
   // learned surrogate of the FP32 path (null sur_r: none attached): the
-  // network's combined rate R = ssin - sds - sbf per bin as bfloat16 bits,
+  // network's spreading factor per bin as dm^2 - 1 in bfloat16 bits,
   // [o*pm+p][ldi]; per point the input-range verdict (sur_ok) and the
   // choice k_rates made for the current substep (sur_use, read by
   // k_integrate); statistics (SurStat, surrogate.hpp); the policy: every
   // sur_audit_stride-th workgroup of points (rotating with the substep) is
-  // also evaluated by the plain kernel, and a point falls back where its
-  // negative implied dissipation or its audited error exceeds the
-  // tolerances (energy-weighted fractions)
+  // also evaluated by the plain kernel, and a point falls back where the
+  // energy-weighted share of its bins whose dm^2 leaves the prefix bounds,
+  // or its audited dissipation error, exceeds the tolerances
   const uint16_t* sur_r;
   int32_t *sur_ok, *sur_use;
   double* sur_stats;

@@ -87,6 +87,12 @@ struct DomainImpl : DomainBase {
   // attached learned surrogate (borrowed; FP32 domains only)
   SurrogateImpl* sur = nullptr;
   double sur_margin = 0.1, sur_zmax = 8.0;
+  // one steady-state substep captured as a hipGraph (run_dtg and
+  // substeps replay it); `gsig` identifies what was captured
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  uint64_t gsig = 0;
+  bool use_graph = true;
   DiagDev diag{};
   OcnDev ocn{};  // allocated with the operator arrays
   StokesDev stokes{};
@@ -140,8 +146,8 @@ int launch_fused(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, i
                  int first, int rotation, Tile tile, bool keep, hipStream_t s, hipEvent_t* ev,
                  bool sur = false);
 
-// The attached surrogate's rates for the domain's current spectrum and
-// forcing (features, network, R into DomainDev::sur_r), on stream s.
+// The attached surrogate's spreading factors for the domain's current
+// spectrum and forcing (dm^2 - 1 into DomainDev::sur_r), on stream s.
 int launch_surrogate_inference(DomainImpl<float>& D, hipStream_t s);
 
 }  // namespace umwm

@@ -457,37 +457,40 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       if (over) mb |= MB(1) << q;
     }
     // Surrogate (FP32): lanes whose inputs passed the range check take the
-    // network's combined rate R = ssin - sds - sbf, from which the implied
-    // dissipation sds = ssin - sbf - R (clamped at 0) enters the rate as
-    // the plain one would.  Audited workgroups and waves holding lanes that
-    // failed the check also evaluate the plain sds; a lane whose implied
-    // dissipation is negative in more than the residual tolerance of its
-    // energy-weighted rates, or whose audited error exceeds the audit
-    // tolerance, falls back to the plain rates (a second, plain-only sweep
-    // when its wave did not evaluate them).  All waves of a workgroup hold
-    // the same 64 points, so these wave-level choices agree across waves.
+    // network's spreading factor dm^2 in place of the one the moment
+    // prefix gives, so that sds = dm^2 s with s = cf coth (e k^4)^p the
+    // closed form of the bin.  The residual check holds the network's dm^2
+    // against the bounds the prefix sets, [1, (1 + 2 mss_fac m0)^2] (m0 the
+    // frequency prefix of the zeroth moment, which bounds the cos^2-weighted
+    // ones), and the value used is clamped to them; a lane whose
+    // energy-weighted share of bins outside the bounds (widened by
+    // kSurBoundTol) exceeds the residual tolerance, or whose audited
+    // dissipation error exceeds the audit tolerance, falls back to the plain
+    // rates, which these waves always evaluate.  All waves of a workgroup
+    // hold the same 64 points, so these choices agree across waves.
     [[maybe_unused]] int sok = 0;
     [[maybe_unused]] bool aud = false;
     bool do_sur = false;
-    bool do_plain = true;
     [[maybe_unused]] const uint16_t* const rsur = d.sur_r;
     if constexpr (SUR) {
       if (rsur) {
         sok = valid ? pload(d.sur_ok, ldi, j) : 0;
         aud = d.sur_audit_stride > 0 && ((tk.local + nsub0) % d.sur_audit_stride) == 0;
         do_sur = __any(sok != 0);
-        do_plain = aud || __any(valid && sok == 0);
       }
     }
     T rmax = 0;
     [[maybe_unused]] T rmax_s = 0, neg = 0, tot = 0, anum = 0, aden = 0;
-    // One sweep over frequencies: `dp` evaluates the plain dissipation,
-    // `ds` the network's; the first pass (`pass0`) also stores the prefix
-    // moments and the rotation maximum.
-    auto sweep = [&](bool dp, bool ds, bool pass0) {
+    // One sweep over frequencies: `ds` also evaluates the network's
+    // dissipation; `pass0` stores the prefix moments and the rotation
+    // maximum.
+    auto sweep = [&](bool ds, bool pass0) {
     double A0 = 0, AC = 0, AS = 0;
     T ec[DMAX];
-    [[maybe_unused]] T rc[DMAX];
+    // the network's dm^2 - 1 of this and the next frequency as raw bf16
+    // bits; the direction's row offset joins the lane offset (a vector add)
+    // rather than holding DMAX more scalar registers
+    [[maybe_unused]] uint32_t rc[DMAX];
     FreqTab<T> tc;
     {
       const rsrc_t ep = freq_rsrc(eb, 0, pm, ldi);
@@ -497,7 +500,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
         if (ds) {
           const rsrc_t rp = sur_rsrc(rsur, 0, pm, ldi);
 #pragma unroll
-          for (int q = 0; q < DMAX; ++q) rc[q] = bld_bf16(FULL || q < nd ? rp : rz, vj2, dirp(q) * rb2);
+          for (int q = 0; q < DMAX; ++q) rc[q] = bld_u16(FULL || q < nd ? rp : rz, vj2 + dirp(q) * rb2);
         }
       }
     }
@@ -509,15 +512,15 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       // issue the next frequency's loads before computing this one
       const int on = o + 1 < om ? o + 1 : o;
       T en[DMAX];
-      [[maybe_unused]] T rn[DMAX];
       const rsrc_t epn = freq_rsrc(eb, on, pm, ldi);
 #pragma unroll
       for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(FULL || q < nd ? epn : rz, vj, dirp(q) * rb);
+      [[maybe_unused]] uint32_t rn[DMAX];
       if constexpr (SUR) {
         if (ds) {
           const rsrc_t rp = sur_rsrc(rsur, on, pm, ldi);
 #pragma unroll
-          for (int q = 0; q < DMAX; ++q) rn[q] = bld_bf16(FULL || q < nd ? rp : rz, vj2, dirp(q) * rb2);
+          for (int q = 0; q < DMAX; ++q) rn[q] = bld_u16(FULL || q < nd ? rp : rz, vj2 + dirp(q) * rb2);
         }
       }
       float knext = 0.f;
@@ -552,22 +555,26 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       [[maybe_unused]] const T sbfv = tc[FT_SBF], kdkv = tc[FT_KDK];
       T m0 = 0, mc = 0, ms = 0, rq = 0;
       [[maybe_unused]] T rqs = 0;
-      // (e k4)^p of |e|: a bin emptied by round-off to about -1e-30 gives a
+      // sds = dm^2 sat with sat = ex(p lg |e| + Lk) = cf coth (e k4)^p: of
+      // |e|, so that a bin emptied by round-off to about -1e-30 gives a
       // vanishing sds, as the 0 the reference's power of a non-positive
       // base is taken to be here, without a clamp instruction
-      auto sds_of = [&](T dm, T e) { return dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk)); };
-      // the network's implied dissipation of one bin: its rate, residual
-      // and audit sums
-      auto sur_bin = [&](T ss, T R, T e, T sdp, bool use_slot) {
-        const T sdss = fmax(ss - sbfv - R, T(0));
-        const T r2 = ss - fma(sdss, cs, crest);
+      // the network's dm^2 of one bin: its rate, the residual of the
+      // spreading-factor bounds and the audit sums (sat = cf coth (e k^4)^p,
+      // sdp = the plain sds = dm^2 sat)
+      [[maybe_unused]] const T qhi = (T(1) + T(2) * mssf * pa0) * (T(1) + T(2) * mssf * pa0);
+      auto sur_bin = [&](T ss, T qn, T e, T sat, T sdp, bool use_slot) {
+        const T dmn = T(1) + qn;  // the network stores dm^2 - 1
+        const T sdn = fmin(fmax(dmn, T(1)), qhi) * sat;
+        const T r2 = ss - fma(sdn, cs, crest);
         rqs = fmax(rqs, use_slot ? fabs(r2) : T(0));
         const T wv = e * kdkv;
-        neg = fma(fmax(R - (ss - sbfv), T(0)), wv, neg);
-        tot = fma(fabs(ss) + sbfv, wv, tot);
+        const bool inb = dmn >= T(1 - kSurBoundTol) && dmn <= qhi * T(1 + kSurBoundTol);
+        neg += inb ? T(0) : wv;  // NaN or Inf fail the bounds too
+        tot += wv;
         if (aud) {
-          anum = fma(fabs(sdss - sdp), wv, anum);
-          aden = fma(sdp + fabs(ss) + sbfv, wv, aden);
+          anum = fma(fabs(sdn - sdp), wv, anum);
+          aden = fma(sdp, wv, aden);
         }
       };
       // an opaque copy per frequency (an empty asm statement): without it the
@@ -589,18 +596,16 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
           const T ssa = ya * (ya > T(0) ? sf.pos : na), ssb = yb * (yb > T(0) ? sf.pos : nb);
           T dm = fma(qc, c2, fma(qs, s2, q0));
           dm = dm * dm;
-          T spa = 0, spb = 0;
-          if (dp) {
-            spa = sds_of(dm, ea);
-            spb = sds_of(dm, eb2);
-            const T ra = ssa - fma(spa, cs, crest);
-            const T rb2 = ssb - fma(spb, cs, crest);
-            rq = fmax(rq, fmax(fabs(ra), fabs(rb2)));
-          }
+          const T sata = Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(ea)), Lk));
+          const T satb = Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(eb2)), Lk));
+          const T spa = dm * sata, spb = dm * satb;
+          const T ra = ssa - fma(spa, cs, crest);
+          const T rbb = ssb - fma(spb, cs, crest);
+          rq = fmax(rq, fmax(fabs(ra), fabs(rbb)));
           if constexpr (SUR) {
             if (ds) {
-              sur_bin(ssa, rc[h], ea, spa, true);
-              sur_bin(ssb, rc[h + H], eb2, spb, true);
+              sur_bin(ssa, bf16_bits_to_f(rc[h]), ea, sata, spa, true);
+              sur_bin(ssb, bf16_bits_to_f(rc[h + H]), eb2, satb, spb, true);
             }
           }
           const T ep2 = ea + eb2;
@@ -617,14 +622,12 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
           const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
           T dm = fma(qc, c2, fma(qs, s2, q0));
           dm = dm * dm;
-          T sp = 0;
-          if (dp) {
-            sp = sds_of(dm, e);
-            const T r = ss - fma(sp, cs, crest);
-            rq = fmax(rq, FULL || q < nd ? fabs(r) : T(0));
-          }
+          const T sat = Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk));
+          const T sp = dm * sat;
+          const T r = ss - fma(sp, cs, crest);
+          rq = fmax(rq, FULL || q < nd ? fabs(r) : T(0));
           if constexpr (SUR) {
-            if (ds) sur_bin(ss, rc[q], e, sp, FULL || q < nd);
+            if (ds) sur_bin(ss, bf16_bits_to_f(rc[q]), e, sat, sp, FULL || q < nd);
           }
           m0 += e;
           mc = fma(e, c2, mc);
@@ -680,7 +683,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       if constexpr (kDerive) tg.advance(o, knext);
     }
     };
-    sweep(do_plain, do_sur, true);
+    sweep(do_sur, true);
     // the surrogate's verdict per lane (the same in every wave)
     [[maybe_unused]] bool use = false;
     if constexpr (SUR) {
@@ -701,28 +704,34 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
             sden += smem[(3 * W + q) * 64 + lane];
           }
         }
-        const bool rfail = sneg > (double)d.sur_resid_tol * stot;
-        const bool afail = aud && snum > (double)d.sur_audit_tol * sden;
+        // written as negated <= so that NaN sums count as failures
+        const bool rfail = !(sneg <= (double)d.sur_resid_tol * stot);
+        const bool afail = aud && !(snum <= (double)d.sur_audit_tol * sden);
         use = valid && sok && !rfail && !afail;
-        if (!do_plain && __any(valid && !use)) {
-          sweep(true, false, false);  // plain rates for the lanes that fell back
-          do_plain = true;
-        }
         if (w == 0) {
           if (valid) pstore(d.sur_use, ldi, j, (int32_t)(use ? 1 : 0));
-          double st[SS_N] = {};
-          st[SS_POINTS] = valid ? 1 : 0;
-          st[SS_USED] = use ? 1 : 0;
-          st[SS_FALL_RANGE] = valid && !sok ? 1 : 0;
-          st[SS_FALL_RESID] = valid && sok && rfail ? 1 : 0;
-          st[SS_FALL_AUDIT] = valid && sok && !rfail && afail ? 1 : 0;
-          st[SS_AUDITED] = valid && aud ? 1 : 0;
-          st[SS_AUDIT_NUM] = valid && aud ? snum : 0;
-          st[SS_AUDIT_DEN] = valid && aud ? sden : 0;
+          // counts as 64-bit integer atomics (ballot counts), the audit sums
+          // as FP64 atomics from audited workgroups only: FP64 atomic adds
+          // from every workgroup to the same eight words serialised and cost
+          // several times the sweep itself
+          const bool cnt[SS_AUDIT_NUM] = {valid,
+                                          use,
+                                          valid && !sok,
+                                          valid && sok && rfail,
+                                          valid && sok && !rfail && afail,
+                                          valid && aud};
+          auto* ctr = reinterpret_cast<unsigned long long*>(d.sur_stats);
 #pragma unroll
-          for (int q = 0; q < SS_N; ++q) {
-            const double v = wave_sum(st[q]);
-            if (lane == 0 && v != 0) atomicAdd(d.sur_stats + q, v);
+          for (int q = 0; q < SS_AUDIT_NUM; ++q) {
+            const unsigned long long c = __popcll(__ballot(cnt[q]));
+            if (lane == 0 && c) atomicAdd(ctr + q, c);
+          }
+          if (aud) {
+            const double a = wave_sum(valid ? snum : 0.0), b = wave_sum(valid ? sden : 0.0);
+            if (lane == 0) {
+              atomicAdd(d.sur_stats + SS_AUDIT_NUM, a);
+              atomicAdd(d.sur_stats + SS_AUDIT_DEN, b);
+            }
           }
         }
       }
@@ -881,12 +890,10 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   for (int q = 0; q < DMAX; ++q) H1[q] = H2[q] = 0;
   T b1m1 = 0, b2m2 = 0, b2m1 = 0;  // bf1r(o-1), bf2r(o-2), bf2r(o-1)
   // Surrogate (FP32): the lanes k_rates let take the network's rates use
-  // the implied dissipation sds = max(ssin - sbf - R, 0) (after cothkd) in
-  // the integration, in snl and in the diagnostic tail, whose equilibrium
-  // k^-4 ((ssin - sdt - sdv + sice) / (cf coth dm))^(1/p) equals
-  // e ((ssin - sdt - sdv + sice) / sds)^(1/p) for the plain sds.
+  // its spreading factor dm^2, clamped to [1, (1 + 2 mss_fac m0)^2] as
+  // there, wherever the moment prefix's dm^2 would enter: the dissipation
+  // in the integration, in snl and in the diagnostic tail.
   [[maybe_unused]] int use = 0;
-  bool do_plain = true;
   [[maybe_unused]] bool do_sur = false;
   [[maybe_unused]] const uint16_t* const rsur = d.sur_r;
   [[maybe_unused]] const uint32_t vj2 = (uint32_t)j * 2u, rb2 = (uint32_t)ldi * 2u;
@@ -894,11 +901,10 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     if (rsur) {
       use = valid ? pload(d.sur_use, ldi, j) : 0;
       do_sur = __any(use != 0);
-      do_plain = __any(valid && use == 0);
     }
   }
   T ec[DMAX];
-  [[maybe_unused]] T rc[DMAX];
+  [[maybe_unused]] uint32_t rc[DMAX];  // raw bf16 bits of dm^2 - 1 (see k_rates)
   {
     const rsrc_t ep = freq_rsrc(eb, 0, pm, ldi);
 #pragma unroll
@@ -907,7 +913,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
       if (do_sur) {
         const rsrc_t rp = sur_rsrc(rsur, 0, pm, ldi);
 #pragma unroll
-        for (int q = 0; q < DMAX; ++q) rc[q] = bld_bf16(FULL || q < nd ? rp : rz, vj2, (p0 + q) * rb2);
+        for (int q = 0; q < DMAX; ++q) rc[q] = bld_u16(FULL || q < nd ? rp : rz, vj2 + (p0 + q) * rb2);
       }
     }
   }
@@ -924,15 +930,15 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   for (int o = 0; o < om; ++o) {
     const int on = o + 1 < om ? o + 1 : o;
     T en[DMAX];
-    [[maybe_unused]] T rn[DMAX];
     const rsrc_t epn = freq_rsrc(eb, on, pm, ldi);
 #pragma unroll
     for (int q = 0; q < DMAX; ++q) en[q] = bld<T>(FULL || q < nd ? epn : rz, vj, (p0 + q) * rb);
+    [[maybe_unused]] uint32_t rn[DMAX];
     if constexpr (SUR) {
       if (do_sur) {
         const rsrc_t rp = sur_rsrc(rsur, on, pm, ldi);
 #pragma unroll
-        for (int q = 0; q < DMAX; ++q) rn[q] = bld_bf16(FULL || q < nd ? rp : rz, vj2, (p0 + q) * rb2);
+        for (int q = 0; q < DMAX; ++q) rn[q] = bld_u16(FULL || q < nd ? rp : rz, vj2 + (p0 + q) * rb2);
       }
     }
     Prefix<T> pn{};
@@ -958,7 +964,8 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
     const T Lk = spow * lk4 + Tr<T>::lg(cf);  // sds before cothkd
     const T c0 = -dtx * crest, dtc = dtx * coth;
     const T icfc = Tr<T>::rcp(cf * coth);
-    [[maybe_unused]] const T icoth = Tr<T>::rcp(coth), sbfv = tc[FT_SBF];
+    // the spreading factor's upper bound, from the zeroth-moment prefix
+    [[maybe_unused]] const T qhi = (T(1) + T(2) * mssf * pc.a0) * (T(1) + T(2) * mssf * pc.a0);
     // dm = q0 + qc cos 2th + qs sin 2th = (q0 - qc) + ct (2 qc ct + 2 qs st),
     // with cos 2th = 2 ct^2 - 1 and sin 2th = 2 ct st: the sweep keeps
     // only ct and st per direction
@@ -988,15 +995,11 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
         const T ss = sin_m(sf, sin_y(sf, ct, st), mq[q]);
         T dm = fma(ct, fma(qc2, ct, qs2 * st), q0m);
         dm = dm * dm;
-        // sds before cothkd; |e|: see k_rates
-        T sdsp = do_plain ? dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk)) : T(0);
-        [[maybe_unused]] T sdss = 0;
         if constexpr (SUR) {
-          if (do_sur) {
-            sdss = fmax(ss - sbfv - rc[q], T(0));
-            sdsp = use ? sdss * icoth : sdsp;
-          }
+          if (do_sur && use) dm = fmin(fmax(T(1) + bf16_bits_to_f(rc[q]), T(1)), qhi);  // dm^2 - 1
         }
+        // sds before cothkd; |e|: see k_rates
+        const T sdsp = dm * Tr<T>::ex(fma(spow, Tr<T>::lg(fabs(e)), Lk));
         const T X = sdsp * e;
         T G = 0;
         if constexpr (decltype(PROG)::value) {
@@ -1008,13 +1011,7 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
           // diagnostic tail, src/umwm_physics.F90:70-80
           const T num = fmax(ss, T(0)) - ctail;
           const T t = fmax(num, T(0)) * (icfc * Tr<T>::rcp(dm));
-          T es_ = Tr<T>::ex(fma(ipow, Tr<T>::lg(t), -lk4));  // k^-4 t^(1/p)
-          if constexpr (SUR) {
-            if (do_sur && use) {
-              const T lq = Tr<T>::lg(fmax(num, T(0))) - Tr<T>::lg(sdss);
-              es_ = sdss > T(0) ? e * Tr<T>::ex(ipow * lq) : T(0);
-            }
-          }
+          const T es_ = Tr<T>::ex(fma(ipow, Tr<T>::lg(t), -lk4));  // k^-4 t^(1/p)
           bst(FULL || q < nd ? rs0 : rz, vtail, T(0.5) * (e + es_), so);
         }
         const T h1 = fma(hdb1, X, H1[q]);  // bf1r(o-1) X(o) into bin o-1
@@ -1421,6 +1418,20 @@ int launch_fused(const DomainDev<T>* doms, const BlockTask* tasks, int ntasks, i
   if (ntasks <= 0) return 0;
   if (tile.W1 < 1 || tile.W1 > 8 || tile.W2 < 1 || tile.W2 > 8)
     return fail(UMWM_ERR_ARGUMENT, "tile W must be 1..8");
+  if (sur) {
+    // the surrogate variants hold the network's dm^2 of this and the next
+    // frequency beside the spectrum: more waves of fewer directions each,
+    // the same W D, keep them off scratch (k_rates above 18 directions and
+    // k_integrate above 9 spill)
+    while (tile.D1 > 18 && tile.D1 % 2 == 0 && tile.W1 * 2 <= 8 && tile_supported(tile.D1 / 2)) {
+      tile.D1 /= 2;
+      tile.W1 *= 2;
+    }
+    while (tile.D2 > 9 && tile.D2 % 2 == 0 && tile.W2 * 2 <= 8 && tile_supported(tile.D2 / 2)) {
+      tile.D2 /= 2;
+      tile.W2 *= 2;
+    }
+  }
   switch (tile.D1) {
     case 6: return launch_d2<T, 6>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev, sur);
     case 9: return launch_d2<T, 9>(doms, tasks, ntasks, omax, first, rotation, tile, keep, s, ev, sur);

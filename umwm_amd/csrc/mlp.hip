@@ -23,6 +23,7 @@
 #include <random>
 #include <string>
 
+#include "bufio.hpp"
 #include "common.hpp"
 #include "mlp.hpp"
 
@@ -147,9 +148,197 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
         } else if constexpr (EP == EP_DGELU) {
           ((bf16*)g.C)[o] = (bf16)(v * gelu_grad_f((float)g.Zin[(int64_t)m * g.ldz + n]));
         } else if constexpr (EP == EP_SUR) {
-          ((bf16*)g.C)[o] = (bf16)(g.oc[m] * sinhf(g.osd[m] * (v + g.bias[m]) + g.omu[m]));
+          ((bf16*)g.C)[o] = (bf16)expm1f(fminf(g.osd[m] * (v + g.bias[m]) + g.omu[m], 30.f));
         } else {
           ((float*)g.C)[(int64_t)blockIdx.z * g.M * g.ldc + o] = v;
+        }
+      }
+}
+
+// Inference GEMM.  A workgroup of 4 waves computes a 256 x 128 block of C;
+// wave (wm, wn) owns a 128 x 64 part as 4 x 2 tiles of the 32x32x16 MFMA,
+// so that each 16-byte LDS read feeds 8/6 of an MFMA (the training GEMM's
+// 2 x 2 wave tiles feed one).  The K loop steps by 32 with the next tile
+// staged in registers, as in k_gemm.  Blocks are numbered so that the
+// N/128 blocks sharing one A block land on the same XCD back to back
+// (workgroups go round-robin over the 8 XCDs): the A block, the large
+// operand, then comes from that XCD's L2 after the first read.
+//
+// A_SPEC stages the first layer's input from the spectrum (fp32, points
+// contiguous): each thread loads 4 points x 8 inputs as 8 float4 reads
+// (a wave reads 1 KB contiguous per input row), forms ln e, normalises,
+// rounds to bf16 and writes each point's 8 inputs as one 16-byte LDS row
+// segment.
+constexpr int IBM = 256, IBN = 128, ILDK = BK + 8;
+
+template <int AM>
+struct InfStage;
+template <>
+struct InfStage<A_MK> {
+  bf16x8 a[4];
+};
+template <>
+struct InfStage<A_SPEC> {
+  f32x4 a[8];
+};
+
+template <int AM, int EP>
+__global__ __launch_bounds__(256, 2) void k_gemm_inf(GemmArgs g, int ntm, int ntn) {
+  __shared__ __attribute__((aligned(16))) bf16 As[2][IBM][ILDK];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[2][IBN][ILDK];
+  // XCD-aware tile order (the grid is padded to a multiple of 8)
+  const int nt = ntm * ntn, per = (int)(gridDim.x / 8);
+  const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (t >= nt) return;
+  const int tm = t / ntn, tn = t % ntn;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int m0 = tm * IBM, n0 = tn * IBN;
+  const int K = g.K;
+  const bf16x8 zero = {};
+  InfStage<AM> ra;
+  bf16x8 rb[2];
+  // A_SPEC: this thread's 4 points and (wave-uniform) 8-input group
+  [[maybe_unused]] const int smc = (tid & 63) * 4;
+  [[maybe_unused]] const int skg = __builtin_amdgcn_readfirstlane(tid >> 6) * 8;
+  [[maybe_unused]] uint32_t bad = 0;  // zmax violations of the 4 points
+  // Every index stays inside its operand: rows are checked against M and
+  // N, k against K, a 16-byte bf16 chunk never straddles K (a multiple of
+  // 8, checked by gemm_infer()), and a float4 of the spectrum is read only
+  // where all 4 points lie below M (else point by point).
+  auto gload = [&](int k0) {
+    if constexpr (AM == A_MK) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int q = tid + c * 256, r = q >> 2, kc = (q & 3) * 8, m = m0 + r, k = k0 + kc;
+        ra.a[c] = (m < g.M && k < K) ? *(const bf16x8*)(g.A + (int64_t)m * g.lda + k) : zero;
+      }
+    } else {
+      // one buffer resource per input row (scalar registers: the row is
+      // wave-uniform) covering this block's points below M; reads past M
+      // return 0
+      const SpecOperand& sa = g.sa;
+      const int kb = k0 + skg;
+      const bool full = m0 + smc + 4 <= g.M;
+      const uint32_t bytes = (uint32_t)std::min(g.M - m0, IBM) * 4u, voff = (uint32_t)smc * 4u;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int k = kb + i;
+        f32x4 v = {0.f, 0.f, 0.f, 0.f};
+        const float* row = k < sa.nb ? sa.e + (int64_t)k * sa.lde
+                                     : sa.ext + (int64_t)(k - sa.nb) * sa.ldx;
+        if (k < sa.nb + 3) {
+          const rsrc_t r = make_rsrc(row + m0, bytes);
+          if (full) {
+            v = bld16<float>(r, voff);
+          } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] = bld<float>(r, voff + 4u * c);
+          }
+        }
+        ra.a[i] = v;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int q = tid + c * 256, r = q >> 2, kc = (q & 3) * 8, n = n0 + r, k = k0 + kc;
+      rb[c] = (n < g.N && k < K) ? *(const bf16x8*)(g.B + (int64_t)n * g.ldb + k) : zero;
+    }
+  };
+  auto sstore = [&](int buf, int k0) {
+    if constexpr (AM == A_MK) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int q = tid + c * 256;
+        *(bf16x8*)&As[buf][q >> 2][(q & 3) * 8] = ra.a[c];
+      }
+    } else {
+      const SpecOperand& sa = g.sa;
+      const int kb = k0 + skg;
+      bf16x8 o[4];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int k = kb + i;
+        const f32x4 v = ra.a[i];
+        float x[4] = {v[0], v[1], v[2], v[3]};
+        if (k < sa.nb) {
+          const float mu = sa.mu[k], is = sa.is[k];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            x[c] = (__logf(fmaxf(x[c], 1e-30f)) - mu) * is;
+            bad |= (fabsf(x[c]) > sa.zmax ? 1u : 0u) << c;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) o[c][i] = (bf16)x[c];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) *(bf16x8*)&As[buf][smc + c][skg] = o[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int q = tid + c * 256;
+      *(bf16x8*)&Bs[buf][q >> 2][(q & 3) * 8] = rb[c];
+    }
+  };
+  f32x16 acc[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.f;
+  int buf = 0;
+  gload(0);
+  sstore(0, 0);
+  __syncthreads();
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    const bool more = k0 + BK < K;
+    if (more) gload(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int kk = ks * 16 + (lane >> 5) * 8;
+      bf16x8 fa[4], fb[2];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) fa[a] = *(const bf16x8*)&As[buf][wm * 128 + a * 32 + (lane & 31)][kk];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) fb[b] = *(const bf16x8*)&Bs[buf][wn * 64 + b * 32 + (lane & 31)][kk];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    if (more) sstore(buf ^ 1, k0 + BK);
+    __syncthreads();
+    buf ^= 1;
+  }
+  if constexpr (AM == A_SPEC) {
+    // the first column of blocks reports each point's zmax verdict
+    if (tn == 0 && bad) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (((bad >> c) & 1u) && m0 + smc + c < g.M) atomicAnd(g.sa.ok + m0 + smc + c, 0);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int m = m0 + wm * 128 + a * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        const int n = n0 + wn * 64 + b * 32 + (lane & 31);
+        if (m >= g.M || n >= g.N) continue;
+        const float v = acc[a][b][i];
+        const int64_t o = (int64_t)m * g.ldc + n;
+        if constexpr (EP == EP_GELU) {
+          ((bf16*)g.C)[o] = (bf16)gelu_f(v + g.bias[n]);
+        } else if constexpr (EP == EP_BF16_BM) {
+          ((bf16*)g.C)[o] = (bf16)(v + g.bias[m]);
+        } else {
+          static_assert(EP == EP_SUR, "inference epilogues: GELU, BF16_BM, SUR");
+          ((bf16*)g.C)[o] = (bf16)expm1f(fminf(g.osd[m] * (v + g.bias[m]) + g.omu[m], 30.f));
         }
       }
 }
@@ -257,6 +446,32 @@ int gemm(int alay, int blay, int ep, const GemmArgs& g0, int splits, hipStream_t
   if (rc) return rc;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(UMWM_ERR_HIP, std::string("gemm: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int gemm_infer(int amode, int ep, const GemmArgs& g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0 || g.K <= 0) return 0;
+  if (g.K % 8 || g.ldb % 8 || (amode == A_MK && g.lda % 8))
+    return fail(UMWM_ERR_ARGUMENT, "gemm_infer: K, lda and ldb must be multiples of 8");
+  if (amode == A_SPEC) {
+    const SpecOperand& a = g.sa;
+    if (!a.e || !a.mu || !a.is || !a.ext || !a.ok || a.nb < 1 || a.nb + 3 > g.K ||
+        a.lde % 4 || reinterpret_cast<uintptr_t>(a.e) % 16)
+      return fail(UMWM_ERR_ARGUMENT, "gemm_infer: bad spectrum operand");
+  }
+  const int ntm = (g.M + IBM - 1) / IBM, ntn = (g.N + IBN - 1) / IBN;
+  const int64_t nt = (int64_t)ntm * ntn;
+  if (nt > (1ll << 30)) return fail(UMWM_ERR_SIZE, "gemm_infer: too many tiles");
+  const dim3 grid((unsigned)((nt + 7) / 8 * 8));
+#define UMWM_INF(AM_, EP_) hipLaunchKernelGGL((k_gemm_inf<AM_, EP_>), grid, dim3(256), 0, s, g, ntm, ntn)
+  if (amode == A_MK && ep == EP_GELU) UMWM_INF(A_MK, EP_GELU);
+  else if (amode == A_SPEC && ep == EP_GELU) UMWM_INF(A_SPEC, EP_GELU);
+  else if (amode == A_MK && ep == EP_SUR) UMWM_INF(A_MK, EP_SUR);
+  else if (amode == A_MK && ep == EP_BF16_BM) UMWM_INF(A_MK, EP_BF16_BM);
+  else return fail(UMWM_ERR_ARGUMENT, "gemm_infer: unsupported operand/epilogue pair");
+#undef UMWM_INF
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(UMWM_ERR_HIP, std::string("gemm_infer: ") + hipGetErrorString(e));
   return 0;
 }
 
@@ -435,12 +650,14 @@ int mlp_adam(Mlp& m, float lr, float b1, float b2, float eps, float wd, hipStrea
   return mlp_sync_bf16(m, s);
 }
 
-int mlp_infer_transposed(Mlp& m, int rows, bf16* out, int64_t ldo, hipStream_t s,
-                         const float* omu, const float* osd, const float* oc) {
-  if (rows > m.cap_rows) return fail(UMWM_ERR_SIZE, "mlp: more rows than the buffers hold");
+namespace {
+// Hidden layers 1.. and the transposed last layer of an inference pass
+// whose first hidden layer already ran into m.h[0].
+int infer_rest(Mlp& m, int rows, bf16* out, int64_t ldo, hipStream_t s, const float* omu,
+               const float* osd) {
   const int nl = (int)m.L.size();
-  const bf16* in = m.x;
-  for (int l = 0; l + 1 < nl; ++l) {
+  const bf16* in = m.h[0];
+  for (int l = 1; l + 1 < nl; ++l) {
     const MlpLayerDev& L = m.L[l];
     GemmArgs g{};
     g.A = in; g.lda = L.K;
@@ -448,7 +665,7 @@ int mlp_infer_transposed(Mlp& m, int rows, bf16* out, int64_t ldo, hipStream_t s
     g.M = rows; g.N = L.N; g.K = L.K;
     g.bias = L.b;
     g.C = m.h[l]; g.ldc = L.N;
-    if (int rc = gemm(A_MK, B_NK, EP_GELU, g, 1, s)) return rc;
+    if (int rc = gemm_infer(A_MK, EP_GELU, g, s)) return rc;
     in = m.h[l];
   }
   // out[o][j] = sum_k W[o][k] h[j][k] + b[o]: the network's outputs of point
@@ -460,11 +677,43 @@ int mlp_infer_transposed(Mlp& m, int rows, bf16* out, int64_t ldo, hipStream_t s
   g.M = m.nout; g.N = rows; g.K = L.K;
   g.bias = L.b;
   g.C = out; g.ldc = ldo;
-  if (omu && osd && oc) {
-    g.omu = omu; g.osd = osd; g.oc = oc;
-    return gemm(A_MK, B_NK, EP_SUR, g, 1, s);
+  if (omu && osd) {
+    g.omu = omu; g.osd = osd;
+    return gemm_infer(A_MK, EP_SUR, g, s);
   }
-  return gemm(A_MK, B_NK, EP_BF16_BM, g, 1, s);
+  return gemm_infer(A_MK, EP_BF16_BM, g, s);
+}
+}  // namespace
+
+int mlp_infer_transposed(Mlp& m, int rows, bf16* out, int64_t ldo, hipStream_t s,
+                         const float* omu, const float* osd) {
+  if (rows > m.cap_rows) return fail(UMWM_ERR_SIZE, "mlp: more rows than the buffers hold");
+  if (m.L.size() < 2) return fail(UMWM_ERR_ARGUMENT, "mlp: inference needs a hidden layer");
+  const MlpLayerDev& L = m.L[0];
+  GemmArgs g{};
+  g.A = m.x; g.lda = L.K;
+  g.B = L.wb; g.ldb = L.K;
+  g.M = rows; g.N = L.N; g.K = L.K;
+  g.bias = L.b;
+  g.C = m.h[0]; g.ldc = L.N;
+  if (int rc = gemm_infer(A_MK, EP_GELU, g, s)) return rc;
+  return infer_rest(m, rows, out, ldo, s, omu, osd);
+}
+
+int mlp_infer_spectrum(Mlp& m, const SpecOperand& in, int rows, bf16* out, int64_t ldo,
+                       hipStream_t s, const float* omu, const float* osd) {
+  if (rows > m.cap_rows) return fail(UMWM_ERR_SIZE, "mlp: more rows than the buffers hold");
+  if (m.L.size() < 2) return fail(UMWM_ERR_ARGUMENT, "mlp: inference needs a hidden layer");
+  const MlpLayerDev& L = m.L[0];
+  if (in.nb + 3 != m.nin) return fail(UMWM_ERR_ARGUMENT, "mlp: spectrum operand width");
+  GemmArgs g{};
+  g.sa = in;
+  g.B = L.wb; g.ldb = L.K;
+  g.M = rows; g.N = L.N; g.K = L.K;
+  g.bias = L.b;
+  g.C = m.h[0]; g.ldc = L.N;
+  if (int rc = gemm_infer(A_SPEC, EP_GELU, g, s)) return rc;
+  return infer_rest(m, rows, out, ldo, s, omu, osd);
 }
 
 }  // namespace umwm

@@ -31,11 +31,30 @@ enum GemmLayout { A_MK = 0, A_KM = 1, B_NK = 0, B_KN = 1 };
 //   EP_BF16_BM   C[m][n] = acc + bias[m] (bf16): the transposed last layer
 //   EP_DGELU     C[m][n] = acc * gelu'(Zin[m][n]) (bf16)
 //   EP_PART      C[z][m][n] = acc (fp32 split-K partial, z = blockIdx.z)
-//   EP_SUR       C[m][n] = oc[m] sinh(osd[m] (acc + bias[m]) + omu[m]) (bf16):
-//                the surrogate's transposed last layer, its normalised
-//                output turned back into the rate it stands for
+//   EP_SUR       C[m][n] = exp(min(osd[m] (acc + bias[m]) + omu[m], 30)) - 1
+//                (bf16): the surrogate's transposed last layer, its
+//                normalised output turned back into the spreading factor
+//                dm^2 it stands for, stored as dm^2 - 1 (>= 0 in exact
+//                arithmetic), whose bf16 rounding keeps 3 significant digits
+//                where dm^2 itself, close to 1, would keep 2
 enum GemmEpilogue { EP_GELU = 0, EP_F32 = 1, EP_BF16_BM = 2, EP_DGELU = 3, EP_PART = 4,
                     EP_SUR = 5 };
+
+// The surrogate's first-layer operand taken straight from a domain's
+// spectrum: A(m, k) = (ln max(e[k lde + m], 1e-30) - mu[k]) is[k] for the
+// nb spectral inputs k < nb, ext[(k - nb) ldx + m] (already normalised)
+// for the next three, 0 beyond.  |A(m, k)| > zmax for a spectral input
+// clears ok[m].
+struct SpecOperand {
+  const float* e = nullptr;
+  int64_t lde = 0;
+  int nb = 0;
+  const float *mu = nullptr, *is = nullptr;
+  const float* ext = nullptr;
+  int64_t ldx = 0;
+  float zmax = 0.f;
+  int32_t* ok = nullptr;
+};
 
 struct GemmArgs {
   const bf16* A;
@@ -47,10 +66,17 @@ struct GemmArgs {
   int64_t lda, ldb, ldc, ldz;
   int M, N, K;
   int kchunk;  // K range of one blockIdx.z (split-K)
-  const float *omu, *osd, *oc;  // EP_SUR per-row output transform
+  const float *omu, *osd;  // EP_SUR per-row output transform
+  SpecOperand sa;          // A_SPEC operand (inference GEMM)
 };
 
 int gemm(int alay, int blay, int ep, const GemmArgs& g, int splits, hipStream_t s);
+
+// Inference GEMM (forward only, B_NK, epilogue EP_GELU without Zout or
+// EP_SUR / EP_BF16_BM): 256 x 128 blocks, the A operand bf16 A_MK or the
+// spectrum (A_SPEC).
+enum { A_SPEC = 2 };
+int gemm_infer(int amode, int ep, const GemmArgs& g, hipStream_t s);
 
 // GELU, tanh form: 0.5 z (1 + tanh(c (z + 0.044715 z^3))), c = sqrt(2/pi),
 // and its derivative.
@@ -111,10 +137,13 @@ int mlp_adam(Mlp& m, float lr, float b1, float b2, float eps, float wd, hipStrea
 int mlp_sync_bf16(Mlp& m, hipStream_t s);
 // Inference with the last layer transposed: out[o][ldo] (bf16) for `rows`
 // input rows in m.x; hidden layers run as in mlp_forward (without keeping
-// z).  With omu/osd/oc the EP_SUR transform is applied, else the plain
-// output plus bias.
+// z).  With omu/osd the EP_SUR transform is applied, else the plain output
+// plus bias.
 int mlp_infer_transposed(Mlp& m, int rows, bf16* out, int64_t ldo, hipStream_t s,
-                         const float* omu = nullptr, const float* osd = nullptr,
-                         const float* oc = nullptr);
+                         const float* omu = nullptr, const float* osd = nullptr);
+// The same with the first layer's input read from a spectrum (SpecOperand)
+// instead of m.x.
+int mlp_infer_spectrum(Mlp& m, const SpecOperand& in, int rows, bf16* out, int64_t ldo,
+                       hipStream_t s, const float* omu, const float* osd);
 
 }  // namespace umwm

@@ -51,13 +51,12 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
 }
 
 // One training sample per sea point: the raw inputs (ln e of every bin,
-// u10, v10, ln depth) and the raw targets asinh(R / (2 pi f s0)) of the
-// source arrays the operator kernels just evaluated (ssin, sds after its
-// cothkd factor, the sbf table).
+// u10, v10, ln depth) and the raw targets ln dm^2, the directional
+// spreading factor (1 + mss_fac mss)^2 that sds_d12 just left in `dummy`
+// (src/umwm_source_functions.F90:92-99).
 template <typename T>
 __global__ void k_sur_collect(DomainDev<T> d, float* __restrict__ feat, float* __restrict__ targ,
-                              int64_t row0, int64_t cap, int nin, int nout,
-                              const float* __restrict__ cscale) {
+                              int64_t row0, int64_t cap, int nin, int nout) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= d.im) return;
   const int64_t row = row0 + j;
@@ -67,11 +66,9 @@ __global__ void k_sur_collect(DomainDev<T> d, float* __restrict__ feat, float* _
   float* f = feat + row * nin;
   float* t = targ + row * nout;
   for (int b = 0; b < nb; ++b) {
-    const int o = b / d.pm;
     const size_t q = (size_t)b * ldi + j;
     f[b] = logf(fmaxf((float)d.e[q], kSurLogFloor));
-    const double R = (double)d.ssin[q] - (double)d.sds[q] - (double)d.sbf[(size_t)o * ldi + j];
-    t[b] = (float)asinh(R / (double)cscale[o]);
+    t[b] = (float)log(fmax((double)d.dummy[q], 1.0));
   }
   const double w = d.wspd[j], wd = d.wdir[j];
   f[nb] = (float)(w * cos(wd));
@@ -79,16 +76,22 @@ __global__ void k_sur_collect(DomainDev<T> d, float* __restrict__ feat, float* _
   f[nb + 2] = (float)log((double)d.depth[j]);
 }
 
-// Column sums and sums of squares over rows [r0, r1) of a [rows][ld]
-// matrix, a chunk of rows per block row y (accumulated in double).
+// Column sums of a [rows][ld] matrix shifted by the column's first value,
+// sum (x - x0) and sum (x - x0)^2, a chunk of rows per block row y
+// (accumulated in double).  The shift keeps the variance free of the
+// cancellation that E[x^2] - E[x]^2 suffers when the mean is large beside
+// the spread: over 2 M rows of a column pinned at ln 1e-30 = -69.08 it left
+// a spurious standard deviation of 6e-6, whose inverse (1.6e5) turned half
+// a float32 ulp of ln e into 0.6 of a normalised input.
 __global__ void k_colsums(const float* __restrict__ a, int64_t rows, int cols, int ld,
                           int64_t chunk, double* __restrict__ s1, double* __restrict__ s2) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cols) return;
   const int64_t r0 = (int64_t)blockIdx.y * chunk, r1 = std::min(rows, r0 + chunk);
+  const double x0 = a[c];
   double a1 = 0, a2 = 0;
   for (int64_t r = r0; r < r1; ++r) {
-    const double v = a[r * ld + c];
+    const double v = (double)a[r * ld + c] - x0;
     a1 += v;
     a2 += v * v;
   }
@@ -96,17 +99,25 @@ __global__ void k_colsums(const float* __restrict__ a, int64_t rows, int cols, i
   atomicAdd(s2 + c, a2);
 }
 
-// Mean and spread of each column; `inv` stores 1/std, else std.  A
-// constant column gets spread 1.
-__global__ void k_colfinal(const double* __restrict__ s1, const double* __restrict__ s2, int cols,
-                           int64_t rows, float* __restrict__ mu, float* __restrict__ sd, int inv) {
+// Mean and standard deviation (or its inverse) per column from the shifted
+// sums of k_colsums.  A standard deviation below `floor` is raised to it,
+// and a column that does not vary at all gets 1: an input column whose ln e
+// barely varies (a bin at the ln 1e-30 floor in nearly every sample) would
+// otherwise amplify rounding into its normalised input, and a target
+// column that barely varies (ln dm^2 of the lowest frequencies, whose
+// prefix holds little energy) would weigh its noise into the loss as much
+// as the columns that matter.
+__global__ void k_colfinal(const float* __restrict__ a, const double* __restrict__ s1,
+                           const double* __restrict__ s2, int cols, int64_t rows,
+                           float* __restrict__ mu, float* __restrict__ sd, int inv, double floor) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cols) return;
-  const double m = s1[c] / (double)rows;
-  double v = s2[c] / (double)rows - m * m;
+  const double d = s1[c] / (double)rows;
+  const double v = s2[c] / (double)rows - d * d;
   double s = v > 0 ? sqrt(v) : 0;
-  if (s < 1e-6) s = 1;
-  mu[c] = (float)m;
+  if (s == 0) s = 1;
+  s = fmax(s, floor);
+  mu[c] = (float)((double)a[c] + d);
   sd[c] = (float)(inv ? 1 / s : s);
 }
 
@@ -136,13 +147,14 @@ __global__ void k_sur_gather(const float* __restrict__ feat, const float* __rest
     tb[(int64_t)r * N3 + c] = c < nout ? (targ[row * nout + c] - tmu[c]) / tsd[c] : 0.f;
 }
 
-// Inference inputs, per point: wind and depth (normalised), padding, and
-// the range verdict on wind speed and depth.
+// Inference inputs, per point: wind and depth (normalised) into ext
+// [3][ldx], and the range verdict on wind speed and depth.  The spectral
+// inputs are formed by the first layer's GEMM from the spectrum itself.
 template <typename T>
-__global__ void k_sur_point(DomainDev<T> d, int j0, int rows, int nb, int K0,
+__global__ void k_sur_point(DomainDev<T> d, int j0, int rows, int nb,
                             const float* __restrict__ fmu, const float* __restrict__ fis,
-                            const float* __restrict__ range, float margin, bf16* __restrict__ x,
-                            int32_t* __restrict__ ok) {
+                            const float* __restrict__ range, float margin,
+                            float* __restrict__ ext, int64_t ldx, int32_t* __restrict__ ok) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
   const int j = j0 + r;
@@ -150,49 +162,10 @@ __global__ void k_sur_point(DomainDev<T> d, int j0, int rows, int nb, int K0,
   w = w > 1e-2f ? w : 1e-2f;  // as sin_d12 floors it (src/umwm_source_functions.F90:22)
   const float wd = (float)d.wdir[j], dep = (float)d.depth[j];
   const float in3[3] = {w * cosf(wd), w * sinf(wd), logf(dep)};
-  bf16* xr = x + (int64_t)r * K0;
-  for (int c = 0; c < 3; ++c) xr[nb + c] = (bf16)((in3[c] - fmu[nb + c]) * fis[nb + c]);
-  for (int c = nb + 3; c < K0; ++c) xr[c] = (bf16)0.f;
+  for (int c = 0; c < 3; ++c) ext[c * ldx + r] = (in3[c] - fmu[nb + c]) * fis[nb + c];
   const float lo = 1.f - margin, hi = 1.f + margin;
   ok[j] = (w >= range[0] * lo && w <= range[1] * hi && dep >= range[2] * lo &&
            dep <= range[3] * hi) ? 1 : 0;
-}
-
-// Inference inputs, spectral part: a 64-point x 64-bin tile of normalised
-// ln e goes through LDS so that both the spectrum reads (points
-// contiguous) and the input-row writes (bins contiguous) are coalesced; a
-// point with an input beyond +-zmax standard deviations fails the range
-// check.
-template <typename T>
-__global__ __launch_bounds__(256) void k_sur_tile(DomainDev<T> d, int j0, int rows, int nb, int K0,
-                                                  const float* __restrict__ fmu,
-                                                  const float* __restrict__ fis, float zmax,
-                                                  bf16* __restrict__ x, int32_t* __restrict__ ok) {
-  __shared__ float tile[64][65];
-  const int r0 = blockIdx.x * 64, b0 = blockIdx.y * 64;
-  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  const size_t ldi = d.ldi;
-  bool bad = false;
-  for (int i = 0; i < 16; ++i) {
-    const int bl = grp * 16 + i, b = b0 + bl, r = r0 + lane;
-    float v = 0.f;
-    if (b < nb && r < rows) {
-      const float e = (float)d.e[(size_t)b * ldi + j0 + r];
-      v = (logf(fmaxf(e, kSurLogFloor)) - fmu[b]) * fis[b];
-      bad |= fabsf(v) > zmax;
-    }
-    tile[bl][lane] = v;
-  }
-  if (bad) atomicAnd(ok + j0 + r0 + lane, 0);
-  __syncthreads();
-  const int rl = threadIdx.x >> 2, cb = (threadIdx.x & 3) * 16, r = r0 + rl;
-  if (r < rows) {
-    bf16* xr = x + (int64_t)r * K0;
-    for (int i = 0; i < 16; ++i) {
-      const int b = b0 + cb + i;
-      if (b < nb) xr[b] = (bf16)tile[cb + i][rl];
-    }
-  }
 }
 
 __global__ void k_store_loss(double* acc, double* out) {
@@ -227,17 +200,17 @@ int surrogate_stats(SurrogateImpl& S, int64_t n) {
   UMWM_HIP_TRY(hipMalloc(&s1, cols * sizeof(double)));
   UMWM_HIP_TRY(hipMalloc(&s2, cols * sizeof(double)));
   const int64_t chunk = 4096, nch = (n + chunk - 1) / chunk;
-  auto one = [&](const float* a, int nc, float* mu, float* sd, int inv) -> int {
+  auto one = [&](const float* a, int nc, float* mu, float* sd, int inv, double floor) -> int {
     hipLaunchKernelGGL(k_fill_f64, g1(nc), dim3(256), 0, s, s1, (int64_t)nc, 0.0);
     hipLaunchKernelGGL(k_fill_f64, g1(nc), dim3(256), 0, s, s2, (int64_t)nc, 0.0);
     hipLaunchKernelGGL(k_colsums, dim3((nc + 255) / 256, (unsigned)nch), dim3(256), 0, s, a, n, nc,
                        nc, chunk, s1, s2);
-    hipLaunchKernelGGL(k_colfinal, g1(nc), dim3(256), 0, s, (const double*)s1, (const double*)s2,
-                       nc, n, mu, sd, inv);
+    hipLaunchKernelGGL(k_colfinal, g1(nc), dim3(256), 0, s, a, (const double*)s1,
+                       (const double*)s2, nc, n, mu, sd, inv, floor);
     return hipGetLastError() == hipSuccess ? 0 : fail(UMWM_ERR_HIP, "surrogate: statistics launch");
   };
-  int rc = one(S.feat, S.nin, S.fmu, S.fis, 1);
-  if (!rc) rc = one(S.targ, S.nout, S.tmu, S.tsd, 0);
+  int rc = one(S.feat, S.nin, S.fmu, S.fis, 1, kSurInputSdFloor);
+  if (!rc) rc = one(S.targ, S.nout, S.tmu, S.tsd, 0, kSurTargetSdFloor);
   // training range of wind speed and depth (host pass over two columns)
   std::vector<float> uv((size_t)n * 3);
   if (!rc) {
@@ -264,19 +237,27 @@ int surrogate_stats(SurrogateImpl& S, int64_t n) {
 
 int launch_surrogate_inference(DomainImpl<float>& D, hipStream_t s) {
   SurrogateImpl& S = *D.sur;
-  const int nb = S.nout, K0 = S.net.L[0].K, im = D.h.im;
+  const int nb = S.nout, im = D.h.im;
   for (int j0 = 0; j0 < im; j0 += S.chunk) {
     const int rows = std::min(S.chunk, im - j0);
-    hipLaunchKernelGGL(k_sur_point<float>, g1(rows), dim3(256), 0, s, D.h, j0, rows, nb, K0,
+    hipLaunchKernelGGL(k_sur_point<float>, g1(rows), dim3(256), 0, s, D.h, j0, rows, nb,
                        (const float*)S.fmu, (const float*)S.fis, (const float*)S.range_d,
-                       (float)D.sur_margin, S.net.x, D.h.sur_ok);
-    hipLaunchKernelGGL(k_sur_tile<float>, dim3((rows + 63) / 64, (nb + 63) / 64), dim3(256), 0, s,
-                       D.h, j0, rows, nb, K0, (const float*)S.fmu, (const float*)S.fis,
-                       (float)D.sur_zmax, S.net.x, D.h.sur_ok);
+                       (float)D.sur_margin, S.ext, (int64_t)S.chunk, D.h.sur_ok);
     UMWM_HIP_TRY(hipGetLastError());
+    // the first layer reads the spectrum itself (ln e, normalised, with the
+    // zmax check), the last writes dm^2 into the [bin][point] array
+    SpecOperand a;
+    a.e = D.h.e + j0;
+    a.lde = (int64_t)D.h.ldi;
+    a.nb = nb;
+    a.mu = S.fmu;
+    a.is = S.fis;
+    a.ext = S.ext;
+    a.ldx = S.chunk;
+    a.zmax = (float)D.sur_zmax;
+    a.ok = D.h.sur_ok + j0;
     bf16* out = reinterpret_cast<bf16*>(const_cast<uint16_t*>(D.h.sur_r)) + j0;
-    if (int rc = mlp_infer_transposed(S.net, rows, out, D.h.ldi, s, S.tmu, S.tsd, S.cscale))
-      return rc;
+    if (int rc = mlp_infer_spectrum(S.net, a, rows, out, D.h.ldi, s, S.tmu, S.tsd)) return rc;
   }
   return 0;
 }
@@ -284,8 +265,10 @@ int launch_surrogate_inference(DomainImpl<float>& D, hipStream_t s) {
 // Appends one sample per sea point of the domain at its current state.
 template <typename T>
 int surrogate_collect(SurrogateImpl& S, DomainImpl<T>& D, int64_t* total) {
-  if (D.h.om != S.om || D.h.pm != S.pm)
-    return fail(UMWM_ERR_ARGUMENT, "surrogate_collect: the domain's om x pm differs from the network's");
+  if (D.h.om != S.om || D.h.pm != S.pm || D.S.P.fmin != S.fmin || D.S.P.fmax != S.fmax)
+    return fail(UMWM_ERR_ARGUMENT,
+                "surrogate_collect: the domain's spectral grid (om, pm, fmin, fmax) differs "
+                "from the network's");
   if (!D.ops_ready) return fail(UMWM_ERR_STATE, "surrogate_collect: operator arrays missing");
   int rc = launch_op_sin(D);
   if (!rc) rc = launch_op_sds(D);
@@ -294,7 +277,7 @@ int surrogate_collect(SurrogateImpl& S, DomainImpl<T>& D, int64_t* total) {
   const int64_t take = std::min<int64_t>(D.h.im, S.capacity - S.count);
   if (take > 0) {
     hipLaunchKernelGGL(k_sur_collect<T>, g1(D.h.im), dim3(256), 0, D.stream, D.h, S.feat, S.targ,
-                       S.count, S.capacity, S.nin, S.nout, (const float*)S.cscale);
+                       S.count, S.capacity, S.nin, S.nout);
     UMWM_HIP_TRY(hipGetLastError());
     S.count += take;
     S.stats_ready = false;
@@ -372,9 +355,11 @@ extern "C" {
 int umwm_hip_surrogate_create(const umwm_surrogate_config* cfg, umwm_surrogate** out) {
   if (!cfg || !out) return fail(UMWM_ERR_ARGUMENT, "null argument");
   if (cfg->om < 3 || cfg->pm < 4 || cfg->hidden < 8 || cfg->layers < 1 || cfg->layers > 8 ||
-      cfg->batch < 8 || cfg->batch % 8 || cfg->capacity < 0 || cfg->chunk < 64)
+      cfg->batch < 8 || cfg->batch % 8 || cfg->capacity < 0 || cfg->chunk < 64 ||
+      cfg->chunk % 64)
     return fail(UMWM_ERR_ARGUMENT, "surrogate config: om >= 3, pm >= 4, hidden >= 8, layers "
-                                   "1..8, batch a positive multiple of 8, chunk >= 64");
+                                   "1..8, batch a positive multiple of 8, chunk a positive "
+                                   "multiple of 64");
   auto* S = new SurrogateImpl();
   S->om = cfg->om;
   S->pm = cfg->pm;
@@ -386,6 +371,8 @@ int umwm_hip_surrogate_create(const umwm_surrogate_config* cfg, umwm_surrogate**
   S->chunk = cfg->chunk;
   S->capacity = cfg->capacity;
   S->seed = cfg->seed;
+  S->fmin = cfg->fmin > 0 ? cfg->fmin : 0.0313;
+  S->fmax = cfg->fmax > 0 ? cfg->fmax : 2.0;
   auto bail = [&](int rc) {
     delete S;
     return rc;
@@ -404,22 +391,9 @@ int umwm_hip_surrogate_create(const umwm_surrogate_config* cfg, umwm_surrogate**
   S->tmu = salloc<float>(*S, S->nout);
   S->tsd = salloc<float>(*S, S->nout);
   S->range_d = salloc<float>(*S, 8);
-  S->freq = salloc<float>(*S, S->om);
-  S->cscale = salloc<float>(*S, S->nout);
   S->tbatch = salloc<float>(*S, (size_t)S->batch * S->net.L.back().N);
-  if (!S->tbatch) return bail(fail(UMWM_ERR_NOMEM, "surrogate: device allocation failed"));
-  // per output bin 2 pi f_o s0 on the model's frequency grid
-  // (src/umwm_init.F90:1254-1262: f_o = exp(ln fmin + o dlnf))
-  const double fmin = cfg->fmin > 0 ? cfg->fmin : 0.0313, fmax = cfg->fmax > 0 ? cfg->fmax : 2.0;
-  const double dlnf = (std::log(fmax) - std::log(fmin)) / (S->om - 1);
-  std::vector<float> cs(S->nout), fr(S->om);
-  for (int o = 0; o < S->om; ++o) {
-    fr[o] = (float)std::exp(std::log(fmin) + o * dlnf);
-    for (int p = 0; p < S->pm; ++p) cs[o * S->pm + p] = (float)(6.283185307179586 * fr[o] * kSurS0);
-  }
-  if (hipMemcpy(S->cscale, cs.data(), cs.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(S->freq, fr.data(), fr.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
-    return bail(fail(UMWM_ERR_HIP, "surrogate: upload failed"));
+  S->ext = salloc<float>(*S, (size_t)3 * S->chunk);
+  if (!S->tbatch || !S->ext) return bail(fail(UMWM_ERR_NOMEM, "surrogate: device allocation failed"));
   *out = new umwm_surrogate{S};
   return 0;
 }
@@ -461,6 +435,30 @@ int umwm_hip_surrogate_forward(umwm_surrogate* s, const float* x, int32_t n, flo
   UMWM_HIP_TRY(hipStreamSynchronize(S.stream));
   UMWM_HIP_TRY(hipMemcpy2D(y, S.nout * sizeof(float), S.net.y, N3 * sizeof(float),
                            S.nout * sizeof(float), n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int umwm_hip_surrogate_infer(umwm_surrogate* s, const float* x, int32_t n, float* y) {
+  if (int rc = sur_check(s)) return rc;
+  SurrogateImpl& S = *s->impl;
+  if (!x || !y || n < 1 || n > S.net.cap_rows)
+    return fail(UMWM_ERR_ARGUMENT, "surrogate_infer: 1 <= n <= max(batch, chunk) rows");
+  if (int rc = put_inputs(S, x, n)) return rc;
+  const int ldo = (n + 7) / 8 * 8;
+  bf16* out = nullptr;
+  UMWM_HIP_TRY(hipMalloc(&out, (size_t)S.nout * ldo * sizeof(bf16)));
+  int rc = mlp_infer_transposed(S.net, n, out, ldo, S.stream);
+  std::vector<uint16_t> h((size_t)S.nout * ldo);
+  if (!rc && hipStreamSynchronize(S.stream) != hipSuccess) rc = fail(UMWM_ERR_HIP, "surrogate_infer");
+  if (!rc && hipMemcpy(h.data(), out, h.size() * 2, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(UMWM_ERR_HIP, "surrogate_infer: download");
+  (void)hipFree(out);
+  if (rc) return rc;
+  for (int r = 0; r < n; ++r)
+    for (int o = 0; o < S.nout; ++o) {
+      const uint32_t bits = (uint32_t)h[(size_t)o * ldo + r] << 16;
+      std::memcpy(y + (size_t)r * S.nout + o, &bits, 4);
+    }
   return 0;
 }
 
@@ -550,6 +548,21 @@ int umwm_hip_surrogate_train(umwm_surrogate* s, int32_t steps, double lr, int64_
   }
   if (loss_first) *loss_first = a / w;
   if (loss_last) *loss_last = b / w;
+  return 0;
+}
+
+int umwm_hip_surrogate_get_norm(umwm_surrogate* s, float* in_mean, float* in_inv_sd,
+                                float* out_mean, float* out_sd) {
+  if (int rc = sur_check(s)) return rc;
+  SurrogateImpl& S = *s->impl;
+  if (!S.stats_ready) return fail(UMWM_ERR_STATE, "surrogate_get_norm: train first");
+  const std::pair<float*, const float*> cp[4] = {
+      {in_mean, S.fmu}, {in_inv_sd, S.fis}, {out_mean, S.tmu}, {out_sd, S.tsd}};
+  for (int i = 0; i < 4; ++i) {
+    if (!cp[i].first) continue;
+    const size_t n = i < 2 ? S.nin : S.nout;
+    UMWM_HIP_TRY(hipMemcpy(cp[i].first, cp[i].second, n * sizeof(float), hipMemcpyDeviceToHost));
+  }
   return 0;
 }
 

@@ -1,17 +1,26 @@
 // Learned source-term surrogate: a network (mlp.hpp) that maps one sea
-// point's spectrum and forcing to the combined rate of wind input, whitecap
-// dissipation and bottom friction, R = ssin - sds - sbf (1/s per bin; sds
-// after its cothkd factor, as src/umwm_physics.F90:59-68 combines them).
+// point's spectrum and forcing to the one non-local factor of the combined
+// source rate src/umwm_physics.F90:59-68 integrates, R = ssin - sds - sbf.
+// Whitecap dissipation is sds = dm^2 cf coth (e k^4)^p, a closed form of
+// the bin itself times the directional spreading factor dm^2 = (1 +
+// mss_fac mss)^2, where mss is the cos^2-weighted convolution of the whole
+// spectrum below the bin's frequency (src/umwm_source_functions.F90:85-105);
+// ssin and sbf are closed forms of the bin and the forcing.  The network
+// learns ln dm^2 for every bin, and the kernels form R from it and the
+// closed forms.  Learning R itself, or ln sds, was measured to fail: R is
+// dominated by the wind input where sds is a small remainder, and ln sds
+// spans so many decades per bin that a small normalised error is a large
+// relative one (0.43 energy-weighted error in sds on a fresh seed).
 //
 // Inputs per point (om pm + 3): ln max(e, 1e-30) of every bin (bin b =
 // o pm + p), the wind components u10, v10 and ln depth, each normalised by
 // the mean and standard deviation of the training samples.  Outputs (om
-// pm): y_b = asinh(R_b / (2 pi f_o s0)), s0 = 1e-7, normalised the same
-// way; the inference epilogue undoes both and stores R itself.
+// pm): ln dm^2, normalised the same way; the inference epilogue undoes both
+// and stores dm^2 - 1 as bf16.
 //
 // Training samples come from the plain FP32 (or FP64) source kernels of
 // domains the caller runs (umwm_hip_surrogate_collect): the operator
-// kernels evaluate ssin, sds and the sbf table on the current spectrum.
+// kernels evaluate sds_d12's spreading factor on the current spectrum.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,24 +36,41 @@ namespace umwm {
 template <typename T>
 struct DomainImpl;
 
-constexpr double kSurS0 = 1e-7;
 constexpr float kSurLogFloor = 1e-30f;
+// Floor of a target column's standard deviation in the normalisation:
+// 0.005 in ln dm^2, a 0.5 % change of the spreading factor.  Columns that
+// barely vary (the lowest frequencies, whose prefix holds no energy) then
+// do not weigh their noise into the loss; the energetic bins about the
+// spectral peak, whose dm^2 stays within a few per cent of 1, keep their
+// weight.
+constexpr double kSurTargetSdFloor = 0.005;
+// Floor of an input column's standard deviation: 0.1 in ln e, a 10 % change
+// of a bin's energy.  The spectral inputs that vary at all vary by units of
+// ln e; below the floor a column's spread is rounding.
+constexpr double kSurInputSdFloor = 0.1;
+// Relative widening of the spreading-factor bounds of the residual check
+// (fused.hip k_rates): the network's dm^2 may stray this far outside [1,
+// (1 + 2 mss_fac m0)^2] in a bin before the bin counts as violating.
+constexpr float kSurBoundTol = 0.25f;
 
-// Device statistics of one attached domain, accumulated over substeps.
+// Device statistics of one attached domain, accumulated over substeps: the
+// counts (SS_POINTS .. SS_AUDITED) as unsigned 64-bit integers, the two
+// audit sums as doubles, in one array of SS_N 8-byte words.
 enum SurStat {
   SS_POINTS = 0,      // point-substeps evaluated
   SS_USED,            // of which took the surrogate's rates
   SS_FALL_RANGE,      // fell back: inputs outside the training range
-  SS_FALL_RESID,      // fell back: implied dissipation negative beyond tolerance
+  SS_FALL_RESID,      // fell back: implied spreading factor outside its bounds
   SS_FALL_AUDIT,      // fell back: audited and found wrong beyond tolerance
   SS_AUDITED,         // point-substeps audited by the plain kernel
-  SS_AUDIT_NUM,       // sum over audited points of sum_b |R - R_plain| e kdk
-  SS_AUDIT_DEN,       // ... of sum_b (|ssin| + sds_plain + sbf) e kdk
+  SS_AUDIT_NUM,       // sum over audited points of sum_b |sds_net - sds| e kdk
+  SS_AUDIT_DEN,       // ... of sum_b sds e kdk (sds_net = dm^2 from the network)
   SS_N
 };
 
 struct SurrogateImpl {
   int om = 0, pm = 0, nin = 0, nout = 0, hidden = 0, layers = 0;
+  double fmin = 0, fmax = 0;  // frequency grid the network serves
   int batch = 0, chunk = 0;
   int64_t capacity = 0, count = 0;
   uint64_t seed = 0;
@@ -62,8 +88,9 @@ struct SurrogateImpl {
   // Hs), for the out-of-range check
   float range[6] = {0, 0, 0, 0, 0, 0};
   float* range_d = nullptr;
-  // frequencies of the served grid (fp32 [om]) and per output 2 pi f s0
-  float *freq = nullptr, *cscale = nullptr;
+  // inference: the normalised forcing inputs (u10, v10, ln depth) of one
+  // chunk of points, fp32 [3][chunk]
+  float* ext = nullptr;
   // training minibatch: row indices, normalised targets
   int32_t* idx = nullptr;
   float* tbatch = nullptr;

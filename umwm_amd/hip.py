@@ -33,7 +33,9 @@ EXPORTS = [
     "umwm_hip_surrogate_create", "umwm_hip_surrogate_destroy", "umwm_hip_surrogate_dims",
     "umwm_hip_surrogate_get_layer", "umwm_hip_surrogate_set_layer", "umwm_hip_surrogate_forward",
     "umwm_hip_surrogate_gradients", "umwm_hip_surrogate_collect", "umwm_hip_surrogate_train",
-    "umwm_hip_surrogate_eval", "umwm_hip_surrogate_attach", "umwm_hip_surrogate_stats",
+    "umwm_hip_surrogate_eval", "umwm_hip_surrogate_get_norm", "umwm_hip_surrogate_attach",
+    "umwm_hip_surrogate_stats",
+    "umwm_hip_surrogate_infer", "umwm_hip_surrogate_fields",
 ]
 
 
@@ -106,8 +108,12 @@ def lib():
     L.umwm_hip_surrogate_train.argtypes = [vp, C.c_int32, C.c_double, C.c_int64, P(C.c_double),
                                            P(C.c_double)]
     L.umwm_hip_surrogate_eval.argtypes = [vp, C.c_int64, C.c_int64, P(C.c_double)]
+    L.umwm_hip_surrogate_get_norm.argtypes = [vp, P(C.c_float), P(C.c_float), P(C.c_float),
+                                              P(C.c_float)]
     L.umwm_hip_surrogate_attach.argtypes = [vp, vp, P(abi.SurrogatePolicy)]
     L.umwm_hip_surrogate_stats.argtypes = [vp, P(C.c_double), C.c_int]
+    L.umwm_hip_surrogate_infer.argtypes = [vp, fp, C.c_int32, fp]
+    L.umwm_hip_surrogate_fields.argtypes = [vp, P(C.c_double), P(C.c_double)]
     _lib = L
     return L
 
@@ -417,6 +423,14 @@ class Surrogate:
         check(lib().umwm_hip_surrogate_forward(self.h, _fptr(x), x.shape[0], _fptr(y)))
         return y
 
+    def infer(self, x):
+        """Outputs through the inference kernels (last layer transposed,
+        bfloat16)."""
+        x = _f32(x)
+        y = np.empty((x.shape[0], self.nout), np.float32)
+        check(lib().umwm_hip_surrogate_infer(self.h, _fptr(x), x.shape[0], _fptr(y)))
+        return y
+
     def gradients(self, x, t):
         x, t = _f32(x), _f32(t)
         n = x.shape[0]
@@ -441,6 +455,14 @@ class Surrogate:
                                              C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def get_norm(self):
+        """Input means and inverse standard deviations, output (ln dm^2)
+        means and standard deviations of the last training."""
+        a, b = np.empty(self.nin, np.float32), np.empty(self.nin, np.float32)
+        c, d = np.empty(self.nout, np.float32), np.empty(self.nout, np.float32)
+        check(lib().umwm_hip_surrogate_get_norm(self.h, _fptr(a), _fptr(b), _fptr(c), _fptr(d)))
+        return a, b, c, d
+
     def eval(self, first, n):
         v = C.c_double()
         check(lib().umwm_hip_surrogate_eval(self.h, int(first), int(n), C.byref(v)))
@@ -453,6 +475,18 @@ class Surrogate:
     @staticmethod
     def detach(dom):
         check(lib().umwm_hip_surrogate_attach(dom.h, None, None))
+
+    @staticmethod
+    def fields(dom):
+        """The network's spreading factor dm^2 of the last substep (im, pm,
+        om), before the kernels clamp it to its bounds, and each point's
+        choice (im,)."""
+        n = dom.im * dom.pm * dom.om
+        sds = np.empty(n)
+        use = np.empty(dom.im)
+        check(lib().umwm_hip_surrogate_fields(dom.h, sds.ctypes.data_as(C.POINTER(C.c_double)),
+                                              use.ctypes.data_as(C.POINTER(C.c_double))))
+        return sds.reshape(dom.im, dom.pm, dom.om), use
 
     @staticmethod
     def stats(dom, reset=False):
