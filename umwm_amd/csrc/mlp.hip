@@ -183,74 +183,92 @@ struct InfStage<A_SPEC> {
 };
 
 template <int AM, int EP>
-__global__ __launch_bounds__(256, 2) void k_gemm_inf(GemmArgs g, int ntm, int ntn) {
+__global__ __launch_bounds__(256, 2) void k_gemm_inf(GemmArgs g, int ntm, int ntn, int nmajor) {
   __shared__ __attribute__((aligned(16))) bf16 As[2][IBM][ILDK];
   __shared__ __attribute__((aligned(16))) bf16 Bs[2][IBN][ILDK];
-  // XCD-aware tile order (the grid is padded to a multiple of 8)
+  // XCD-aware tile order (the grid is padded to a multiple of 8): row
+  // blocks outer (the blocks of one A block adjacent) or, with nmajor,
+  // column blocks outer (the blocks of one B block adjacent)
   const int nt = ntm * ntn, per = (int)(gridDim.x / 8);
   const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
   if (t >= nt) return;
-  const int tm = t / ntn, tn = t % ntn;
+  const int tm = nmajor ? t % ntm : t / ntn, tn = nmajor ? t / ntm : t % ntn;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int m0 = tm * IBM, n0 = tn * IBN;
   const int K = g.K;
-  const bf16x8 zero = {};
-  InfStage<AM> ra;
-  bf16x8 rb[2];
+  // register stages in flight: two for bf16 operands, one for the
+  // spectrum's fp32 (two would not fit beside the accumulators)
+  constexpr int NS = 2;
+  struct Stage {
+    InfStage<AM> a;
+    bf16x8 b[2];
+  };
+  Stage st[NS];
   // A_SPEC: this thread's 4 points and (wave-uniform) 8-input group
   [[maybe_unused]] const int smc = (tid & 63) * 4;
   [[maybe_unused]] const int skg = __builtin_amdgcn_readfirstlane(tid >> 6) * 8;
   [[maybe_unused]] uint32_t bad = 0;  // zmax violations of the 4 points
-  // Every index stays inside its operand: rows are checked against M and
-  // N, k against K, a 16-byte bf16 chunk never straddles K (a multiple of
-  // 8, checked by gemm_infer()), and a float4 of the spectrum is read only
-  // where all 4 points lie below M (else point by point).
-  auto gload = [&](int k0) {
+  // Every load stays inside its operand and none is conditional (a
+  // masked load would make the compiler wait for every load in flight
+  // before the next LDS store): K is a multiple of 64 (checked by
+  // gemm_infer()), so the K tiles hold no partial 16-byte chunk; rows past
+  // M or N read row M - 1 or N - 1 again, and their results are never
+  // stored; a spectrum float4 lies wholly inside its row or wholly past it
+  // (lde, the chunk start and 4 points per thread all multiples of 4) and
+  // reads 0 there.  Points past M feed only their own rows of C.
+  // bf16 operands through buffer resources: each thread's 16-byte chunks
+  // keep a fixed 32-bit byte offset (row, column within the K tile), and the
+  // K tile's position is the scalar offset (gemm_infer() checks that the
+  // operands span less than 2 GiB)
+  [[maybe_unused]] const rsrc_t rA =
+      make_rsrc(AM == A_MK ? (const void*)g.A : (const void*)g.B,
+                AM == A_MK ? (uint32_t)((int64_t)g.M * g.lda * 2) : 0u);
+  const rsrc_t rB = make_rsrc(g.B, (uint32_t)((int64_t)g.N * g.ldb * 2));
+  [[maybe_unused]] uint32_t offA[4];
+  uint32_t offB[2];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int q = tid + c * 256;
+    if constexpr (AM == A_MK)
+      offA[c] = (uint32_t)(((int64_t)min(m0 + (q >> 2), g.M - 1) * g.lda + (q & 3) * 8) * 2);
+    if (c < 2) offB[c] = (uint32_t)(((int64_t)min(n0 + (q >> 2), g.N - 1) * g.ldb + (q & 3) * 8) * 2);
+  }
+  auto gload = [&](Stage& S, int k0) {
     if constexpr (AM == A_MK) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int q = tid + c * 256, r = q >> 2, kc = (q & 3) * 8, m = m0 + r, k = k0 + kc;
-        ra.a[c] = (m < g.M && k < K) ? *(const bf16x8*)(g.A + (int64_t)m * g.lda + k) : zero;
-      }
+      for (int c = 0; c < 4; ++c)
+        S.a.a[c] = __builtin_bit_cast(bf16x8, bld16<float>(rA, offA[c], (uint32_t)k0 * 2u));
     } else {
       // one buffer resource per input row (scalar registers: the row is
-      // wave-uniform) covering this block's points below M; reads past M
-      // return 0
+      // wave-uniform) covering this block's columns of the row; a float4
+      // past the row's end (the last row of the spectrum only) reads 0
       const SpecOperand& sa = g.sa;
       const int kb = k0 + skg;
-      const bool full = m0 + smc + 4 <= g.M;
-      const uint32_t bytes = (uint32_t)std::min(g.M - m0, IBM) * 4u, voff = (uint32_t)smc * 4u;
+      const uint32_t voff = (uint32_t)smc * 4u;
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
+        // rows past the inputs (K padding) get an empty resource: no
+        // branch, the load reads 0
         const int k = kb + i;
-        f32x4 v = {0.f, 0.f, 0.f, 0.f};
-        const float* row = k < sa.nb ? sa.e + (int64_t)k * sa.lde
-                                     : sa.ext + (int64_t)(k - sa.nb) * sa.ldx;
-        if (k < sa.nb + 3) {
-          const rsrc_t r = make_rsrc(row + m0, bytes);
-          if (full) {
-            v = bld16<float>(r, voff);
-          } else {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) v[c] = bld<float>(r, voff + 4u * c);
-          }
-        }
-        ra.a[i] = v;
+        const bool spec = k < sa.nb, in = k < sa.nb + 3;
+        const int kk = in ? k : sa.nb;
+        const float* row = spec ? sa.e + (int64_t)k * sa.lde : sa.ext + (int64_t)(kk - sa.nb) * sa.ldx;
+        const int cols = spec ? sa.cols : (int)sa.ldx;
+        const uint32_t bytes = in ? (uint32_t)std::min(cols - m0, IBM) * 4u : 0u;
+        S.a.a[i] = bld16<float>(make_rsrc(row + m0, bytes), voff);
       }
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int q = tid + c * 256, r = q >> 2, kc = (q & 3) * 8, n = n0 + r, k = k0 + kc;
-      rb[c] = (n < g.N && k < K) ? *(const bf16x8*)(g.B + (int64_t)n * g.ldb + k) : zero;
-    }
+    for (int c = 0; c < 2; ++c)
+      S.b[c] = __builtin_bit_cast(bf16x8, bld16<float>(rB, offB[c], (uint32_t)k0 * 2u));
   };
-  auto sstore = [&](int buf, int k0) {
+  auto sstore = [&](const Stage& S, int buf, int k0) {
     if constexpr (AM == A_MK) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int q = tid + c * 256;
-        *(bf16x8*)&As[buf][q >> 2][(q & 3) * 8] = ra.a[c];
+        *(bf16x8*)&As[buf][q >> 2][(q & 3) * 8] = S.a.a[c];
       }
     } else {
       const SpecOperand& sa = g.sa;
@@ -259,7 +277,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_inf(GemmArgs g, int ntm, int nt
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const int k = kb + i;
-        const f32x4 v = ra.a[i];
+        const f32x4 v = S.a.a[i];
         float x[4] = {v[0], v[1], v[2], v[3]};
         if (k < sa.nb) {
           const float mu = sa.mu[k], is = sa.is[k];
@@ -278,7 +296,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_inf(GemmArgs g, int ntm, int nt
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       const int q = tid + c * 256;
-      *(bf16x8*)&Bs[buf][q >> 2][(q & 3) * 8] = rb[c];
+      *(bf16x8*)&Bs[buf][q >> 2][(q & 3) * 8] = S.b[c];
     }
   };
   f32x16 acc[4][2];
@@ -288,13 +306,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_inf(GemmArgs g, int ntm, int nt
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.f;
-  int buf = 0;
-  gload(0);
-  sstore(0, 0);
-  __syncthreads();
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    const bool more = k0 + BK < K;
-    if (more) gload(k0 + BK);
+  auto mfma = [&](int buf) {
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
       const int kk = ks * 16 + (lane >> 5) * 8;
@@ -309,9 +321,44 @@ __global__ __launch_bounds__(256, 2) void k_gemm_inf(GemmArgs g, int ntm, int nt
         for (int b = 0; b < 2; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
     }
-    if (more) sstore(buf ^ 1, k0 + BK);
+  };
+  // K tiles 0 .. nkt-1 of BK; tile u lives in LDS buffer u % 2.  Loads and
+  // stores are unconditional (tile indices past the end are clamped to the
+  // last tile, whose reload and restore change nothing): a load under a
+  // branch leaves the compiler unsure which stage's loads are the newest,
+  // and it then waits for all of them before each LDS store.
+  const int nkt = K / BK, last = nkt - 1;
+  auto tile = [&](int u) { return min(u, last) * BK; };
+  gload(st[0], 0);
+  sstore(st[0], 0, 0);
+  if constexpr (NS == 2) {
+    // odd tiles wait in st[0], even ones in st[1]; each pair of steps
+    // stores the next tile, refills its stage three tiles ahead and
+    // multiplies the current one (nkt is even: K a multiple of 64)
+    gload(st[0], tile(1));
+    gload(st[1], tile(2));
     __syncthreads();
-    buf ^= 1;
+#pragma unroll 1
+    for (int u = 0; u + 1 < nkt; u += 2) {
+      sstore(st[0], 1, tile(u + 1));
+      gload(st[0], tile(u + 3));
+      mfma(0);
+      __syncthreads();
+      sstore(st[1], 0, tile(u + 2));
+      gload(st[1], tile(u + 4));
+      mfma(1);
+      __syncthreads();
+    }
+  } else {
+    __syncthreads();
+#pragma unroll 1
+    for (int u = 0; u < nkt; ++u) {
+      const int buf = u & 1;
+      gload(st[0], tile(u + 1));
+      mfma(buf);
+      sstore(st[0], buf ^ 1, tile(u + 1));
+      __syncthreads();
+    }
   }
   if constexpr (AM == A_SPEC) {
     // the first column of blocks reports each point's zmax verdict
@@ -321,26 +368,60 @@ __global__ __launch_bounds__(256, 2) void k_gemm_inf(GemmArgs g, int ntm, int nt
         if (((bad >> c) & 1u) && m0 + smc + c < g.M) atomicAnd(g.sa.ok + m0 + smc + c, 0);
     }
   }
+  // Epilogue through LDS: each wave writes its 128 x 64 part of C as two
+  // 64 x 64 halves into its own LDS region (the MFMA layout puts one column
+  // per lane), then reads rows back 16 bytes per lane, so that every store
+  // to C is a whole 128-byte row segment.  Scattered 2-byte stores of the
+  // MFMA layout ran at about 0.6 TB/s and made the output-heavy last layer
+  // store-bound.
+  __syncthreads();  // all waves done with As and Bs
+  constexpr int SLD = 64 + 8;  // staging row, padded
+  bf16(*stg)[SLD] = reinterpret_cast<bf16(*)[SLD]>(&As[0][0][0]) + w * 64;
+  static_assert(4 * 64 * SLD * sizeof(bf16) <= sizeof(As), "staging fits in As");
+  bf16* const C = static_cast<bf16*>(g.C);
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int h = 0; h < 2; ++h) {
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int aa = 0; aa < 2; ++aa)
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int m = m0 + wm * 128 + a * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-        const int n = n0 + wn * 64 + b * 32 + (lane & 31);
-        if (m >= g.M || n >= g.N) continue;
-        const float v = acc[a][b][i];
-        const int64_t o = (int64_t)m * g.ldc + n;
-        if constexpr (EP == EP_GELU) {
-          ((bf16*)g.C)[o] = (bf16)gelu_f(v + g.bias[n]);
-        } else if constexpr (EP == EP_BF16_BM) {
-          ((bf16*)g.C)[o] = (bf16)(v + g.bias[m]);
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const int r = aa * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), c = b * 32 + (lane & 31);
+          // indices of the epilogue's vectors, clamped: rows past M and
+          // columns past N are staged but never stored
+          const int m = min(m0 + wm * 128 + h * 64 + r, g.M - 1), n = min(n0 + wn * 64 + c, g.N - 1);
+          const float v = acc[2 * h + aa][b][i];
+          float y;
+          if constexpr (EP == EP_GELU) {
+            y = gelu_f(v + g.bias[n]);
+          } else if constexpr (EP == EP_BF16_BM) {
+            y = v + g.bias[m];
+          } else {
+            static_assert(EP == EP_SUR, "inference epilogues: GELU, BF16_BM, SUR");
+            y = expm1f(fminf(g.osd[m] * (v + g.bias[m]) + g.omu[m], 30.f));
+          }
+          stg[r][c] = (bf16)y;
+        }
+    // LDS accesses of one wave complete in order: the rows read below are
+    // the ones written above, by any lane of the wave
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const int q = it * 64 + lane, r = q >> 3, cc = (q & 7) * 8;
+      const int m = m0 + wm * 128 + h * 64 + r, n = n0 + wn * 64 + cc;
+      const bf16x8 v = *(const bf16x8*)&stg[r][cc];
+      if (m < g.M) {
+        bf16* dst = C + (int64_t)m * g.ldc + n;
+        if (n + 8 <= g.N) {
+          *(bf16x8*)dst = v;
         } else {
-          static_assert(EP == EP_SUR, "inference epilogues: GELU, BF16_BM, SUR");
-          ((bf16*)g.C)[o] = (bf16)expm1f(fminf(g.osd[m] * (v + g.bias[m]) + g.omu[m], 30.f));
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            if (n + e < g.N) dst[e] = v[e];
         }
       }
+    }
+  }
 }
 
 // gw[i] = sum over s of part[s][i] (deterministic split-K reduction).
@@ -451,19 +532,29 @@ int gemm(int alay, int blay, int ep, const GemmArgs& g0, int splits, hipStream_t
 
 int gemm_infer(int amode, int ep, const GemmArgs& g, hipStream_t s) {
   if (g.M <= 0 || g.N <= 0 || g.K <= 0) return 0;
-  if (g.K % 8 || g.ldb % 8 || (amode == A_MK && g.lda % 8))
-    return fail(UMWM_ERR_ARGUMENT, "gemm_infer: K, lda and ldb must be multiples of 8");
+  if (g.K % (2 * BK) || g.ldb % 8 || g.ldc % 8 || (amode == A_MK && g.lda % 8) ||
+      reinterpret_cast<uintptr_t>(g.C) % 16)
+    return fail(UMWM_ERR_ARGUMENT, "gemm_infer: K a multiple of 64, lda, ldb and ldc of 8, C "
+                                   "16-byte aligned");
+  if ((int64_t)g.N * g.ldb * 2 >= (1ll << 31) || (amode == A_MK && (int64_t)g.M * g.lda * 2 >= (1ll << 31)))
+    return fail(UMWM_ERR_SIZE, "gemm_infer: an operand spans 2 GiB or more; use smaller chunks");
   if (amode == A_SPEC) {
     const SpecOperand& a = g.sa;
     if (!a.e || !a.mu || !a.is || !a.ext || !a.ok || a.nb < 1 || a.nb + 3 > g.K ||
-        a.lde % 4 || reinterpret_cast<uintptr_t>(a.e) % 16)
+        a.lde % 4 || a.ldx % 4 || a.cols < g.M || a.cols % 4 || a.ldx < g.M ||
+        reinterpret_cast<uintptr_t>(a.e) % 16 || reinterpret_cast<uintptr_t>(a.ext) % 16)
       return fail(UMWM_ERR_ARGUMENT, "gemm_infer: bad spectrum operand");
   }
   const int ntm = (g.M + IBM - 1) / IBM, ntn = (g.N + IBN - 1) / IBN;
   const int64_t nt = (int64_t)ntm * ntn;
   if (nt > (1ll << 30)) return fail(UMWM_ERR_SIZE, "gemm_infer: too many tiles");
   const dim3 grid((unsigned)((nt + 7) / 8 * 8));
-#define UMWM_INF(AM_, EP_) hipLaunchKernelGGL((k_gemm_inf<AM_, EP_>), grid, dim3(256), 0, s, g, ntm, ntn)
+  // the transposed last layer (EP_SUR, EP_BF16_BM) has few row blocks (the
+  // outputs) and many column blocks (the points): column blocks outer, so
+  // that the row blocks of one activation block share it in L2
+  const int nmajor = ep == EP_GELU ? 0 : 1;
+#define UMWM_INF(AM_, EP_) \
+  hipLaunchKernelGGL((k_gemm_inf<AM_, EP_>), grid, dim3(256), 0, s, g, ntm, ntn, nmajor)
   if (amode == A_MK && ep == EP_GELU) UMWM_INF(A_MK, EP_GELU);
   else if (amode == A_SPEC && ep == EP_GELU) UMWM_INF(A_SPEC, EP_GELU);
   else if (amode == A_MK && ep == EP_SUR) UMWM_INF(A_MK, EP_SUR);
@@ -488,6 +579,7 @@ U* dalloc(Mlp& m, size_t n) {
   return static_cast<U*>(p);
 }
 inline int pad8(int v) { return (v + 7) / 8 * 8; }
+inline int pad64(int v) { return (v + 63) / 64 * 64; }
 }  // namespace
 
 int mlp_create(const std::vector<int>& dims, uint64_t seed, int rows, Mlp& m) {
@@ -501,8 +593,11 @@ int mlp_create(const std::vector<int>& dims, uint64_t seed, int rows, Mlp& m) {
   size_t part = 0;
   for (int l = 0; l < nl; ++l) {
     MlpLayerDev L{};
-    L.K = pad8(dims[l]);
-    L.N = pad8(dims[l + 1]);
+    // inputs padded to an even number of 32-wide K tiles of the inference
+    // GEMM (a hidden layer's outputs are the next layer's inputs), the
+    // network's outputs to 8
+    L.K = pad64(dims[l]);
+    L.N = l + 1 < nl ? pad64(dims[l + 1]) : pad8(dims[l + 1]);
     const size_t nw = (size_t)L.N * L.K;
     L.w = dalloc<float>(m, nw); L.b = dalloc<float>(m, L.N); L.wb = dalloc<bf16>(m, nw);
     L.gw = dalloc<float>(m, nw); L.gb = dalloc<float>(m, L.N);

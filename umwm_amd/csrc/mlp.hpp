@@ -5,9 +5,9 @@
 //
 // Layer l maps K_l inputs to N_l outputs, z = W_l x + b_l with W_l stored
 // [N_l][K_l] (row per output, inputs contiguous); hidden layers apply the
-// tanh form of GELU, the last layer is linear.  Dimensions are padded to
-// multiples of 8 (the padding rows and columns of W are zero and stay
-// zero: their gradients are zero).
+// tanh form of GELU, the last layer is linear.  Layer inputs are padded to
+// multiples of 64, the network's outputs to 8 (the padding rows and
+// columns of W are zero and stay zero: their gradients are zero).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -48,6 +48,7 @@ enum GemmEpilogue { EP_GELU = 0, EP_F32 = 1, EP_BF16_BM = 2, EP_DGELU = 3, EP_PA
 struct SpecOperand {
   const float* e = nullptr;
   int64_t lde = 0;
+  int cols = 0;  // columns of a row from e on (>= M, a multiple of 4)
   int nb = 0;
   const float *mu = nullptr, *is = nullptr;
   const float* ext = nullptr;

@@ -249,6 +249,7 @@ int launch_surrogate_inference(DomainImpl<float>& D, hipStream_t s) {
     SpecOperand a;
     a.e = D.h.e + j0;
     a.lde = (int64_t)D.h.ldi;
+    a.cols = (int)(D.h.ldi - j0);
     a.nb = nb;
     a.mu = S.fmu;
     a.is = S.fis;
