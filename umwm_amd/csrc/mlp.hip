@@ -155,37 +155,49 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
       }
 }
 
-// Inference GEMM.  A workgroup of 4 waves computes a 256 x 128 block of C;
-// wave (wm, wn) owns a 128 x 64 part as 4 x 2 tiles of the 32x32x16 MFMA,
-// so that each 16-byte LDS read feeds 8/6 of an MFMA (the training GEMM's
-// 2 x 2 wave tiles feed one).  The K loop steps by 32 with the next tile
-// staged in registers, as in k_gemm.  Blocks are numbered so that the
-// N/128 blocks sharing one A block land on the same XCD back to back
-// (workgroups go round-robin over the 8 XCDs): the A block, the large
-// operand, then comes from that XCD's L2 after the first read.
+// Inference GEMM.  A workgroup of 4 waves (2 x 2) computes a BMT x BNT
+// block of C, each wave a (BMT/2) x (BNT/2) part as TA x TB tiles of the
+// 32x32x16 MFMA (8 tiles: each 16-byte LDS read feeds 8/6 of an MFMA, the
+// training GEMM's 2 x 2 wave tiles feed one).  bf16 A operands use 256 x
+// 128 blocks; the spectrum operand uses 128 x 256, so that its ln /
+// normalise / bf16 transform, done once per column block, is repeated
+// over 2 column blocks of the 512 hidden outputs rather than 4.  The K
+// loop steps by 32 with two tiles staged in registers ahead.  Blocks are
+// numbered so that the blocks sharing one A block (or, for the transposed
+// last layer, one B block) land on the same XCD back to back (workgroups
+// go round-robin over the 8 XCDs): the shared operand then comes from that
+// XCD's L2 after the first read.
 //
 // A_SPEC stages the first layer's input from the spectrum (fp32, points
-// contiguous): each thread loads 4 points x 8 inputs as 8 float4 reads
-// (a wave reads 1 KB contiguous per input row), forms ln e, normalises,
-// rounds to bf16 and writes each point's 8 inputs as one 16-byte LDS row
-// segment.
-constexpr int IBM = 256, IBN = 128, ILDK = BK + 8;
+// contiguous): each thread loads BMT/64 points x 8 inputs as 8 vector
+// reads (a wave reads BMT x 4 bytes contiguous per input row), forms
+// ln e, normalises, rounds to bf16 and writes each point's 8 inputs as
+// one 16-byte LDS row segment.
+constexpr int ILDK = BK + 8;
 
-template <int AM>
+template <int AM, int BMT>
 struct InfStage;
-template <>
-struct InfStage<A_MK> {
-  bf16x8 a[4];
+template <int BMT>
+struct InfStage<A_MK, BMT> {
+  bf16x8 a[BMT / 64];
 };
-template <>
-struct InfStage<A_SPEC> {
-  f32x4 a[8];
+template <int BMT>
+struct InfStage<A_SPEC, BMT> {
+  using V = float __attribute__((ext_vector_type(BMT / 64)));
+  V a[8];
 };
 
-template <int AM, int EP>
+template <int AM, int EP, int BMT, int BNT>
 __global__ __launch_bounds__(256, 2) void k_gemm_inf(GemmArgs g, int ntm, int ntn, int nmajor) {
-  __shared__ __attribute__((aligned(16))) bf16 As[2][IBM][ILDK];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[2][IBN][ILDK];
+  constexpr int TA = BMT / 64, TB = BNT / 64;  // MFMA tiles per wave
+  constexpr int CA = BMT / 64, CB = BNT / 64;  // 16-byte bf16 chunks per thread per K tile
+  constexpr int PPT = BMT / 64;                // A_SPEC: points per thread
+  static_assert(TA * TB == 8 && (BMT == 256 || BMT == 128), "block shapes 256x128, 128x256");
+  // one LDS array: the A and B tiles of the K loop, then the epilogue's
+  // staging rows
+  __shared__ __attribute__((aligned(16))) bf16 lds[2 * (BMT + BNT) * ILDK];
+  bf16(*const As)[BMT][ILDK] = reinterpret_cast<bf16(*)[BMT][ILDK]>(lds);
+  bf16(*const Bs)[BNT][ILDK] = reinterpret_cast<bf16(*)[BNT][ILDK]>(lds + 2 * BMT * ILDK);
   // XCD-aware tile order (the grid is padded to a multiple of 8): row
   // blocks outer (the blocks of one A block adjacent) or, with nmajor,
   // column blocks outer (the blocks of one B block adjacent)
@@ -195,28 +207,27 @@ __global__ __launch_bounds__(256, 2) void k_gemm_inf(GemmArgs g, int ntm, int nt
   const int tm = nmajor ? t % ntm : t / ntn, tn = nmajor ? t / ntm : t % ntn;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
-  const int m0 = tm * IBM, n0 = tn * IBN;
+  const int m0 = tm * BMT, n0 = tn * BNT;
   const int K = g.K;
-  // register stages in flight: two for bf16 operands, one for the
-  // spectrum's fp32 (two would not fit beside the accumulators)
-  constexpr int NS = 2;
   struct Stage {
-    InfStage<AM> a;
-    bf16x8 b[2];
+    InfStage<AM, BMT> a;
+    bf16x8 b[CB];
   };
-  Stage st[NS];
-  // A_SPEC: this thread's 4 points and (wave-uniform) 8-input group
-  [[maybe_unused]] const int smc = (tid & 63) * 4;
+  Stage st[2];  // two K tiles in flight
+  // A_SPEC: this thread's PPT points and (wave-uniform) 8-input group
+  [[maybe_unused]] const int smc = (tid & 63) * PPT;
   [[maybe_unused]] const int skg = __builtin_amdgcn_readfirstlane(tid >> 6) * 8;
-  [[maybe_unused]] uint32_t bad = 0;  // zmax violations of the 4 points
+  // A_SPEC: the largest |normalised input| of each of its points
+  [[maybe_unused]] float zabs[PPT] = {};
   // Every load stays inside its operand and none is conditional (a
   // masked load would make the compiler wait for every load in flight
   // before the next LDS store): K is a multiple of 64 (checked by
   // gemm_infer()), so the K tiles hold no partial 16-byte chunk; rows past
   // M or N read row M - 1 or N - 1 again, and their results are never
-  // stored; a spectrum float4 lies wholly inside its row or wholly past it
-  // (lde, the chunk start and 4 points per thread all multiples of 4) and
-  // reads 0 there.  Points past M feed only their own rows of C.
+  // stored; a spectrum vector lies wholly inside its row or wholly past it
+  // (lde, the chunk start and the points per thread all multiples of 4 or
+  // of PPT) and reads 0 there.  Points past M feed only their own rows of
+  // C.
   // bf16 operands through buffer resources: each thread's 16-byte chunks
   // keep a fixed 32-bit byte offset (row, column within the K tile), and the
   // K tile's position is the scalar offset (gemm_infer() checks that the
@@ -225,203 +236,211 @@ __global__ __launch_bounds__(256, 2) void k_gemm_inf(GemmArgs g, int ntm, int nt
       make_rsrc(AM == A_MK ? (const void*)g.A : (const void*)g.B,
                 AM == A_MK ? (uint32_t)((int64_t)g.M * g.lda * 2) : 0u);
   const rsrc_t rB = make_rsrc(g.B, (uint32_t)((int64_t)g.N * g.ldb * 2));
-  [[maybe_unused]] uint32_t offA[4];
-  uint32_t offB[2];
+  [[maybe_unused]] uint32_t offA[CA];
+  uint32_t offB[CB];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
+  for (int c = 0; c < CA; ++c) {
     const int q = tid + c * 256;
     if constexpr (AM == A_MK)
       offA[c] = (uint32_t)(((int64_t)min(m0 + (q >> 2), g.M - 1) * g.lda + (q & 3) * 8) * 2);
-    if (c < 2) offB[c] = (uint32_t)(((int64_t)min(n0 + (q >> 2), g.N - 1) * g.ldb + (q & 3) * 8) * 2);
+  }
+#pragma unroll
+  for (int c = 0; c < CB; ++c) {
+    const int q = tid + c * 256;
+    offB[c] = (uint32_t)(((int64_t)min(n0 + (q >> 2), g.N - 1) * g.ldb + (q & 3) * 8) * 2);
   }
   auto gload = [&](Stage& S, int k0) {
     if constexpr (AM == A_MK) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
+      for (int c = 0; c < CA; ++c)
         S.a.a[c] = __builtin_bit_cast(bf16x8, bld16<float>(rA, offA[c], (uint32_t)k0 * 2u));
     } else {
       // one buffer resource per input row (scalar registers: the row is
-      // wave-uniform) covering this block's columns of the row; a float4
-      // past the row's end (the last row of the spectrum only) reads 0
+      // wave-uniform) covering this block's columns of the row; a vector
+      // past the row's end (the last row of the spectrum only) reads 0,
+      // and rows past the inputs (K padding) get an empty resource
       const SpecOperand& sa = g.sa;
       const int kb = k0 + skg;
       const uint32_t voff = (uint32_t)smc * 4u;
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        // rows past the inputs (K padding) get an empty resource: no
-        // branch, the load reads 0
         const int k = kb + i;
         const bool spec = k < sa.nb, in = k < sa.nb + 3;
         const int kk = in ? k : sa.nb;
         const float* row = spec ? sa.e + (int64_t)k * sa.lde : sa.ext + (int64_t)(kk - sa.nb) * sa.ldx;
         const int cols = spec ? sa.cols : (int)sa.ldx;
-        const uint32_t bytes = in ? (uint32_t)std::min(cols - m0, IBM) * 4u : 0u;
-        S.a.a[i] = bld16<float>(make_rsrc(row + m0, bytes), voff);
+        const rsrc_t r = make_rsrc(row + m0, in ? (uint32_t)std::min(cols - m0, BMT) * 4u : 0u);
+        if constexpr (PPT == 4) {
+          S.a.a[i] = bld16<float>(r, voff);
+        } else {
+          S.a.a[i] = __builtin_bit_cast(typename InfStage<AM, BMT>::V,
+                                        __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, 0, 0));
+        }
       }
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
+    for (int c = 0; c < CB; ++c)
       S.b[c] = __builtin_bit_cast(bf16x8, bld16<float>(rB, offB[c], (uint32_t)k0 * 2u));
   };
   auto sstore = [&](const Stage& S, int buf, int k0) {
     if constexpr (AM == A_MK) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < CA; ++c) {
         const int q = tid + c * 256;
         *(bf16x8*)&As[buf][q >> 2][(q & 3) * 8] = S.a.a[c];
       }
     } else {
       const SpecOperand& sa = g.sa;
       const int kb = k0 + skg;
-      bf16x8 o[4];
+      bf16x8 o[PPT];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const int k = kb + i;
-        const f32x4 v = S.a.a[i];
-        float x[4] = {v[0], v[1], v[2], v[3]};
-        if (k < sa.nb) {
-          const float mu = sa.mu[k], is = sa.is[k];
+        float x[PPT];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            x[c] = (__logf(fmaxf(x[c], 1e-30f)) - mu) * is;
-            bad |= (fabsf(x[c]) > sa.zmax ? 1u : 0u) << c;
+        for (int c = 0; c < PPT; ++c) x[c] = S.a.a[i][c];
+        if (k < sa.nb) {
+          // (ln e - mu) is = log2(e) (ln 2 is) - mu is: the hardware log2
+          // and one fma per input (wave-uniform constants per k)
+          const float is = sa.is[k], a = is * 0.6931471805599453f, c0 = -sa.mu[k] * is;
+#pragma unroll
+          for (int c = 0; c < PPT; ++c) {
+            x[c] = fmaf(__builtin_amdgcn_logf(fmaxf(x[c], 1e-30f)), a, c0);
+            zabs[c] = fmaxf(zabs[c], fabsf(x[c]));
           }
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) o[c][i] = (bf16)x[c];
+        for (int c = 0; c < PPT; ++c) o[c][i] = (bf16)x[c];
       }
 #pragma unroll
-      for (int c = 0; c < 4; ++c) *(bf16x8*)&As[buf][smc + c][skg] = o[c];
+      for (int c = 0; c < PPT; ++c) *(bf16x8*)&As[buf][smc + c][skg] = o[c];
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < CB; ++c) {
       const int q = tid + c * 256;
       *(bf16x8*)&Bs[buf][q >> 2][(q & 3) * 8] = S.b[c];
     }
   };
-  f32x16 acc[4][2];
+  f32x16 acc[TA][TB];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < TA; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < TB; ++b)
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.f;
   auto mfma = [&](int buf) {
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks) {
       const int kk = ks * 16 + (lane >> 5) * 8;
-      bf16x8 fa[4], fb[2];
+      bf16x8 fa[TA], fb[TB];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) fa[a] = *(const bf16x8*)&As[buf][wm * 128 + a * 32 + (lane & 31)][kk];
+      for (int a = 0; a < TA; ++a)
+        fa[a] = *(const bf16x8*)&As[buf][wm * (BMT / 2) + a * 32 + (lane & 31)][kk];
 #pragma unroll
-      for (int b = 0; b < 2; ++b) fb[b] = *(const bf16x8*)&Bs[buf][wn * 64 + b * 32 + (lane & 31)][kk];
+      for (int b = 0; b < TB; ++b)
+        fb[b] = *(const bf16x8*)&Bs[buf][wn * (BNT / 2) + b * 32 + (lane & 31)][kk];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int a = 0; a < TA; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < TB; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
     }
   };
-  // K tiles 0 .. nkt-1 of BK; tile u lives in LDS buffer u % 2.  Loads and
-  // stores are unconditional (tile indices past the end are clamped to the
-  // last tile, whose reload and restore change nothing): a load under a
-  // branch leaves the compiler unsure which stage's loads are the newest,
-  // and it then waits for all of them before each LDS store.
+  // K tiles 0 .. nkt-1 of BK (nkt even: K a multiple of 64); tile u lives
+  // in LDS buffer u % 2, odd tiles wait in st[0], even ones in st[1].  Each
+  // step stores the next tile, refills its stage three tiles ahead and
+  // multiplies the current one.  Loads and stores are unconditional (tile
+  // indices past the end are clamped to the last tile, whose reload and
+  // restore change nothing): a load under a branch leaves the compiler
+  // unsure which stage's loads are the newest, and it then waits for all of
+  // them before each LDS store.
   const int nkt = K / BK, last = nkt - 1;
   auto tile = [&](int u) { return min(u, last) * BK; };
   gload(st[0], 0);
   sstore(st[0], 0, 0);
-  if constexpr (NS == 2) {
-    // odd tiles wait in st[0], even ones in st[1]; each pair of steps
-    // stores the next tile, refills its stage three tiles ahead and
-    // multiplies the current one (nkt is even: K a multiple of 64)
-    gload(st[0], tile(1));
-    gload(st[1], tile(2));
-    __syncthreads();
+  gload(st[0], tile(1));
+  gload(st[1], tile(2));
+  __syncthreads();
 #pragma unroll 1
-    for (int u = 0; u + 1 < nkt; u += 2) {
-      sstore(st[0], 1, tile(u + 1));
-      gload(st[0], tile(u + 3));
-      mfma(0);
-      __syncthreads();
-      sstore(st[1], 0, tile(u + 2));
-      gload(st[1], tile(u + 4));
-      mfma(1);
-      __syncthreads();
-    }
-  } else {
+  for (int u = 0; u + 1 < nkt; u += 2) {
+    sstore(st[0], 1, tile(u + 1));
+    gload(st[0], tile(u + 3));
+    mfma(0);
     __syncthreads();
-#pragma unroll 1
-    for (int u = 0; u < nkt; ++u) {
-      const int buf = u & 1;
-      gload(st[0], tile(u + 1));
-      mfma(buf);
-      sstore(st[0], buf ^ 1, tile(u + 1));
-      __syncthreads();
-    }
+    sstore(st[1], 0, tile(u + 2));
+    gload(st[1], tile(u + 4));
+    mfma(1);
+    __syncthreads();
   }
   if constexpr (AM == A_SPEC) {
-    // the first column of blocks reports each point's zmax verdict
-    if (tn == 0 && bad) {
+    // the first column of blocks reports each point's zmax verdict (inputs
+    // are at least 1e-30 before the log, so no normalised input is NaN)
+    if (tn == 0) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (((bad >> c) & 1u) && m0 + smc + c < g.M) atomicAnd(g.sa.ok + m0 + smc + c, 0);
+      for (int c = 0; c < PPT; ++c)
+        if (!(zabs[c] <= g.sa.zmax) && m0 + smc + c < g.M) atomicAnd(g.sa.ok + m0 + smc + c, 0);
     }
   }
-  // Epilogue through LDS: each wave writes its 128 x 64 part of C as two
-  // 64 x 64 halves into its own LDS region (the MFMA layout puts one column
-  // per lane), then reads rows back 16 bytes per lane, so that every store
-  // to C is a whole 128-byte row segment.  Scattered 2-byte stores of the
-  // MFMA layout ran at about 0.6 TB/s and made the output-heavy last layer
-  // store-bound.
-  __syncthreads();  // all waves done with As and Bs
+  // Epilogue through LDS: each wave writes its part of C in 64 x 64 pieces
+  // into its own LDS region (the MFMA layout puts one column per lane),
+  // then reads rows back 16 bytes per lane, so that every store to C is a
+  // whole 128-byte row segment.  Scattered 2-byte stores of the MFMA layout
+  // ran at about 0.6 TB/s and made the output-heavy last layer store-bound.
   constexpr int SLD = 64 + 8;  // staging row, padded
-  bf16(*stg)[SLD] = reinterpret_cast<bf16(*)[SLD]>(&As[0][0][0]) + w * 64;
-  static_assert(4 * 64 * SLD * sizeof(bf16) <= sizeof(As), "staging fits in As");
+  static_assert(4 * 64 * SLD <= 2 * (BMT + BNT) * ILDK, "staging fits in the LDS array");
+  __syncthreads();  // every wave done with the K loop's tiles
+  bf16(*stg)[SLD] = reinterpret_cast<bf16(*)[SLD]>(lds) + w * 64;
   bf16* const C = static_cast<bf16*>(g.C);
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int pr = 0; pr < TA / 2; ++pr)
 #pragma unroll
-    for (int aa = 0; aa < 2; ++aa)
+    for (int pc = 0; pc < TB / 2; ++pc) {
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+      for (int aa = 0; aa < 2; ++aa)
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int r = aa * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), c = b * 32 + (lane & 31);
-          // indices of the epilogue's vectors, clamped: rows past M and
-          // columns past N are staged but never stored
-          const int m = min(m0 + wm * 128 + h * 64 + r, g.M - 1), n = min(n0 + wn * 64 + c, g.N - 1);
-          const float v = acc[2 * h + aa][b][i];
-          float y;
-          if constexpr (EP == EP_GELU) {
-            y = gelu_f(v + g.bias[n]);
-          } else if constexpr (EP == EP_BF16_BM) {
-            y = v + g.bias[m];
-          } else {
-            static_assert(EP == EP_SUR, "inference epilogues: GELU, BF16_BM, SUR");
-            y = expm1f(fminf(g.osd[m] * (v + g.bias[m]) + g.omu[m], 30.f));
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+          for (int i = 0; i < 16; ++i) {
+            const int r = aa * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), c = bb * 32 + (lane & 31);
+            // indices of the epilogue's vectors, clamped: rows past M and
+            // columns past N are staged but never stored
+            const int m = min(m0 + wm * (BMT / 2) + pr * 64 + r, g.M - 1);
+            const int n = min(n0 + wn * (BNT / 2) + pc * 64 + c, g.N - 1);
+            const float v = acc[2 * pr + aa][2 * pc + bb][i];
+            float y;
+            if constexpr (EP == EP_GELU) {
+              y = gelu_fast_f(v + g.bias[n]);
+            } else if constexpr (EP == EP_BF16_BM) {
+              y = v + g.bias[m];
+            } else {
+              static_assert(EP == EP_SUR, "inference epilogues: GELU, BF16_BM, SUR");
+              // exp(x) - 1 with the hardware exp2: its cancellation near
+              // x = 0 costs relative precision only where dm^2 - 1 is below
+              // 1e-4, which moves dm^2 by less than bf16 resolves
+              const float x = fminf(g.osd[m] * (v + g.bias[m]) + g.omu[m], 30.f);
+              y = __builtin_amdgcn_exp2f(1.4426950408889634f * x) - 1.f;
+            }
+            stg[r][c] = (bf16)y;
           }
-          stg[r][c] = (bf16)y;
-        }
-    // LDS accesses of one wave complete in order: the rows read below are
-    // the ones written above, by any lane of the wave
+      // LDS accesses of one wave complete in order: the rows read below are
+      // the ones written above, by any lane of the wave
 #pragma unroll
-    for (int it = 0; it < 8; ++it) {
-      const int q = it * 64 + lane, r = q >> 3, cc = (q & 7) * 8;
-      const int m = m0 + wm * 128 + h * 64 + r, n = n0 + wn * 64 + cc;
-      const bf16x8 v = *(const bf16x8*)&stg[r][cc];
-      if (m < g.M) {
-        bf16* dst = C + (int64_t)m * g.ldc + n;
-        if (n + 8 <= g.N) {
-          *(bf16x8*)dst = v;
-        } else {
+      for (int it = 0; it < 8; ++it) {
+        const int q = it * 64 + lane, r = q >> 3, cc = (q & 7) * 8;
+        const int m = m0 + wm * (BMT / 2) + pr * 64 + r, n = n0 + wn * (BNT / 2) + pc * 64 + cc;
+        const bf16x8 v = *(const bf16x8*)&stg[r][cc];
+        if (m < g.M) {
+          bf16* dst = C + (int64_t)m * g.ldc + n;
+          if (n + 8 <= g.N) {
+            *(bf16x8*)dst = v;
+          } else {
 #pragma unroll
-          for (int e = 0; e < 8; ++e)
-            if (n + e < g.N) dst[e] = v[e];
+            for (int e = 0; e < 8; ++e)
+              if (n + e < g.N) dst[e] = v[e];
+          }
         }
       }
     }
-  }
 }
 
 // gw[i] = sum over s of part[s][i] (deterministic split-K reduction).
@@ -545,7 +564,9 @@ int gemm_infer(int amode, int ep, const GemmArgs& g, hipStream_t s) {
         reinterpret_cast<uintptr_t>(a.e) % 16 || reinterpret_cast<uintptr_t>(a.ext) % 16)
       return fail(UMWM_ERR_ARGUMENT, "gemm_infer: bad spectrum operand");
   }
-  const int ntm = (g.M + IBM - 1) / IBM, ntn = (g.N + IBN - 1) / IBN;
+  // the spectrum operand takes 128 x 256 blocks, bf16 operands 256 x 128
+  const int bm = amode == A_SPEC ? 128 : 256, bn = amode == A_SPEC ? 256 : 128;
+  const int ntm = (g.M + bm - 1) / bm, ntn = (g.N + bn - 1) / bn;
   const int64_t nt = (int64_t)ntm * ntn;
   if (nt > (1ll << 30)) return fail(UMWM_ERR_SIZE, "gemm_infer: too many tiles");
   const dim3 grid((unsigned)((nt + 7) / 8 * 8));
@@ -553,12 +574,12 @@ int gemm_infer(int amode, int ep, const GemmArgs& g, hipStream_t s) {
   // outputs) and many column blocks (the points): column blocks outer, so
   // that the row blocks of one activation block share it in L2
   const int nmajor = ep == EP_GELU ? 0 : 1;
-#define UMWM_INF(AM_, EP_) \
-  hipLaunchKernelGGL((k_gemm_inf<AM_, EP_>), grid, dim3(256), 0, s, g, ntm, ntn, nmajor)
-  if (amode == A_MK && ep == EP_GELU) UMWM_INF(A_MK, EP_GELU);
-  else if (amode == A_SPEC && ep == EP_GELU) UMWM_INF(A_SPEC, EP_GELU);
-  else if (amode == A_MK && ep == EP_SUR) UMWM_INF(A_MK, EP_SUR);
-  else if (amode == A_MK && ep == EP_BF16_BM) UMWM_INF(A_MK, EP_BF16_BM);
+#define UMWM_INF(AM_, EP_, BM_, BN_) \
+  hipLaunchKernelGGL((k_gemm_inf<AM_, EP_, BM_, BN_>), grid, dim3(256), 0, s, g, ntm, ntn, nmajor)
+  if (amode == A_MK && ep == EP_GELU) UMWM_INF(A_MK, EP_GELU, 256, 128);
+  else if (amode == A_SPEC && ep == EP_GELU) UMWM_INF(A_SPEC, EP_GELU, 128, 256);
+  else if (amode == A_MK && ep == EP_SUR) UMWM_INF(A_MK, EP_SUR, 256, 128);
+  else if (amode == A_MK && ep == EP_BF16_BM) UMWM_INF(A_MK, EP_BF16_BM, 256, 128);
   else return fail(UMWM_ERR_ARGUMENT, "gemm_infer: unsupported operand/epilogue pair");
 #undef UMWM_INF
   hipError_t e = hipGetLastError();
