@@ -423,6 +423,27 @@ def cpu_baseline(budget_s=12.0):
                       f"substeps after the cold one; serial and {cores} threads"}
 
 
+def counters_run(steps):
+    """For rocprofv3 --pmc: a briefly trained network, one FP32 hurricane
+    member with the bench's tile, two plain substeps, then `steps`
+    substeps with the surrogate attached, one at a time (no batch, no
+    graph)."""
+    net, _ = train_surrogate(2, 20000, 100)
+    d = make_domain(configs.c5(member=0, hours=1), 4)
+    try:
+        d.set_tile(*C5_TILE32)
+        d.substep()
+        d.substep()
+        net.attach(d, abi.make_surrogate_policy())
+        for _ in range(steps):
+            d.substep()
+        d.synchronize()
+        log(f"bench: counters run, {steps} substeps with the surrogate")
+    finally:
+        d.close()
+        net.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,7 +462,25 @@ def main():
                     help="profiling runs: the surrogate and the chosen wave only")
     ap.add_argument("--no-graph", action="store_true",
                     help="batch without hipGraph replay (profiler runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device three times: the known "
+                         "traffic that scales the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--counters", action="store_true",
+                    help="counter runs: one hurricane member with the surrogate (trained "
+                         "briefly) through single-domain substeps, no batch")
     args = ap.parse_args()
+    if args.counters:
+        counters_run(args.steps)
+        return
+    if args.copy_probe:
+        import torch
+        a = torch.ones(1 << 28, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("bench: copy probe, 3 copies of 1 GiB")
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU (how the path would shard over "
               "several is a design note only), so --gpus must be 1", file=sys.stderr)
