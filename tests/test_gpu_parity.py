@@ -202,6 +202,20 @@ def test_fp32_tiny_tracks_fp64(case, oracle_lib):
     assert hs_rmse(d.get("ht"), o.get("ht")) <= 0.005
 
 
+def test_fp32_tracks_fp64_below_one_metre(oracle_lib):
+    """A shelf whose shallow edge lies between 0.3 and 0.8 m with dmin =
+    0.2 m (the reference requires only dmin > 0, src/umwm_init.F90:147):
+    the FP32 sweeps derive their tables from the true depth there, as the
+    FP64 records and the oracle do."""
+    wl = configs.tiny(seed=7, winds=True, dmin=0.2, dshelf=0.6)
+    z = np.asarray(wl.grid["z"])
+    assert (-z[z < 0]).min() < 1.0  # sea points shallower than 1 m exist
+    o, d = make_pair(wl, precision=4)
+    o.run_dtg()
+    d.run_dtg()
+    assert hs_rmse(d.get("ht"), o.get("ht")) <= 0.005
+
+
 def test_stress_ocn_needs_operator_sources():
     """After a fused substep the source arrays of stress('ocn') were never
     materialised: without keep_sources the call reports a state error

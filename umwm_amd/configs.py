@@ -254,12 +254,13 @@ def c5(member=0, seed=5, mm=1500, nm=1200, om=49, pm=72, hours=24, dx=2000.0):
 
 def tiny(seed=0, mm=14, nm=11, om=25, pm=24, land=True, shelf=True, isglobal=False,
          winds=False, currents=False, wspd=12.0, wdir=0.3, ice=False, density=False,
-         dmin=10.0):
-    """Small domains for parity tests: optional land, sloping depth,
-    periodic lat-lon grid, time-varying wind and currents, sea ice (a field
-    spread over [0, 1], so that points lie below fice_lth, between the two
-    thresholds where Kohout attenuation acts, and above fice_uth where
-    advection is switched off) and air/water density fields."""
+         dmin=10.0, dshelf=20.0):
+    """Small domains for parity tests: optional land, sloping depth (from
+    dshelf at the shallow edge), periodic lat-lon grid, time-varying wind
+    and currents, sea ice (a field spread over [0, 1], so that points lie
+    below fice_lth, between the two thresholds where Kohout attenuation
+    acts, and above fice_uth where advection is switched off) and air/water
+    density fields."""
     rng = np.random.default_rng(seed)
     kw = dict(mm=mm, nm=nm, om=om, pm=pm, fmin=0.0313, fmax=2.0, fprog=2.0, dtg=3600.0,
               delx=2e4, dely=1.5e4, wspd0=wspd, wdir0=wdir, stokes=1, dmin=dmin)
@@ -274,7 +275,7 @@ def tiny(seed=0, mm=14, nm=11, om=25, pm=24, land=True, shelf=True, isglobal=Fal
     if land or shelf or isglobal:
         depth = np.full((mm, nm), 800.0)
         if shelf:
-            depth = 20.0 + 1500.0 * (np.arange(mm)[:, None] / (mm - 1)) ** 2 * np.ones((1, nm))
+            depth = dshelf + 1500.0 * (np.arange(mm)[:, None] / (mm - 1)) ** 2 * np.ones((1, nm))
             depth = depth * (1 + 0.3 * rng.uniform(-1, 1, size=(mm, nm)))
         z = -depth
         if land:
