@@ -453,38 +453,55 @@ __global__ void k_reduce_parts(const float* __restrict__ part, int splits, int64
   out[i] = s;
 }
 
-// gb[n] = sum over rows of d[r][n] (bf16 [rows][ld]).
-__global__ void k_colsum(const bf16* __restrict__ d, int rows, int cols, int64_t ld,
-                         float* __restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= cols) return;
+// gb[n] = sum over rows of d[r][n] (bf16 [rows][ld]).  A block of 256
+// threads takes 32 columns: 8 row groups sum every 8th row each, then the 8
+// partial sums are added in a fixed order (the result does not depend on
+// scheduling).
+__global__ __launch_bounds__(256) void k_colsum(const bf16* __restrict__ d, int rows, int cols,
+                                                int64_t ld, float* __restrict__ out) {
+  __shared__ float part[8][33];
+  const int c = threadIdx.x & 31, rg = threadIdx.x >> 5, n = blockIdx.x * 32 + c;
   float s = 0.f;
-  for (int r = 0; r < rows; ++r) s += (float)d[(int64_t)r * ld + n];
-  out[n] = s;
+  if (n < cols)
+    for (int r = rg; r < rows; r += 8) s += (float)d[(int64_t)r * ld + n];
+  part[rg][c] = s;
+  __syncthreads();
+  if (rg == 0 && n < cols) {
+    float t = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t += part[q][c];
+    out[n] = t;
+  }
 }
 
-// Mean squared error over the real outputs with per-output weights.
-__global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t,
-                       const float* __restrict__ wout, int rows, int nout, int ld,
-                       bf16* __restrict__ dy, double* __restrict__ loss) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t n = (int64_t)rows * ld;
+// Mean squared error over the real outputs with per-output weights: one
+// block per row (no index division), one atomic per block.
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y,
+                                              const float* __restrict__ t,
+                                              const float* __restrict__ wout, int rows, int nout,
+                                              int ld, bf16* __restrict__ dy,
+                                              double* __restrict__ loss) {
+  __shared__ double wsum[4];
+  const int64_t r0 = (int64_t)blockIdx.x * ld;
+  const float inv = 1.f / ((float)rows * (float)nout);
   double l = 0;
-  if (i < n) {
-    const int c = (int)(i % ld);
+  for (int c = threadIdx.x; c < ld; c += 256) {
     float g = 0.f;
     if (c < nout) {
       const float wo = wout ? wout[c] : 1.f;
-      const float d = y[i] - t[i];
-      const float inv = 1.f / ((float)rows * (float)nout);
+      const float d = y[r0 + c] - t[r0 + c];
       g = 2.f * wo * d * inv;
-      l = (double)wo * d * d * inv;
+      l += (double)wo * d * d * inv;
     }
-    dy[i] = (bf16)g;
+    dy[r0 + c] = (bf16)g;
   }
-  // wave sum, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
-  if ((threadIdx.x & 63) == 0 && l != 0) atomicAdd(loss, l);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double b = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (b != 0) atomicAdd(loss, b);
+  }
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ gr, float* __restrict__ m,
@@ -698,8 +715,8 @@ int mlp_forward(Mlp& m, int rows, hipStream_t s) {
 
 int mlp_loss(Mlp& m, const float* t, const float* wout, int rows, hipStream_t s) {
   const int ld = m.L.back().N;
-  const int64_t n = (int64_t)rows * ld;
-  hipLaunchKernelGGL(k_loss, grid_for(n), dim3(256), 0, s, (const float*)m.y, t, wout, rows,
+  if (rows < 1) return 0;
+  hipLaunchKernelGGL(k_loss, dim3(rows), dim3(256), 0, s, (const float*)m.y, t, wout, rows,
                      m.nout, ld, m.dy, m.loss_acc);
   UMWM_HIP_TRY(hipGetLastError());
   return 0;
@@ -733,7 +750,8 @@ int mlp_backward(Mlp& m, int rows, hipStream_t s) {
     const int64_t nw = (int64_t)L.N * L.K;
     hipLaunchKernelGGL(k_reduce_parts, grid_for(nw), dim3(256), 0, s, (const float*)m.part, used,
                        nw, L.gw);
-    hipLaunchKernelGGL(k_colsum, grid_for(L.N), dim3(256), 0, s, dz, rows, L.N, (int64_t)L.N, L.gb);
+    hipLaunchKernelGGL(k_colsum, dim3((L.N + 31) / 32), dim3(256), 0, s, dz, rows, L.N,
+                       (int64_t)L.N, L.gb);
     UMWM_HIP_TRY(hipGetLastError());
     if (l > 0) {
       // d a_{l-1} = dz W_l, times gelu'(z_{l-1})
