@@ -163,6 +163,27 @@ int umwm_hip_stokes_drift(umwm_domain* dom, int init);
 int umwm_hip_forcinginterpolate(umwm_domain* dom);
 /* forcinginput (src/umwm_forcing.F90:11-37): level n <- n+1, load n+1. */
 int umwm_hip_forcinginput(umwm_domain* dom, const umwm_forcing_input* next);
+
+/* A translating tropical cyclone as wind forcing, generated on the device:
+ * no counterpart in the reference, which reads winds from files
+ * (src/umwm_io.F90:13-93); it stands for the snapshot a driver would pass to
+ * umwm_hip_forcinginput for a storm.  Holland (1980) gradient wind
+ * v(r) = sqrt(B dp / rho (Rmax/r)^B exp(-(Rmax/r)^B) + (r f / 2)^2) - r f / 2,
+ * r = distance from (xc, yc) + 1 m, f = 2 Omega sin(lat), turned inward by
+ * `inflow_deg` from the cyclonic tangent, plus half the translation velocity
+ * (trans_u, trans_v).  dp <= 0 takes dp = rho e vmax^2 / B.  Planar grids
+ * only (gridfromfile = 0, isglobal = 0): the point (m, n) lies at
+ * ((m-1) delx, (n-1) dely). */
+typedef struct umwm_holland {
+  double xc, yc;            /* storm centre (m) */
+  double vmax, rmax, b;     /* m/s, m, shape parameter */
+  double rho, dp;           /* air density (kg/m3), pressure deficit (Pa) */
+  double trans_u, trans_v;  /* translation velocity (m/s) */
+  double lat_deg, inflow_deg;
+} umwm_holland;
+/* umwm_hip_forcinginput with the next level's winds computed on the device
+ * from `storm` (the domain's only switched forcing must be the wind). */
+int umwm_hip_holland_forcinginput(umwm_domain* dom, const umwm_holland* storm);
 /* sin_d12, sds_d12, snl_d12, s_ice (src/umwm_source_functions.F90:12). */
 int umwm_hip_sin_d12(umwm_domain* dom);
 int umwm_hip_sds_d12(umwm_domain* dom);

@@ -308,3 +308,31 @@ def test_ice_density_fused_hours(precision):
     finally:
         d.close()
         o.close()
+
+
+def test_holland_forcing_generated_on_the_device():
+    """The hurricane's wind snapshots generated on the device from the storm
+    parameters (umwm_hip_holland_forcinginput) against the same snapshots
+    computed on the host and passed through umwm_hip_forcinginput: two hours
+    of the reduced hurricane grid in FP64, whose interpolated winds, Hs and
+    ustar agree to rounding (the two differ only in the last bits of the
+    libm functions)."""
+    wl = configs.c5(member=3, mm=100, nm=80, dx=9000.0, hours=2)
+    a = hip.Domain(wl.params, wl.grid_input(), wl.forcing_input(0), precision=8)
+    b = hip.Domain(wl.params, wl.grid_input(), wl.forcing_input(0), precision=8)
+    try:
+        for d in (a, b):
+            d.init_jonswap(wl.jonswap)
+            d.stokes_drift(True)
+        for hour in (1, 2):
+            a.forcinginput(wl.forcing_input(hour))
+            b.holland_forcinginput(wl.storm(hour))
+            a.run_dtg()
+            b.run_dtg()
+            for f in ("wspd", "wdir"):
+                assert relerr(b.get(f)[:a.im], a.get(f)[:a.im]) <= 1e-12, (hour, f)
+            assert relerr(b.get("ht"), a.get("ht")) <= 1e-10
+            assert relerr(b.get("ustar"), a.get("ustar")) <= 1e-10
+    finally:
+        a.close()
+        b.close()

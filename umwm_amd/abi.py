@@ -70,6 +70,17 @@ class SurrogateConfig(C.Structure):
     ]
 
 
+class Holland(C.Structure):
+    _fields_ = [(n, F64) for n in ("xc", "yc", "vmax", "rmax", "b", "rho", "dp", "trans_u",
+                                   "trans_v", "lat_deg", "inflow_deg")]
+
+
+def make_holland(xc, yc, vmax, rmax, b=1.5, rho=1.15, dp=0.0, trans=(0.0, 0.0), lat_deg=20.0,
+                 inflow_deg=20.0):
+    return Holland(xc=xc, yc=yc, vmax=vmax, rmax=rmax, b=b, rho=rho, dp=dp, trans_u=trans[0],
+                   trans_v=trans[1], lat_deg=lat_deg, inflow_deg=inflow_deg)
+
+
 class SurrogatePolicy(C.Structure):
     _fields_ = [
         ("audit_stride", I32), ("reserved0", I32), ("resid_tol", F64), ("audit_tol", F64),

@@ -38,7 +38,7 @@ class Workload:
     """Everything needed to create a domain and start it."""
 
     def __init__(self, name, params, grid=None, forcing=None, jonswap=None, hours=1,
-                 snapshots=None, note=""):
+                 snapshots=None, note="", storm=None):
         self.name = name
         self.params = params
         self.grid = grid          # dict(lon=, lat=, z=, exclude=) of (mm, nm) arrays
@@ -47,6 +47,7 @@ class Workload:
         self.hours = hours
         self.snapshots = snapshots  # callable(hour) -> forcing dict at hour
         self.note = note
+        self.storm = storm          # callable(hour) -> abi.Holland of that snapshot, or None
 
     def grid_input(self):
         if not self.grid:
@@ -245,9 +246,14 @@ def c5(member=0, seed=5, mm=1500, nm=1200, om=49, pm=72, hours=24, dx=2000.0):
         u, v = holland_wind(xy[..., 0], xy[..., 1], xc, yc, vmax, 3.0e4, 1.5, trans=(vx, vy))
         return dict(uw=u, vw=v)
 
+    def storm(hour):
+        """The same snapshot's storm, for generation on the device."""
+        return abi.make_holland(x0 + vx * 3600 * hour, y0 + vy * 3600 * hour, vmax, 3.0e4, 1.5,
+                                trans=(vx, vy))
+
     js = abi.make_jonswap(alpha=0.01, gamma=3.3, fp_from_wind=True, dir_from_wind=True)
     return Workload(f"c5_m{member}", p, grid=dict(z=z), forcing=snap(0), jonswap=js,
-                    hours=hours, snapshots=snap,
+                    hours=hours, snapshots=snap, storm=storm,
                     note=f"hurricane member {member}: heading {math.degrees(heading):.1f} deg, "
                          f"vmax {vmax:.1f} m/s")
 

@@ -1085,6 +1085,27 @@ int sync_and_check(DomainImpl<T>& D) {
 
 }  // namespace
 
+namespace {
+// Holland (1980) wind of a translating cyclone at every grid point of a
+// planar domain, written into the next forcing level ([n][m], m fastest, as
+// k_forcing reads it); FP64 throughout.
+template <typename T>
+__global__ void k_holland(T* __restrict__ uw, T* __restrict__ vw, int mm, int nm, double dx,
+                          double dy, umwm_holland st, double dp, double f) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= mm * nm) return;
+  const int m = q % mm, n = q / mm;
+  const double x = m * dx - st.xc, y = n * dy - st.yc;
+  const double r = sqrt(x * x + y * y) + 1.0;
+  const double a = pow(st.rmax / r, st.b);
+  const double hf = 0.5 * r * f;
+  const double v = sqrt(st.b * dp / st.rho * a * exp(-a) + hf * hf) - hf;
+  const double th = atan2(y, x) + 0.5 * M_PI + st.inflow_deg * (M_PI / 180.0);
+  uw[q] = T(v * cos(th) + 0.5 * st.trans_u);
+  vw[q] = T(v * sin(th) + 0.5 * st.trans_v);
+}
+}  // namespace
+
 extern "C" {
 
 const char* umwm_hip_error_string(int status) {
@@ -1214,6 +1235,36 @@ int umwm_hip_forcinginput(umwm_domain* dom, const umwm_forcing_input* nx) {
     F.rhoab = D.rhoab; F.rhoaf = D.rhoaf; F.rhowb = D.rhowb; F.rhowf = D.rhowf;
     // the kernels (and any graph that captured them) read the levels
     // through the device record
+    UMWM_HIP_TRY(hipMemcpyAsync(D.fd, &F, sizeof(ForcingDev<T>), hipMemcpyHostToDevice, D.stream));
+    UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+    return 0;
+  });
+}
+
+int umwm_hip_holland_forcinginput(umwm_domain* dom, const umwm_holland* st) {
+  if (!st) return fail(UMWM_ERR_ARGUMENT, "null storm");
+  return dispatch(dom, [&](auto& D) {
+    using T = std::remove_reference_t<decltype(*D.h.e)>;
+    const umwm_params& P = D.S.P;
+    if (!P.winds || P.currents || P.seaice || P.air_density || P.water_density)
+      return fail(UMWM_ERR_STATE, "holland_forcinginput: the wind must be the only forcing");
+    if (P.gridfromfile || P.isglobal)
+      return fail(UMWM_ERR_STATE, "holland_forcinginput: planar grids only");
+    if (!(st->rmax > 0) || !(st->b > 0) || !(st->rho > 0) || !(st->vmax >= 0))
+      return fail(UMWM_ERR_ARGUMENT, "holland_forcinginput: rmax, b, rho > 0, vmax >= 0");
+    UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
+    // level n <- level n+1 (src/umwm_forcing.F90:11-37), the new level
+    // generated in place
+    std::swap(D.uwb, D.uwf);
+    std::swap(D.vwb, D.vwf);
+    const int mm = D.S.mm, nm = D.S.nm;
+    const double dp = st->dp > 0 ? st->dp : st->rho * M_E * st->vmax * st->vmax / st->b;
+    const double f = 2 * 7.292e-5 * std::sin(st->lat_deg * M_PI / 180.0);
+    hipLaunchKernelGGL(k_holland<T>, dim3((unsigned)(((int64_t)mm * nm + 255) / 256)), dim3(256),
+                       0, D.stream, D.uwf, D.vwf, mm, nm, P.delx, P.dely, *st, dp, f);
+    UMWM_HIP_TRY(hipGetLastError());
+    ForcingDev<T>& F = D.fh;
+    F.uwb = D.uwb; F.uwf = D.uwf; F.vwb = D.vwb; F.vwf = D.vwf;
     UMWM_HIP_TRY(hipMemcpyAsync(D.fd, &F, sizeof(ForcingDev<T>), hipMemcpyHostToDevice, D.stream));
     UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
     return 0;

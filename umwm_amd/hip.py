@@ -19,6 +19,7 @@ EXPORTS = [
     "umwm_hip_error_string", "umwm_hip_last_error", "umwm_hip_device_count",
     "umwm_hip_create", "umwm_hip_destroy", "umwm_hip_dims", "umwm_hip_init_jonswap",
     "umwm_hip_stokes_drift", "umwm_hip_forcinginterpolate", "umwm_hip_forcinginput",
+    "umwm_hip_holland_forcinginput",
     "umwm_hip_sin_d12", "umwm_hip_sds_d12", "umwm_hip_snl_d12", "umwm_hip_s_ice",
     "umwm_hip_source", "umwm_hip_propagation", "umwm_hip_refraction", "umwm_hip_update",
     "umwm_hip_stress", "umwm_hip_diag", "umwm_hip_end_first", "umwm_hip_substep", "umwm_hip_substeps",
@@ -68,6 +69,7 @@ def lib():
     L.umwm_hip_init_jonswap.argtypes = [vp, P(abi.Jonswap)]
     L.umwm_hip_stokes_drift.argtypes = [vp, C.c_int]
     L.umwm_hip_forcinginput.argtypes = [vp, P(abi.ForcingInput)]
+    L.umwm_hip_holland_forcinginput.argtypes = [vp, P(abi.Holland)]
     for fn in ("forcinginterpolate", "sin_d12", "sds_d12", "snl_d12", "s_ice", "source",
                "propagation", "refraction", "update", "diag", "end_first", "synchronize"):
         getattr(L, "umwm_hip_" + fn).argtypes = [vp]
@@ -207,6 +209,11 @@ class Domain:
     def forcinginput(self, forcing):
         f, self._fk2 = forcing
         check(lib().umwm_hip_forcinginput(self.h, C.byref(f)))
+
+    def holland_forcinginput(self, storm):
+        """The next forcing level's winds generated on the device from an
+        abi.Holland storm (planar grids, winds the only forcing)."""
+        check(lib().umwm_hip_holland_forcinginput(self.h, C.byref(storm)))
 
     def sin_d12(self):
         check(lib().umwm_hip_sin_d12(self.h))
