@@ -59,6 +59,7 @@
 
 #include "bufio.hpp"
 #include "domain.hpp"
+#include "fmath64.hpp"
 #include "physics.hpp"
 #include "surrogate.hpp"
 
@@ -330,7 +331,27 @@ struct TabGen {
 
 // Transcendentals of the sweeps.  FP32 works in base 2 on the hardware
 // log2/exp2 (exponents pre-scaled by log2 e where a natural exponential is
-// meant); FP64 uses the device library's natural log and exp.
+// meant).  FP64 uses the table-driven natural log and exp of fmath64.hpp
+// (within 1 ulp, about 0.65x the instructions of the device library's) and
+// a reciprocal from the hardware estimate refined by two Newton steps (its
+// arguments are positive and normal: the spreading factor >= 1 and
+// cf coth).  The tables (4.2 KB) are staged into LDS by every block of the
+// FP64 sweeps: read from global memory they were 3.5 KB of divergent
+// vector loads per wave and bin, more than the spectrum stream itself.
+constexpr int kFm64Len = fm64::kExpTabLen + fm64::kLogTabLen + fm64::kInvTabLen;
+__device__ const double kFm64Tab[kFm64Len] = {UMWM_FM64_EXP_TABLE, UMWM_FM64_LOG_TABLE,
+                                              UMWM_FM64_INV_TABLE};
+__shared__ __attribute__((aligned(16))) double sFm64[kFm64Len];
+
+// Fills the LDS tables; every thread of the block must call it.
+template <typename T>
+__device__ __forceinline__ void stage_fm64() {
+  if constexpr (std::is_same_v<T, double>) {
+    for (int k = threadIdx.x; k < kFm64Len; k += blockDim.x) sFm64[k] = kFm64Tab[k];
+    __syncthreads();
+  }
+}
+
 template <typename T>
 struct Tr;
 template <>
@@ -343,9 +364,15 @@ struct Tr<float> {
 template <>
 struct Tr<double> {
   static constexpr double kExpScale = 1.0;
-  __device__ static __forceinline__ double lg(double x) { return log(x); }
-  __device__ static __forceinline__ double ex(double x) { return exp(x); }
-  __device__ static __forceinline__ double rcp(double x) { return 1.0 / x; }
+  __device__ static __forceinline__ double lg(double x) {
+    return fm64::log_t(x, sFm64 + fm64::kExpTabLen, sFm64 + fm64::kExpTabLen + fm64::kLogTabLen);
+  }
+  __device__ static __forceinline__ double ex(double x) { return fm64::exp_t(x, sFm64); }
+  __device__ static __forceinline__ double rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+  }
 };
 
 // Per-frequency wind-input factors in the form the sweeps use: ssin =
@@ -371,6 +398,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
                                                const BlockTask* __restrict__ tasks, int first,
                                                int rot_all) {
   extern __shared__ double smem[];
+  stage_fm64<T>();
   const BlockTask tk = tasks[blockIdx.x];
   const DomainDev<T>& d = doms[tk.dom];
   // rotation tendency on the first substep of the launch (rot_all) and on
@@ -842,7 +870,8 @@ __global__ __launch_bounds__(512) void k_integrate(const DomainDev<T>* __restric
   const BlockTask tk = tasks[blockIdx.x];
   const DomainDev<T>& d = doms[tk.dom];
   const Scalars* sc = d.sc;
-  if (!sc->active) return;
+  if (!sc->active) return;  // uniform over the block
+  stage_fm64<T>();
   const int W = blockDim.x >> 6, w = wave_id();
   const int j = tk.i0 + (threadIdx.x & 63);
   const bool valid = j < d.im;
