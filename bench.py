@@ -44,37 +44,43 @@ KERNELS = ("k_rates", "k_integrate", "k_advect", "k_stress")
 # k_advect reads E* and E^n, writes E^{n+1}; k_stress works on O(om) per point
 ALG_WORDS = {"k_rates": 1, "k_integrate": 2, "k_advect": 3, "k_stress": 0}
 PROFILE = os.path.join(ROOT, "profiles", "k_advect_c5_fp32_surrogate.md")
+PROFILE64 = os.path.join(ROOT, "profiles", "k_integrate_c5_fp64.md")
 # tile of the fused kernels for the 49 x 72 members in FP32 (the autotuner's
 # choice on a full-size member, profiles/k_advect_c5_fp32_surrogate.md)
 C5_TILE32 = (36, 2, 18, 4, 2)
+# and in FP64 (the plain path; the autotuner's choice on a full-size member)
+C5_TILE64 = (18, 4, 9, 8, 2)
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def profile_traffic(path=PROFILE):
-    """HBM bytes per k_advect launch on this workload, measured with
-    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a 1 GiB copy) and
-    recorded in the committed profile summary; None when absent."""
+def profile_traffic(path):
+    """HBM bytes per launch of each fused kernel on this workload, measured
+    with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a 1 GiB copy) and
+    recorded in a committed profile summary as
+    `traffic_bytes_per_launch[kernel]: bytes`; {} when absent."""
     try:
         txt = open(path).read()
     except OSError:
-        return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
-    return float(m.group(1)) if m else None
+        return {}
+    return {k: float(v) for k, v in
+            re.findall(r"traffic_bytes_per_launch\[(\w+)\]:\s*([0-9.eE+]+)", txt)}
 
 
 def roofline(kms, bins, word, traffic=None):
     """Roofline of the dominant fused kernel (the longest of the four): its
-    algorithmic bytes per launch over its mean HIP-event duration."""
+    algorithmic bytes per launch over its mean HIP-event duration, with its
+    measured traffic per launch from `traffic` (kernel -> bytes) if known."""
     kd = dict(zip(KERNELS, kms))
     dom = max(kd, key=kd.get)
     alg = ALG_WORDS[dom] * word * bins
     ach = alg / (kd[dom] * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "alg_bytes_per_launch": alg, "ms": round(kd[dom], 4)}
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": (traffic or {}).get(dom), "alg_bytes_per_launch": alg,
+            "ms": round(kd[dom], 4)}
 
 
 def make_domain(wl, precision, hour=1):
@@ -141,7 +147,7 @@ def plain_fp64(wl, steps, warmup, tile):
         fields["us"] = np.hypot(d.get("us")[0], d.get("vs")[0])
         return {"value": rate, "dtype": "f64", "ms_per_step": 1e3 * wall / steps,
                 "kernel_ms": dict(zip(KERNELS, [round(x, 3) for x in kms[:4]])),
-                "tile": d.tile(), "roofline": roofline(kms[:4], bins, 8),
+                "tile": d.tile(), "roofline": roofline(kms[:4], bins, 8, profile_traffic(PROFILE64)),
                 "substeps_hour1": int(d.scalar("nsub"))}, fields
     finally:
         d.close()
@@ -316,12 +322,12 @@ def c2_leg(steps, warmup):
     return out
 
 
-def c3_leg(n, hours=3, seed=3):
-    """C3: n uneven regional domains (100x100..600x400, om 25-49, pm 24-72)
-    advanced together through all `hours` hours (one launch per kernel per
-    pm class per substep, hipGraph replay), then the same members one domain
-    at a time for the same hours."""
-    wls = configs.c3(seed=seed, n=n, hours=hours)
+def c3_leg(n, hours=3, seed=3, scale=1.0):
+    """C3: n uneven regional domains (100x100..600x400, om 25-49, pm 24-72;
+    `scale` shrinks the grids) advanced together through all `hours` hours
+    (one launch per kernel per pm class per substep, hipGraph replay), then
+    the same members one domain at a time for the same hours."""
+    wls = configs.c3(seed=seed, n=n, hours=hours, scale=scale)
 
     def make(wl):
         d = hip.Domain(wl.params, wl.grid_input(), wl.forcing_input(0), precision=4)
@@ -346,7 +352,8 @@ def c3_leg(n, hours=3, seed=3):
     # the cold substep of hour 1 updates no bin
     upd = sum(sum(ns * bb for ns, bb in zip(h, bins)) for h in nsub) - sum(bins)
     out = {"value": upd / wall, "unit": "bin-updates/s", "members": n, "hours": hours,
-           "wall_s": wall, "substeps_per_hour": [int(np.mean(h)) for h in nsub]}
+           "grid_scale": scale, "wall_s": wall,
+           "substeps_per_hour": [int(np.mean(h)) for h in nsub]}
     log(f"bench: c3 batched {upd / wall / 1e9:.2f} G bin-updates/s over {wall:.1f} s")
     doms = [make(wl) for wl in wls]
     try:
@@ -444,6 +451,21 @@ def counters_run(steps):
         net.close()
 
 
+def counters_plain_run(steps):
+    """For rocprofv3 --pmc: one FP64 hurricane member (the plain path) with
+    the bench's tile, one cold substep, then `steps` substeps one at a time."""
+    d = make_domain(configs.c5(member=0, hours=1), 8)
+    try:
+        d.set_tile(*C5_TILE64)
+        d.substep()
+        for _ in range(steps):
+            d.substep()
+        d.synchronize()
+        log(f"bench: counters run, {steps} FP64 substeps")
+    finally:
+        d.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -468,7 +490,13 @@ def main():
     ap.add_argument("--counters", action="store_true",
                     help="counter runs: one hurricane member with the surrogate (trained "
                          "briefly) through single-domain substeps, no batch")
+    ap.add_argument("--counters-plain", action="store_true",
+                    help="counter runs: one FP64 hurricane member (the plain path) through "
+                         "single-domain substeps")
     args = ap.parse_args()
+    if args.counters_plain:
+        counters_plain_run(args.steps)
+        return
     if args.counters:
         counters_run(args.steps)
         return
@@ -491,7 +519,7 @@ def main():
     plain, f64 = (None, None)
     if not args.only_chosen:
         log("bench: c5 plain fp64 member")
-        plain, f64 = plain_fp64(wl0, args.plain_steps, args.warmup, (18, 4, 9, 8, 2))
+        plain, f64 = plain_fp64(wl0, args.plain_steps, args.warmup, C5_TILE64)
     net, sur_info = None, None
     if not args.no_surrogate:
         log("bench: surrogate training")
@@ -504,7 +532,7 @@ def main():
     if net is not None:
         net.close()
     bins = chosen["bins_per_member_step"]
-    traffic = profile_traffic()
+    traffic = profile_traffic(PROFILE)
     out = {
         "metric": METRIC,
         "value": rate,
@@ -552,6 +580,10 @@ def main():
         if args.c3_members > 0:
             log("bench: c3 leg")
             out["batched_c3"] = c3_leg(args.c3_members)
+            # the same generator at a quarter of the grid size: members too
+            # small to fill the GPU alone, where batching is what pays
+            log("bench: c3 leg, quarter-scale grids")
+            out["batched_c3_quarter_scale"] = c3_leg(args.c3_members, scale=0.25)
         if not args.no_cpu:
             log("bench: cpu baseline")
             out["cpu_baseline"] = cpu_baseline()
