@@ -28,6 +28,12 @@ extern "C" void fm_exp(const double* x, double* y, long n) {
 extern "C" void fm_log(const double* x, double* y, long n) {
   for (long i = 0; i < n; ++i) y[i] = log_t(x[i], LT, IT);
 }
+extern "C" void fm_exp_fast(const double* x, double* y, long n) {
+  for (long i = 0; i < n; ++i) y[i] = exp_t<false>(x[i], ET);
+}
+extern "C" void fm_log_fast(const double* x, double* y, long n) {
+  for (long i = 0; i < n; ++i) y[i] = log_t<false>(x[i], LT, IT);
+}
 """
 
 
@@ -119,3 +125,15 @@ def test_tables_match_generator():
         sys.path.pop(0)
     with open(os.path.join(CSRC, "fmath64_tables.inc")) as fh:
         assert fh.read() == gen_fmath64.render(gen_fmath64.tables())
+
+
+def test_unchecked_variants_on_their_domain(fm):
+    # the sweeps' variants: the same bits wherever they are defined (exp of
+    # anything but NaN, log of finite x >= 0), log 0 = -inf and exp(-inf) = 0
+    rng = np.random.default_rng(14)
+    xe = np.concatenate([rng.uniform(-800, 800, 100_000), [-np.inf, np.inf, 0.0]])
+    assert np.array_equal(fm("fm_exp_fast", xe), fm("fm_exp", xe))
+    xl = np.concatenate([np.exp(rng.uniform(-745, 709, 100_000)), [0.0, 5e-324, 1.0]])
+    with np.errstate(all="ignore"):
+        assert np.array_equal(fm("fm_log_fast", xl), fm("fm_log", xl))
+    assert fm("fm_exp_fast", fm("fm_log_fast", np.array([0.0])) * 2.4)[0] == 0.0

@@ -43,7 +43,10 @@ constexpr int kInvTabLen = 129;
 UMWM_FM64_HD inline uint64_t bits_of(double x) { return __builtin_bit_cast(uint64_t, x); }
 UMWM_FM64_HD inline double double_of(uint64_t b) { return __builtin_bit_cast(double, b); }
 
-// e^x.  Arguments below -746 give 0 and above 710 give +inf; NaN stays NaN.
+// e^x.  Arguments below -746 give 0 and above 710 give +inf; NaN stays NaN
+// (with kNaN false NaN gives 0 or +inf: for callers whose NaN reaches the
+// result by another operand).
+template <bool kNaN = true>
 UMWM_FM64_HD inline double exp_t(double x, const double* __restrict__ tab) {
   const double xc = __builtin_fmin(__builtin_fmax(x, -746.0), 710.0);
   const double t = __builtin_rint(xc * kExpInvL);
@@ -56,10 +59,13 @@ UMWM_FM64_HD inline double exp_t(double x, const double* __restrict__ tab) {
   const int j = n & 63;
   const double hi = tab[2 * j], lo = tab[2 * j + 1];
   const double v = __builtin_ldexp(hi + __builtin_fma(hi, p, lo), n >> 6);
-  return x == x ? v : x;
+  if constexpr (kNaN) return x == x ? v : x;
+  return v;
 }
 
-// log x.  0 gives -inf, negative arguments NaN, +inf stays +inf.
+// log x.  0 gives -inf, negative arguments NaN, +inf stays +inf (with
+// kSpecial false only 0 is treated apart: for finite x >= 0).
+template <bool kSpecial = true>
 UMWM_FM64_HD inline double log_t(double x, const double* __restrict__ ltab,
                                  const double* __restrict__ itab) {
   int e;
@@ -84,6 +90,7 @@ UMWM_FM64_HD inline double log_t(double x, const double* __restrict__ ltab,
   // finite positive arguments (normal or not) take v; a select, not a
   // branch, so that the compiler does not sink the evaluation into a
   // conditional block
+  if constexpr (!kSpecial) return x == 0.0 ? -__builtin_inf() : v;
   const double sp = x == 0.0 ? -__builtin_inf() : (x < 0.0 ? __builtin_nan("") : x);
   return (x > 0.0 && x < __builtin_inf()) ? v : sp;
 }
