@@ -116,6 +116,29 @@ def timed_substeps(d, steps):
     return wall
 
 
+def hour_fields(d):
+    """The fields compared with FP64 at the end of an hour: Hs, Tm, ustar
+    and the surface Stokes drift speed."""
+    f = {k: d.get(k) for k in ("ht", "mwp", "ustar")}
+    f["us"] = np.hypot(d.get("us")[0], d.get("vs")[0])
+    return f
+
+
+def field_errors(h32, ref):
+    """Relative errors of hour_fields() against the FP64 ones."""
+    def rmse(a, b):
+        return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-30))
+    e = {"hs_rmse_rel": rmse(h32["ht"], ref["ht"]),
+         "hs_max_rel": float(np.max(np.abs(h32["ht"] - ref["ht"]) / np.maximum(ref["ht"], 1e-3))),
+         "tm_max_rel": float(np.max(np.abs(h32["mwp"] - ref["mwp"]) / np.maximum(ref["mwp"], 1e-3))),
+         "ustar_max_rel": float(np.max(np.abs(h32["ustar"] - ref["ustar"])
+                                       / np.maximum(np.abs(ref["ustar"]), 1e-3))),
+         "ustar_rmse_rel": rmse(h32["ustar"], ref["ustar"]),
+         "stokes_surface_rmse_rel": rmse(h32["us"], ref["us"]),
+         "nonfinite_points": int(np.sum(~np.isfinite(h32["ht"])))}
+    return e
+
+
 def finish_hour(d):
     while d.scalar("sumt") < 3600.0:
         d.substeps(8)
@@ -143,8 +166,7 @@ def plain_fp64(wl, steps, warmup, tile):
         rate = steps * bins / wall
         log(f"bench: c5 fp64 {rate / 1e9:.2f} G bin-updates/s ({1e3 * wall / steps:.1f} ms/substep)")
         finish_hour(d)
-        fields = {k: d.get(k) for k in ("ht", "mwp", "ustar")}
-        fields["us"] = np.hypot(d.get("us")[0], d.get("vs")[0])
+        fields = hour_fields(d)
         return {"value": rate, "dtype": "f64", "ms_per_step": 1e3 * wall / steps,
                 "kernel_ms": dict(zip(KERNELS, [round(x, 3) for x in kms[:4]])),
                 "tile": d.tile(), "roofline": roofline(kms[:4], bins, 8, profile_traffic(PROFILE64)),
@@ -153,12 +175,13 @@ def plain_fp64(wl, steps, warmup, tile):
         d.close()
 
 
-def train_surrogate(members, samples, steps):
+def train_surrogate(members, samples, steps, hours=1):
     """The 49 x 72 network, trained on samples the plain FP32 kernel produces
     on reduced hurricane grids (100 x 80 cells of 9 km, the same generator:
-    shelf, coastline, storm): one sample per sea point of snapshots spread
-    evenly over the first hour of `members` ensemble members; the last
-    member's samples are held out."""
+    shelf, coastline, storm, winds advanced each hour): one sample per sea
+    point of snapshots spread evenly over `hours` hours of `members`
+    ensemble members, so that the sea states of the whole storm are seen;
+    the last member's samples are held out."""
     cfg = abi.make_surrogate_config(om=49, pm=72, hidden=512, layers=3, batch=1024, chunk=65536,
                                     capacity=samples, seed=11)
     net = hip.Surrogate(cfg)
@@ -167,16 +190,23 @@ def train_surrogate(members, samples, steps):
     for m in range(members):
         if m == members - 1:
             ntrain = net.info()["samples"]
-        d = make_domain(configs.c5(member=m, mm=100, nm=80, dx=9000.0, hours=1), 4)
+        wl = configs.c5(member=m, mm=100, nm=80, dx=9000.0, hours=hours)
+        d = make_domain(wl, 4)
         try:
             per = max(2, samples // (members * d.im))
+            span = hours * 3600.0
             net.collect(d)
             taken = 1
-            while d.scalar("sumt") < 3600.0 or taken == 1:
-                d.substep()
-                if taken < per and d.scalar("sumt") >= taken * 3600.0 / per:
-                    net.collect(d)
-                    taken += 1
+            for h in range(hours):
+                if h:
+                    d.holland_forcinginput(wl.storm(h + 1))
+                    d.set_scalar("sumt", 0.0)
+                while d.scalar("sumt") < 3600.0:
+                    d.substep()
+                    t = h * 3600.0 + d.scalar("sumt")
+                    if taken < per and t >= taken * span / per:
+                        net.collect(d)
+                        taken += 1
         finally:
             d.close()
     n = net.info()["samples"]
@@ -190,8 +220,8 @@ def train_surrogate(members, samples, steps):
             "loss_first": first, "loss_last": last, "loss_heldout": held,
             "collect_s": round(t_collect, 1), "train_s": round(t_train, 1),
             "network": "3531-512-512-512-3528 GELU, BF16 weights/activations, FP32 accumulate",
-            "source": f"{members} hurricane members on 100x80 cells of 9 km, first hour; "
-                      f"the last member held out"}
+            "source": f"{members} hurricane members on 100x80 cells of 9 km, snapshots over "
+                      f"{hours} h; the last member held out"}
     log(f"bench: surrogate {n} samples ({t_collect:.1f} s), loss {first:.4f} -> {last:.4f}, "
         f"held-out {held:.4f} ({t_train:.1f} s)")
     return net, info
@@ -218,8 +248,7 @@ def chosen_path(members, steps, warmup, net, fields64, tile, graph=True):
         t0 = time.perf_counter()
         n1 = b.run_dtg(use_graph=graph)
         log(f"bench: c5 hour 1 of the wave in {time.perf_counter() - t0:.1f} s, substeps {n1}")
-        h32 = {k: doms[0].get(k) for k in ("ht", "mwp", "ustar")}
-        h32["us"] = np.hypot(doms[0].get("us")[0], doms[0].get("vs")[0])
+        h32 = hour_fields(doms[0])
         stats1 = hip.Surrogate.stats(doms[0], reset=True) if net is not None else None
         # hour 2: next snapshot, then warmup and `steps` timed substeps
         for wl, d in zip(wls, doms):
@@ -269,19 +298,7 @@ def chosen_path(members, steps, warmup, net, fields64, tile, graph=True):
             used = stats2["used"] / max(stats2["points"], 1)
             out["surrogate_stats"] = {"hour1": stats1, "timed": stats2, "used_fraction": used}
         # errors of the chosen path against FP64 after the first hour
-        e = {}
-        ref = fields64
-        e["hs_rmse_rel"] = float(np.sqrt(np.mean((h32["ht"] - ref["ht"]) ** 2))
-                                 / np.sqrt(np.mean(ref["ht"] ** 2)))
-        e["hs_max_rel"] = float(np.max(np.abs(h32["ht"] - ref["ht"]) / np.maximum(ref["ht"], 1e-3)))
-        e["tm_max_rel"] = float(np.max(np.abs(h32["mwp"] - ref["mwp"]) / np.maximum(ref["mwp"], 1e-3)))
-        e["ustar_max_rel"] = float(np.max(np.abs(h32["ustar"] - ref["ustar"])
-                                          / np.maximum(np.abs(ref["ustar"]), 1e-3)))
-        e["ustar_rmse_rel"] = float(np.sqrt(np.mean((h32["ustar"] - ref["ustar"]) ** 2))
-                                    / np.sqrt(np.mean(ref["ustar"] ** 2)))
-        e["stokes_surface_rmse_rel"] = float(np.sqrt(np.mean((h32["us"] - ref["us"]) ** 2))
-                                             / max(np.sqrt(np.mean(ref["us"] ** 2)), 1e-30))
-        e["nonfinite_points"] = int(np.sum(~np.isfinite(h32["ht"])))
+        e = field_errors(h32, fields64)
         out["errors_vs_fp64_after_1h"] = e
         log("bench: c5 errors vs fp64 after 1 h " + json.dumps({k: round(v, 5) for k, v in e.items()}))
         return rate, kms, out
@@ -435,7 +452,7 @@ def counters_run(steps):
     member with the bench's tile, two plain substeps, then `steps`
     substeps with the surrogate attached, one at a time (no batch, no
     graph)."""
-    net, _ = train_surrogate(2, 20000, 100)
+    net, _ = train_surrogate(2, 20000, 100, hours=1)
     d = make_domain(configs.c5(member=0, hours=1), 4)
     try:
         d.set_tile(*C5_TILE32)
@@ -449,6 +466,77 @@ def counters_run(steps):
     finally:
         d.close()
         net.close()
+
+
+def ensemble_run(hours, members, sur_members, sur_samples, sur_steps, sur_hours):
+    """The hurricane ensemble simulated for `hours` hours: member 0 in FP64
+    (the plain path; its fields held on the host hour by hour, since it and
+    a wave of FP32 members do not fit in HBM together), then a wave of
+    `members` members on the chosen path (FP32, batched with graph replay,
+    the surrogate attached).  Both take the storm's winds generated on the
+    device each hour.  Per hour: member 0's errors against FP64, the
+    surrogate's share of point-substeps, the wave's bin-update rate."""
+    t_start = time.perf_counter()
+    wl0 = configs.c5(member=0, hours=hours)
+    ref = []
+    d = make_domain(wl0, 8)
+    try:
+        d.set_tile(*C5_TILE64)
+        for h in range(hours):
+            if h:
+                d.holland_forcinginput(wl0.storm(h + 1))
+            n = d.run_dtg()
+            ref.append(hour_fields(d))
+            log(f"bench: ensemble fp64 member 0 hour {h + 1}: {n} substeps "
+                f"({time.perf_counter() - t_start:.0f} s)")
+    finally:
+        d.close()
+    net, sur_info = train_surrogate(sur_members, sur_samples, sur_steps, sur_hours)
+    wls = [configs.c5(member=m, hours=hours) for m in range(members)]
+    doms = [make_domain(wl, 4) for wl in wls]
+    bins = [x.im * x.om * x.pm for x in doms]
+    b = None
+    per_hour = []
+    try:
+        pol = abi.make_surrogate_policy(audit_stride=16)
+        for x in doms:
+            x.set_tile(*C5_TILE32)
+            net.attach(x, pol)
+        b = hip.Batch(doms)
+        b.set_tile(*C5_TILE32)
+        for h in range(hours):
+            if h:
+                for wl, x in zip(wls, doms):
+                    x.holland_forcinginput(wl.storm(h + 1))
+            t0 = time.perf_counter()
+            nsub = b.run_dtg(use_graph=True)
+            wall = time.perf_counter() - t0
+            st = hip.Surrogate.stats(doms[0], reset=True)
+            upd = sum(n * bb for n, bb in zip(nsub, bins)) - (sum(bins) if h == 0 else 0)
+            row = {"hour": h + 1, "substeps": [int(n) for n in nsub], "wall_s": round(wall, 2),
+                   "rate": upd / wall, "surrogate_used": st["used"] / max(st["points"], 1),
+                   "fallback_audit": st["fallback_audit"] / max(st["points"], 1),
+                   "audit_error": st["audit_error"]}
+            row.update(field_errors(hour_fields(doms[0]), ref[h]))
+            per_hour.append(row)
+            log(f"bench: ensemble wave hour {h + 1}: {row['rate'] / 1e9:.1f} G/s, used "
+                f"{row['surrogate_used']:.3f} (audit error {row['audit_error']:.3f}, audit "
+                f"fallbacks {row['fallback_audit']:.3f}), hs rmse {row['hs_rmse_rel']:.4f}, ustar rmse "
+                f"{row['ustar_rmse_rel']:.4f} ({time.perf_counter() - t_start:.0f} s)")
+    finally:
+        if b is not None:
+            b.close()
+        for x in doms:
+            x.close()
+        net.close()
+    worst = {k: max(r[k] for r in per_hour) for k in
+             ("hs_rmse_rel", "ustar_rmse_rel", "stokes_surface_rmse_rel", "nonfinite_points")}
+    return {"mode": "ensemble", "workload": "c5: hurricane ensemble, 1500x1200 at 2 km, om=49 "
+            "pm=72, device-generated Holland winds each hour", "hours": hours,
+            "members": members, "surrogate": sur_info, "per_hour": per_hour,
+            "worst_hour": worst,
+            "mean_rate": float(np.mean([r["rate"] for r in per_hour])),
+            "wall_s": round(time.perf_counter() - t_start, 1)}
 
 
 def counters_plain_run(steps):
@@ -475,6 +563,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=30, help="timed FP64 substeps")
     ap.add_argument("--sur-members", type=int, default=6)
     ap.add_argument("--sur-samples", type=int, default=300_000)
+    ap.add_argument("--sur-hours", type=int, default=1,
+                    help="hours of the reduced-grid storms the training samples span")
     ap.add_argument("--sur-steps", type=int, default=3000)
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
@@ -490,10 +580,19 @@ def main():
     ap.add_argument("--counters", action="store_true",
                     help="counter runs: one hurricane member with the surrogate (trained "
                          "briefly) through single-domain substeps, no batch")
+    ap.add_argument("--ensemble-hours", type=int, default=0,
+                    help="instead of the bench line: the hurricane simulated for this many "
+                         "hours (member 0 in FP64, then a wave on the chosen path), errors "
+                         "per hour")
     ap.add_argument("--counters-plain", action="store_true",
                     help="counter runs: one FP64 hurricane member (the plain path) through "
                          "single-domain substeps")
     args = ap.parse_args()
+    if args.ensemble_hours > 0:
+        print(json.dumps(ensemble_run(args.ensemble_hours, args.members, args.sur_members,
+                                      args.sur_samples, args.sur_steps, args.sur_hours)),
+              flush=True)
+        return
     if args.counters_plain:
         counters_plain_run(args.steps)
         return
@@ -523,7 +622,8 @@ def main():
     net, sur_info = None, None
     if not args.no_surrogate:
         log("bench: surrogate training")
-        net, sur_info = train_surrogate(args.sur_members, args.sur_samples, args.sur_steps)
+        net, sur_info = train_surrogate(args.sur_members, args.sur_samples, args.sur_steps,
+                                        args.sur_hours)
     if f64 is None:  # profiling: compare with the FP32 path itself
         f64 = {k: np.zeros(1) + 1 for k in ("ht", "mwp", "ustar", "us")}
     log("bench: c5 chosen path")

@@ -396,7 +396,11 @@ typedef struct umwm_surrogate_config {
  * dm^2 leaves the bounds the moment prefix sets, [1, (1 + 2 mss_fac m0)^2]
  * widened by a quarter (m0 the zeroth moment below the bin's frequency),
  * or where an audit finds its energy-weighted error above audit_tol.
- * Where the network serves, its dm^2 is clamped to those bounds. */
+ * The audits of one substep together also gate the whole domain: when
+ * their summed energy-weighted error exceeds audit_tol, every point takes
+ * the plain source terms in the next substep (the audits go on, and reopen
+ * the gate once the error is back within audit_tol).  Where the network
+ * serves, its dm^2 is clamped to those bounds. */
 typedef struct umwm_surrogate_policy {
   int32_t audit_stride;
   int32_t reserved0;

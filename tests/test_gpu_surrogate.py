@@ -13,7 +13,9 @@
     network with its last layer zeroed), both against the FP64 oracle;
   * the fallbacks: an untrained network under a full audit falls back
     everywhere (and the run equals the plain FP32 one), and inputs outside
-    the training range fall back on the range check.
+    the training range fall back on the range check;
+  * the domain-wide audit gate: closed by an audit above the tolerance, it
+    sends every point to the plain rates in the next substeps.
 """
 import os
 import time
@@ -349,3 +351,43 @@ def test_out_of_range_inputs_fall_back(trained):
         assert st["points"] > 0 and st["fallback_range"] == st["points"]
     finally:
         d.close()
+
+
+def test_audit_gate_closes_and_reopens(trained):
+    """The domain-wide gate: a quarter of the workgroups audited each
+    substep.  With a zero audit tolerance the first substep's audit closes
+    the gate and from the second substep on no point takes the network,
+    audited or not; with the tolerance above every error it never closes,
+    and the network serves as without the gate."""
+    net, _ = trained
+    wl = configs.c1_seeded(HELDOUT_SEED)
+    a, b = fp32_domain(wl), fp32_domain(wl)
+    try:
+        a.substep()
+        b.substep()
+        net.attach(a, abi.make_surrogate_policy(audit_stride=4, audit_tol=0.0, resid_tol=1e30))
+        net.attach(b, abi.make_surrogate_policy(audit_stride=4, audit_tol=1e30, resid_tol=1e30))
+        a.substep()
+        b.substep()
+        s_a = hip.Surrogate.stats(a, reset=True)
+        s_b = hip.Surrogate.stats(b, reset=True)
+        # first substep with the surrogate: the gate is open in both, the
+        # zero tolerance rejects only the audited points of `a`
+        assert s_a["audited"] > 0 and s_a["used"] > 0
+        assert s_a["used"] + s_a["fallback_audit"] == s_b["used"]
+        for _ in range(3):
+            a.substep()
+            b.substep()
+        s_a = hip.Surrogate.stats(a)
+        s_b = hip.Surrogate.stats(b)
+        assert s_a["used"] == 0
+        assert s_a["fallback_audit"] + s_a["fallback_range"] == s_a["points"]
+        assert s_b["used"] >= 0.9 * (s_b["points"] - s_b["fallback_range"]) > 0
+        # reattaching resets the gate
+        net.attach(a, abi.make_surrogate_policy(audit_stride=4, audit_tol=1e30, resid_tol=1e30))
+        hip.Surrogate.stats(a, reset=True)
+        a.substep()
+        assert hip.Surrogate.stats(a)["used"] > 0
+    finally:
+        a.close()
+        b.close()

@@ -2119,6 +2119,14 @@ int attach_impl(DomainImpl<float>& D, SurrogateImpl* S, const umwm_surrogate_pol
   D.sur_zmax = p.zmax;
   D.sur = S;
   UMWM_HIP_TRY(hipMemcpyAsync(D.d, &D.h, sizeof(DomainDev<float>), hipMemcpyHostToDevice, D.stream));
+  // the audit gate starts open, with empty sums
+  struct {
+    double anum, aden;
+    int32_t closed, pad;
+  } gate{0.0, 0.0, 0, 0};
+  static_assert(sizeof(gate) == sizeof(Scalars) - offsetof(Scalars, sur_anum), "gate fields");
+  UMWM_HIP_TRY(hipMemcpyAsync(reinterpret_cast<char*>(D.h.sc) + offsetof(Scalars, sur_anum), &gate,
+                              sizeof(gate), hipMemcpyHostToDevice, D.stream));
   UMWM_HIP_TRY(hipStreamSynchronize(D.stream));
   return 0;
 }

@@ -50,6 +50,11 @@ struct Scalars {
   // the rates kernel's running grid min of pts and max of the rotation
   // rate as order-preserving keys (0 = none yet; its last block clears them)
   unsigned long long pts_key, rot_key;
+  // surrogate (FP32): this substep's sampled audit sums (error and weight)
+  // and the domain's verdict on the previous substep's audit (0: the
+  // network may serve, 1: every point takes the plain rates)
+  double sur_anum, sur_aden;
+  int32_t sur_closed, sur_pad;
 };
 
 // Everything a kernel needs to know about one domain.

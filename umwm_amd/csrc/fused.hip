@@ -507,15 +507,21 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
     // dissipation error exceeds the audit tolerance, falls back to the plain
     // rates, which these waves always evaluate.  All waves of a workgroup
     // hold the same 64 points, so these choices agree across waves.
+    // The domain-wide gate: when the previous substep's audited workgroups
+    // together found the network's dissipation error above the audit
+    // tolerance, every point takes the plain rates this substep; the
+    // audited workgroups still evaluate the network, so that the audit can
+    // open the gate again.
     [[maybe_unused]] int sok = 0;
-    [[maybe_unused]] bool aud = false;
+    [[maybe_unused]] bool aud = false, gate = false;
     bool do_sur = false;
     [[maybe_unused]] const uint16_t* const rsur = d.sur_r;
     if constexpr (SUR) {
       if (rsur) {
         sok = valid ? pload(d.sur_ok, ldi, j) : 0;
         aud = d.sur_audit_stride > 0 && ((tk.local + nsub0) % d.sur_audit_stride) == 0;
-        do_sur = __any(sok != 0);
+        gate = sc->sur_closed == 0;
+        do_sur = __any(sok != 0) && (gate || aud);
       }
     }
     T rmax = 0;
@@ -746,7 +752,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
         // written as negated <= so that NaN sums count as failures
         const bool rfail = !(sneg <= (double)d.sur_resid_tol * stot);
         const bool afail = aud && !(snum <= (double)d.sur_audit_tol * sden);
-        use = valid && sok && !rfail && !afail;
+        use = valid && sok && !rfail && !afail && gate;
         if (w == 0) {
           if (valid) pstore(d.sur_use, ldi, j, (int32_t)(use ? 1 : 0));
           // counts as 64-bit integer atomics (ballot counts), the audit sums
@@ -757,7 +763,7 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
                                           use,
                                           valid && !sok,
                                           valid && sok && rfail,
-                                          valid && sok && !rfail && afail,
+                                          valid && sok && !rfail && (afail || !gate),
                                           valid && aud};
           auto* ctr = reinterpret_cast<unsigned long long*>(d.sur_stats);
 #pragma unroll
@@ -770,6 +776,10 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
             if (lane == 0) {
               atomicAdd(d.sur_stats + SS_AUDIT_NUM, a);
               atomicAdd(d.sur_stats + SS_AUDIT_DEN, b);
+              // this substep's sums for the gate (thread 0 of the block,
+              // which waits for its atomics before the arrival count)
+              atomicAdd(&sc->sur_anum, a);
+              atomicAdd(&sc->sur_aden, b);
             }
           }
         }
@@ -843,6 +853,17 @@ __global__ __launch_bounds__(512) void k_rates(const DomainDev<T>* __restrict__ 
       sc->dtr = fmin(sc->dtr_temp, dts);
       sc->nsub = sc->nsub + 1;
       sc->last = sc->sumt >= d.dtg ? 1 : 0;
+      if constexpr (SUR) {
+        if (d.sur_r) {
+          // the gate for the next substep: closed while the sampled error
+          // exceeds the tolerance; a substep with no audited point keeps it
+          const double an = __longlong_as_double(
+              (long long)atomicExch(reinterpret_cast<unsigned long long*>(&sc->sur_anum), 0ull));
+          const double ad = __longlong_as_double(
+              (long long)atomicExch(reinterpret_cast<unsigned long long*>(&sc->sur_aden), 0ull));
+          if (ad > 0) sc->sur_closed = an <= (double)d.sur_audit_tol * ad ? 0 : 1;
+        }
+      }
     }
     sc->active = active ? 1 : 0;
     sc->counter = 0;
