@@ -635,10 +635,21 @@ template <typename T>
 __global__ void k_hs_ice(DomainDev<T> d) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= d.im) return;
-  double h = 0;
-  for (int p = 0; p < d.pm; ++p)
-    for (int o = 0; o < d.om; ++o)
-      h += (double)d.e[((size_t)o * d.pm + p) * d.ldi + j] * (double)d.kdk[(size_t)o * d.ldi + j];
+  // four independent partial sums (o mod 4): one serial chain of FP64
+  // adds per point made the kernel latency-bound
+  double h4[4] = {0, 0, 0, 0};
+  auto term = [&](int o, int p) {
+    return (double)d.e[((size_t)o * d.pm + p) * d.ldi + j] * (double)d.kdk[(size_t)o * d.ldi + j];
+  };
+  for (int p = 0; p < d.pm; ++p) {
+    int o = 0;
+    for (; o + 3 < d.om; o += 4) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) h4[r] += term(o + r, p);
+    }
+    for (; o < d.om; ++o) h4[0] += term(o, p);
+  }
+  const double h = (h4[0] + h4[1]) + (h4[2] + h4[3]);
   d.hs_ice[j] = 4 * sqrt(h * d.dth);
 }
 
@@ -665,13 +676,26 @@ __global__ __launch_bounds__(64) void k_stokes(DomainDev<T> d, StokesDev S) {
   const size_t ldi = d.ldi;
   const double dj = d.depth[j];
   for (int o = 0; o < d.om; ++o) {
-    double a = 0, b = 0;
+    // four independent partial sums over the directions: one serial chain
+    // of FP64 adds per point made the kernel latency-bound
+    double a4[4] = {0, 0, 0, 0}, b4[4] = {0, 0, 0, 0};
     const T* eo = d.e + (size_t)o * d.pm * ldi + j;
-    for (int p = 0; p < d.pm; ++p) {
-      const double e = eo[(size_t)p * ldi];
-      a += e * (double)d.cth[p];
-      b += e * (double)d.sth[p];
+    int p = 0;
+    for (; p + 3 < d.pm; p += 4) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double e = eo[(size_t)(p + r) * ldi];
+        a4[r] += e * (double)d.cth[p + r];
+        b4[r] += e * (double)d.sth[p + r];
+      }
     }
+    for (; p < d.pm; ++p) {
+      const double e = eo[(size_t)p * ldi];
+      a4[0] += e * (double)d.cth[p];
+      b4[0] += e * (double)d.sth[p];
+    }
+    const double a = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+    const double b = (b4[0] + b4[1]) + (b4[2] + b4[3]);
     const size_t tq = (size_t)o * ldi + j;
     const double k = d.k[tq], kd = k * dj, dwn = d.dwn[tq];
     const double fo = kTwoPiD * (double)d.f[o];
